@@ -1,0 +1,169 @@
+/*
+ * tokura_blas.h — C ABI of the MI355X-native batched complex eigenvalue solver.
+ *
+ * Drop-in for the reference library's public interface
+ * (tokura_zgeev_batched_library/include/tokura_blas.h:129-151, documented in
+ * README.md:25-79).  All four reference entry points keep their names,
+ * argument order, argument meaning, memory layouts and return codes:
+ *
+ *   matrices  A[k*n*n + r + c*n]  (column-major, one matrix after another)
+ *   output    eigenvalues[i + j*n]
+ *   flag[j]   0 when the eigenvalues of matrix j converged, non-zero otherwise
+ *   stream    NULL selects the default stream; nothing synchronises the host
+ *
+ * Types: hipDoubleComplex has the layout of the reference's cuDoubleComplex
+ * ({double x, y}, 16-byte aligned); hipStream_t replaces cudaStream_t.
+ *
+ * The library allocates no device memory on the forward paths: callers own
+ * A, eigenvalues, work and flag.  A is read, not clobbered (the reference
+ * clobbers it; callers that rely on README.md:69 "do not use A" are unaffected).
+ *
+ * Flag codes (the reference defines only 0 / non-zero, README.md:72):
+ *   0  converged
+ *   1  iteration cap reached (30*max(10,n) QR sweeps per eigenvalue, LAPACK's
+ *      cap; the reference uses n^4 total)
+ *   2  non-finite entry in the input matrix (eigenvalues are NaN)
+ *
+ * One handle must not be used by two host threads at once (the reference
+ * keeps per-call state in the handle, tokura_zgeev_batched.cu:224-235).
+ */
+#ifndef TOKURA_BLAS_MI355X_H
+#define TOKURA_BLAS_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+#include <hip/hip_complex.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE 64
+
+/* Opaque handle.  The reference exposes its struct (tokura_blas.h:59-100) but
+ * no caller outside its build-time tuner reads a field. */
+typedef struct tokurablas_t tokurablas_t;
+
+/* ---------------------------------------------------------------------------
+ * Reference entry points
+ * ------------------------------------------------------------------------- */
+
+/* Replaces tokuraCreate (tokura_zgeev_batched.cu:64-160).  Allocates the host
+ * handle, queries the current HIP device and loads the autotuned launch table
+ * for it.  Returns 0 on success, -1 when the handle cannot be allocated, 1
+ * when no HIP device can be queried (as the reference returns 1 when
+ * cudaGetDeviceProperties fails; *handle is still set and must be destroyed). */
+int tokuraCreate(tokurablas_t **handle);
+
+/* Replaces tokuraDestroy (tokura_zgeev_batched.cu:161-190).  NULL is accepted. */
+int tokuraDestroy(tokurablas_t *handle);
+
+/* Replaces tokura_get_zgeeveigenvaluesgetworspacesize
+ * (tokura_zgeev_batched.cu:192-210): 16*(n*n*batch + n*batch) bytes,
+ * (size_t)-1 when n is outside [1,64], (size_t)-2 when batch_count <= 0. */
+size_t tokura_get_zgeeveigenvaluesgetworspacesize(int n, int batch_count);
+
+/* Replaces tokura_zgeev_batched_gpu (tokura_zgeev_batched.cu:212-285).
+ * Eigenvalues of batch_count n x n complex matrices, n in [1,64].
+ * Returns 0 on success; -1 handle NULL, -2 n out of range, -3 A NULL,
+ * -4 eigenvalues NULL, -5 work NULL, -6 batch_count <= 0, -7 flag NULL (the
+ * reference's validation order); 1 Hessenberg stage launch failure, 2 QR stage
+ * launch failure.  Asynchronous on `stream`. */
+int tokura_zgeev_batched_gpu(tokurablas_t *handle, int n, int batch_count,
+                             hipDoubleComplex *A, hipDoubleComplex *eigenvalues,
+                             hipDoubleComplex *work, char *flag, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Extensions (no counterpart in the reference)
+ * ------------------------------------------------------------------------- */
+
+/* Debug knob: cap on QR sweeps per eigenvalue (< 0 restores LAPACK's
+ * 30*max(10,n)).  Lets tests force flag 1 from the host. */
+int tokuraSetMaxSweeps(tokurablas_t *handle, int max_sweeps);
+
+/* Launch configuration chosen for size n (variant id, lanes per matrix,
+ * matrices per workgroup); for diagnostics and the autotuner's tests. */
+int tokuraGetLaunchConfig(tokurablas_t *handle, int n, int *variant, int *lanes_per_matrix,
+                          int *matrices_per_block);
+
+/* Stage timing for profiling: while enabled, every same-size call records
+ * HIP events on its stream before the first kernel, between the Hessenberg
+ * and the QR kernel (split variants) and after the last kernel.
+ * tokuraGetStageTimes waits for those events and returns the number of timed
+ * calls and the summed milliseconds of each stage (a fused variant reports
+ * its whole time as stage 2), then clears the record. */
+int tokuraSetStageTiming(tokurablas_t *handle, int enable);
+int tokuraGetStageTimes(tokurablas_t *handle, int *calls, float *hessenberg_ms, float *qr_ms);
+
+/* Runtime autotuner (replaces the reference's compile-time tuner,
+ * TOKURA_ZHSEQR_BATCHED_GEHRD_TUNING_main.cu:89-242, which rebuilds the .so).
+ * For every n in [n_lo, n_hi] it times each eligible kernel variant on
+ * `batch_count` seeded random matrices on `stream` (device buffers are
+ * allocated and freed inside this call), keeps the fastest in the handle and,
+ * when cache_path is non-NULL, writes the table there.  Returns 0 on success. */
+int tokuraAutotune(tokurablas_t *handle, int n_lo, int n_hi, int batch_count,
+                   const char *cache_path, hipStream_t stream);
+
+/* Loads a table written by tokuraAutotune (entries whose device key — arch
+ * and CU count — differ are ignored).  Returns the number of entries read or
+ * a negative value on error.  tokuraCreate already loads the file named by
+ * the environment variable TOKURA_TUNE_CACHE when it is set. */
+int tokuraLoadTuning(tokurablas_t *handle, const char *cache_path);
+
+/* Sets the variant used for size n by hand (a variant id as reported by
+ * tokuraGetLaunchConfig).  Returns 0 or -2 for an invalid n / variant. */
+int tokuraSetVariant(tokurablas_t *handle, int n, int variant);
+
+/* ---- ragged batches: n varies per matrix --------------------------------
+ * Matrix b is stored column-major at A + offsets[b] and is n_b x n_b with
+ * n_b*n_b = offsets[b+1] - offsets[b], 1 <= n_b <= 64 (offsets are int64,
+ * counted in complex elements, batch_count+1 entries, non-decreasing; matrices
+ * need not be contiguous with one another).  Eigenvalues of matrix b are
+ * written to eigenvalues[e_b + i], e_b = n_0 + ... + n_{b-1}; flag[b] as above.
+ *
+ * tokura_zgeev_ragged_prepare reads the host copy of offsets, builds the
+ * schedule (cost-descending work queue, eigenvalue offsets) and copies it
+ * into `work` with an asynchronous copy on `stream`; it must run before the
+ * solves that use that `work`, and the schedule stays valid for any number of
+ * solves of batches with the same offsets.
+ * tokura_zgeev_ragged_gpu then runs one persistent kernel that pulls matrices
+ * from the queue; tokura_zgeev_ragged_graph does the same through a hipGraph
+ * that the handle captures on first use and replays afterwards (re-captured
+ * when the pointers or batch change). */
+size_t tokura_get_zgeev_ragged_worksize(int batch_count);
+int tokura_zgeev_ragged_prepare(tokurablas_t *handle, int batch_count, const int64_t *offsets_host,
+                                void *work, size_t work_bytes, hipStream_t stream);
+int tokura_zgeev_ragged_gpu(tokurablas_t *handle, int batch_count, const hipDoubleComplex *A,
+                            hipDoubleComplex *eigenvalues, void *work, char *flag,
+                            hipStream_t stream);
+int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, const hipDoubleComplex *A,
+                              hipDoubleComplex *eigenvalues, void *work, char *flag,
+                              hipStream_t stream);
+
+/* ---- differentiation ----------------------------------------------------
+ * Forward with Schur vectors: besides eigenvalues[i + j*n] and flag, writes
+ * the Schur factorisation A_j = U_j T_j U_j^H of every matrix into `schur`
+ * (T_j then U_j, each n*n column-major, 2*n*n complex per matrix; size from
+ * tokura_get_zgeev_schur_worksize).  Eigenvalue i is T_j(i,i).
+ *
+ * Backward: for L = Re sum_ij conj(gbar[i + j*n]) * eigenvalues[i + j*n]
+ * writes dA[j] = dL/dA_j = sum_i gbar_ij * y_i x_i^H / conj(y_i^H x_i)
+ * (the convention of torch.linalg.eigvals' autograd), with x_i, y_i the right
+ * and left eigenvectors, obtained by back-substitution on T_j and back-
+ * transformed by U_j.  When `schur` is NULL the Schur form is recomputed from
+ * A (memory-light mode, A must then be the unmodified input).  flag as in the
+ * forward pass; a matrix with flag != 0 gets dA = NaN. */
+size_t tokura_get_zgeev_schur_worksize(int n, int batch_count);
+int tokura_zgeev_batched_schur_gpu(tokurablas_t *handle, int n, int batch_count,
+                                   const hipDoubleComplex *A, hipDoubleComplex *eigenvalues,
+                                   hipDoubleComplex *schur, char *flag, hipStream_t stream);
+int tokura_zgeev_batched_backward_gpu(tokurablas_t *handle, int n, int batch_count,
+                                      const hipDoubleComplex *A, const hipDoubleComplex *schur,
+                                      const hipDoubleComplex *gbar, hipDoubleComplex *dA,
+                                      char *flag, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TOKURA_BLAS_MI355X_H */

@@ -1,0 +1,215 @@
+"""ORACLE — test infrastructure only.
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this package, and only as the checker.  The shipped
+library (``tokura_zgeev_batched_amd``) never imports, links or calls it.
+
+Contents
+--------
+* ``method_zgeev``  — the C restatement of the GPU method (zgehd2 + zlahqr,
+  eigenvalues only, no balancing), ``zgeev_method.c``.
+* ``lapack_zgeev``  — host LAPACK zgeev (balanced, LAPACK default), the
+  reference harness's oracle (stream_test_with_MKL/get_eig_MKL.c:26-51).
+* ``lapack_hrd_hseqr`` — LAPACK zgehrd + zhseqr('E','N') without balancing.
+* ``lapack_rconde`` — LAPACK zgeevx eigenvalue condition numbers (balanc='N').
+* ``method_schur`` / ``method_adjoint`` — Schur form and dL/dA from the same
+  method, used to check the adjoint kernel.
+* ``match_max_relerr`` — optimal-assignment eigenvalue matching; the
+  reference's harness sorts lexicographically instead
+  (stream_test_with_MKL/main.cu:16-181), which mis-pairs close eigenvalues.
+
+Conventions: a batch of matrices is a numpy complex128 array ``M`` of shape
+(batch, n, n) with ``M[b, r, c]`` = entry (r, c) of matrix b.  The C side
+takes column-major matrices, so the helpers transpose the last two axes.
+"""
+from __future__ import annotations
+
+import ctypes
+import glob
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build(force: bool = False) -> str:
+    path = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(path):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE],
+                       check=True)
+    return path
+
+
+def _lapack_path() -> str:
+    import scipy  # noqa: F401  (locates the OpenBLAS that SciPy ships)
+    base = os.path.dirname(os.path.dirname(scipy.__file__))
+    cands = sorted(glob.glob(os.path.join(base, "scipy.libs", "libscipy_openblas*.so")))
+    if not cands:
+        raise RuntimeError("SciPy's OpenBLAS (scipy.libs/libscipy_openblas*.so) not found")
+    return cands[0]
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(_HERE, "liboracle.so")
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    L.oracle_zgeev_batched.argtypes = [I, I, P, P, P, P, I]
+    L.oracle_zgeev_batched.restype = None
+    L.oracle_zschur.argtypes = [I, P, P, P, P]
+    L.oracle_zschur.restype = I
+    L.oracle_zgeev_adjoint.argtypes = [I, P, P, P, P]
+    L.oracle_zgeev_adjoint.restype = I
+    L.oracle_zgehd2.argtypes = [I, P, I, P, P]
+    L.oracle_zgehd2.restype = None
+    L.oracle_ztrevc.argtypes = [I, P, P, P]
+    L.oracle_ztrevc.restype = None
+    L.lapack_open.argtypes = [ctypes.c_char_p]
+    L.lapack_open.restype = I
+    L.lapack_max_threads.restype = I
+    for name in ("lapack_zgeev_batched", "lapack_zhrd_hseqr_batched"):
+        f = getattr(L, name)
+        f.argtypes = [I, I, P, P, P, I]
+        f.restype = I
+    L.lapack_zgeevx_rcond.argtypes = [I, I, P, P, P, P, I]
+    L.lapack_zgeevx_rcond.restype = I
+    rc = L.lapack_open(_lapack_path().encode())
+    if rc != 0:
+        raise RuntimeError(f"lapack_open failed ({rc})")
+    _LIB = L
+    return L
+
+
+def _colmajor(M: np.ndarray) -> np.ndarray:
+    M = np.asarray(M, dtype=np.complex128)
+    if M.ndim == 2:
+        M = M[None]
+    return np.ascontiguousarray(M.transpose(0, 2, 1))
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def method_zgeev(M, itmax: int = -1, with_sweeps: bool = False):
+    """Eigenvalues by the restated GPU method; returns (eig[b, i], flag[b])."""
+    A = _colmajor(M)
+    b, n, _ = A.shape
+    eig = np.zeros((b, n), np.complex128)
+    flag = np.zeros(b, np.int8)
+    sweeps = np.zeros(b, np.int32)
+    lib().oracle_zgeev_batched(n, b, _p(A), _p(eig), _p(flag), _p(sweeps), itmax)
+    if with_sweeps:
+        return eig, flag, sweeps
+    return eig, flag
+
+
+def hessenberg(M):
+    """zgehd2 restatement on one matrix; returns H (n x n, zeros below sub-diagonal)."""
+    A = _colmajor(M)[0].copy()
+    n = A.shape[0]
+    lib().oracle_zgehd2(n, _p(A), n, None, None)
+    return A.T.copy()
+
+
+def method_schur(M):
+    """Full Schur form A = U T U^H by the restated method: (T, U, w, info)."""
+    A = _colmajor(M)[0]
+    n = A.shape[0]
+    T = np.zeros((n, n), np.complex128)
+    U = np.zeros((n, n), np.complex128)
+    w = np.zeros(n, np.complex128)
+    info = lib().oracle_zschur(n, _p(A), _p(T), _p(U), _p(w))
+    return T.T.copy(), U.T.copy(), w, info
+
+
+def method_adjoint(M, g):
+    """dL/dA for L = Re sum conj(g_i) lambda_i; g is in Schur-diagonal order.
+
+    Returns (G, w) with w the eigenvalues in that order."""
+    A = _colmajor(M)[0]
+    n = A.shape[0]
+    g = np.ascontiguousarray(np.asarray(g, np.complex128))
+    G = np.zeros((n, n), np.complex128)
+    w = np.zeros(n, np.complex128)
+    lib().oracle_zgeev_adjoint(n, _p(A), _p(g), _p(G), _p(w))
+    return G.T.copy(), w
+
+
+def lapack_zgeev(M, nthreads: int = 0):
+    A = _colmajor(M)
+    b, n, _ = A.shape
+    eig = np.zeros((b, n), np.complex128)
+    info = np.zeros(b, np.int32)
+    lib().lapack_zgeev_batched(n, b, _p(A), _p(eig), _p(info), nthreads)
+    return eig, info
+
+
+def lapack_hrd_hseqr(M, nthreads: int = 0):
+    A = _colmajor(M)
+    b, n, _ = A.shape
+    eig = np.zeros((b, n), np.complex128)
+    info = np.zeros(b, np.int32)
+    lib().lapack_zhrd_hseqr_batched(n, b, _p(A), _p(eig), _p(info), nthreads)
+    return eig, info
+
+
+def lapack_rconde(M, nthreads: int = 0):
+    """(eigenvalues, reciprocal condition numbers) of the unbalanced matrices."""
+    A = _colmajor(M)
+    b, n, _ = A.shape
+    w = np.zeros((b, n), np.complex128)
+    rc = np.zeros((b, n), np.float64)
+    info = np.zeros(b, np.int32)
+    lib().lapack_zgeevx_rcond(n, b, _p(A), _p(w), _p(rc), _p(info), nthreads)
+    return w, rc, info
+
+
+def max_threads() -> int:
+    return int(lib().lapack_max_threads())
+
+
+def match(ref: np.ndarray, got: np.ndarray):
+    """Optimal assignment of got[i] to ref[i] (per row); returns the
+    permutation perm such that got[perm] pairs with ref."""
+    from scipy.optimize import linear_sum_assignment
+    cost = np.abs(ref[:, None] - got[None, :])
+    r, c = linear_sum_assignment(cost)
+    perm = np.empty_like(c)
+    perm[r] = c
+    return perm
+
+
+def relerr_matched(ref: np.ndarray, got: np.ndarray, scale: np.ndarray | None = None):
+    """Per-eigenvalue relative error after optimal matching, for a batch.
+
+    ref, got: (batch, n).  The error of eigenvalue i is |got - ref| / |ref|
+    (absolute where ref == 0, as stream_test_with_MKL/main.cu:112-137 does).
+    When ``scale`` (batch, n) is given the error is divided by it (used for
+    condition scaling).  Returns an array (batch, n)."""
+    ref = np.asarray(ref)
+    got = np.asarray(got)
+    out = np.zeros(ref.shape, np.float64)
+    for b in range(ref.shape[0]):
+        perm = match(ref[b], got[b])
+        g = got[b][perm]
+        den = np.abs(ref[b])
+        den = np.where(den == 0, 1.0, den)
+        e = np.abs(g - ref[b]) / den
+        if scale is not None:
+            e = e / scale[b]
+        out[b] = e
+    return out
+
+
+def max_relerr(ref, got) -> float:
+    e = relerr_matched(ref, got)
+    return float(e.max()) if e.size else 0.0

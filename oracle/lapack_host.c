@@ -1,0 +1,122 @@
+/*
+ * ORACLE — test infrastructure only (tests/, the smoke check and bench.py's
+ * CPU-baseline leg).  The shipped library never loads it.
+ *
+ * Host LAPACK driver: the reference's accuracy/throughput harness times
+ * LAPACKE_zgeev('N','N') in an OpenMP parallel-for over matrices with a
+ * sequential LAPACK (stream_test_with_MKL/get_eig_MKL.c:26-51,
+ * compile_mkl.sh:1).  MKL is not available here; the same routine is taken
+ * from the OpenBLAS build that SciPy ships (exports scipy_LAPACKE_*), opened
+ * with dlopen at run time from a path the caller supplies.
+ *
+ *   lapack_open            dlopen + resolve symbols, force 1 BLAS thread
+ *   lapack_zgeev_batched   zgeev (balanced, LAPACK default) per matrix
+ *   lapack_zhrd_hseqr_batched  zgehrd + zhseqr('E','N') without balancing:
+ *                          the exact method of the GPU path, run by LAPACK
+ *   lapack_zgeevx_rcond    eigenvalue condition numbers (RCONDE) of the
+ *                          unbalanced matrix, for condition-scaled tolerances
+ */
+#include <complex.h>
+#include <dlfcn.h>
+#include <omp.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double _Complex zc;
+typedef int (*zgeev_t)(int, char, char, int, zc *, int, zc *, zc *, int, zc *, int);
+typedef int (*zgehrd_t)(int, int, int, int, zc *, int, zc *);
+typedef int (*zhseqr_t)(int, char, char, int, int, int, zc *, int, zc *, zc *, int);
+typedef int (*zgeevx_t)(int, char, char, char, char, int, zc *, int, zc *, zc *, int, zc *, int,
+                        int *, int *, double *, double *, double *, double *);
+typedef void (*setthr_t)(int);
+
+static void *g_lib;
+static zgeev_t p_zgeev;
+static zgehrd_t p_zgehrd;
+static zhseqr_t p_zhseqr;
+static zgeevx_t p_zgeevx;
+
+#define COLMAJOR 102
+
+int lapack_open(const char *path)
+{
+    if (g_lib) return 0;
+    g_lib = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!g_lib) return -1;
+    p_zgeev = (zgeev_t)dlsym(g_lib, "scipy_LAPACKE_zgeev");
+    p_zgehrd = (zgehrd_t)dlsym(g_lib, "scipy_LAPACKE_zgehrd");
+    p_zhseqr = (zhseqr_t)dlsym(g_lib, "scipy_LAPACKE_zhseqr");
+    p_zgeevx = (zgeevx_t)dlsym(g_lib, "scipy_LAPACKE_zgeevx");
+    setthr_t st = (setthr_t)dlsym(g_lib, "scipy_openblas_set_num_threads");
+    if (st) st(1);
+    if (!p_zgeev || !p_zgehrd || !p_zhseqr || !p_zgeevx) return -2;
+    return 0;
+}
+
+int lapack_max_threads(void) { return omp_get_max_threads(); }
+
+/* eig[i + j*n]; info[j] = LAPACK INFO.  nthreads <= 0 means all. */
+int lapack_zgeev_batched(int n, int batch, const zc *A, zc *eig, int *info, int nthreads)
+{
+    if (!p_zgeev) return -1;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    #pragma omp parallel num_threads(nthreads)
+    {
+        zc *a = (zc *)malloc(sizeof(zc) * (size_t)n * n);
+        #pragma omp for schedule(dynamic, 8)
+        for (int b = 0; b < batch; ++b) {
+            memcpy(a, A + (size_t)b * n * n, sizeof(zc) * (size_t)n * n);
+            int r = p_zgeev(COLMAJOR, 'N', 'N', n, a, n, eig + (size_t)b * n, NULL, 1, NULL, 1);
+            if (info) info[b] = r;
+        }
+        free(a);
+    }
+    return 0;
+}
+
+int lapack_zhrd_hseqr_batched(int n, int batch, const zc *A, zc *eig, int *info, int nthreads)
+{
+    if (!p_zgehrd) return -1;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    #pragma omp parallel num_threads(nthreads)
+    {
+        zc *a = (zc *)malloc(sizeof(zc) * (size_t)n * n);
+        zc *tau = (zc *)malloc(sizeof(zc) * (size_t)(n > 1 ? n : 1));
+        zc zdummy = 0;
+        #pragma omp for schedule(dynamic, 8)
+        for (int b = 0; b < batch; ++b) {
+            memcpy(a, A + (size_t)b * n * n, sizeof(zc) * (size_t)n * n);
+            int r = p_zgehrd(COLMAJOR, n, 1, n, a, n, tau);
+            if (r == 0) r = p_zhseqr(COLMAJOR, 'E', 'N', n, 1, n, a, n, eig + (size_t)b * n, &zdummy, 1);
+            if (info) info[b] = r;
+        }
+        free(a); free(tau);
+    }
+    return 0;
+}
+
+/* RCONDE of the unbalanced matrix (balanc='N'), eigenvalues in w. */
+int lapack_zgeevx_rcond(int n, int batch, const zc *A, zc *w, double *rconde, int *info, int nthreads)
+{
+    if (!p_zgeevx) return -1;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    #pragma omp parallel num_threads(nthreads)
+    {
+        size_t nn = (size_t)n * n;
+        zc *a = (zc *)malloc(sizeof(zc) * nn);
+        zc *vl = (zc *)malloc(sizeof(zc) * nn), *vr = (zc *)malloc(sizeof(zc) * nn);
+        double *scale = (double *)malloc(sizeof(double) * (size_t)n);
+        double *rcondv = (double *)malloc(sizeof(double) * (size_t)n);
+        #pragma omp for schedule(dynamic, 8)
+        for (int b = 0; b < batch; ++b) {
+            memcpy(a, A + (size_t)b * nn, sizeof(zc) * nn);
+            int ilo, ihi;
+            double abnrm;
+            int r = p_zgeevx(COLMAJOR, 'N', 'V', 'V', 'E', n, a, n, w + (size_t)b * n, vl, n, vr, n,
+                             &ilo, &ihi, scale, &abnrm, rconde + (size_t)b * n, rcondv);
+            if (info) info[b] = r;
+        }
+        free(a); free(vl); free(vr); free(scale); free(rcondv);
+    }
+    return 0;
+}

@@ -1,0 +1,104 @@
+"""CPU checks of the C ABI boundary (no compute call needs a GPU here).
+
+Mirrors the reference's contract: names and validation order of
+tokura_zgeev_batched_library/src/tokura_zgeev_batched.cu:192-264 and the
+prototypes of tokura_zgeev_batched_library/include/tokura_blas.h:129-151.
+"""
+import ctypes
+import os
+import re
+
+import pytest
+
+import tokura_zgeev_batched_amd as tk
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "tokura_blas.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(tokura\w+)\s*\(", src)))
+
+
+def test_library_exports_every_header_symbol():
+    L = tk.lib()
+    names = header_functions()
+    assert "tokura_zgeev_batched_gpu" in names and "tokuraCreate" in names
+    for name in names:
+        assert hasattr(L, name), name
+
+
+def test_reference_entry_points_present():
+    for name in ("tokuraCreate", "tokuraDestroy", "tokura_get_zgeeveigenvaluesgetworspacesize",
+                 "tokura_zgeev_batched_gpu"):
+        assert name in header_functions()
+
+
+def test_workspace_size_formula_and_errors():
+    f = tk.lib().tokura_get_zgeeveigenvaluesgetworspacesize
+    assert f(16, 10000) == 16 * (16 * 16 * 10000 + 16 * 10000) == 43_520_000
+    assert f(64, 100000) == 6_656_000_000
+    M = (1 << 64) - 1
+    assert f(0, 5) == M and f(65, 5) == M            # (size_t)-1
+    assert f(4, 0) == M - 1 and f(4, -3) == M - 1     # (size_t)-2
+
+
+def make_handle():
+    h = ctypes.c_void_p()
+    rc = tk.lib().tokuraCreate(ctypes.byref(h))
+    assert rc in (0, 1)        # 1: no HIP device in this container
+    assert h.value
+    return h
+
+
+def test_validation_codes_in_reference_order():
+    L = tk.lib()
+    h = make_handle()
+    P = ctypes.c_void_p
+    a, e, w, f = P(0x1000), P(0x2000), P(0x3000), P(0x4000)
+    s = P(0)
+    assert L.tokura_zgeev_batched_gpu(None, 4, 1, a, e, w, f, s) == -1
+    assert L.tokura_zgeev_batched_gpu(h, 0, 1, a, e, w, f, s) == -2
+    assert L.tokura_zgeev_batched_gpu(h, 65, 1, a, e, w, f, s) == -2
+    assert L.tokura_zgeev_batched_gpu(h, 4, 1, None, e, w, f, s) == -3
+    assert L.tokura_zgeev_batched_gpu(h, 4, 1, a, None, w, f, s) == -4
+    assert L.tokura_zgeev_batched_gpu(h, 4, 1, a, e, None, f, s) == -5
+    assert L.tokura_zgeev_batched_gpu(h, 4, 0, a, e, w, f, s) == -6
+    assert L.tokura_zgeev_batched_gpu(h, 4, 1, a, e, w, None, s) == -7
+    # order: n is checked before A, A before batch
+    assert L.tokura_zgeev_batched_gpu(h, 0, 0, None, None, None, None, s) == -2
+    assert L.tokura_zgeev_batched_gpu(h, 4, 0, None, e, w, f, s) == -3
+    assert L.tokuraDestroy(h) == 0
+    assert L.tokuraDestroy(None) == 0
+
+
+def test_handle_controls():
+    L = tk.lib()
+    h = make_handle()
+    v, lpm, mpb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    for n in (1, 4, 8, 16, 32, 48, 64):
+        assert L.tokuraGetLaunchConfig(h, n, ctypes.byref(v), ctypes.byref(lpm), ctypes.byref(mpb)) == 0
+        assert lpm.value * mpb.value == 64
+    assert L.tokuraGetLaunchConfig(h, 65, None, None, None) == -2
+    assert L.tokuraSetMaxSweeps(h, 3) == 0
+    assert L.tokuraSetVariant(h, 64, 0) == -2          # variant 0 holds n <= 4 only
+    assert L.tokuraSetVariant(h, 4, 0) == 0
+    assert L.tokuraDestroy(h) == 0
+
+
+def test_tuning_cache_roundtrip(tmp_path):
+    L = tk.lib()
+    h = make_handle()
+    # the CPU handle has an empty arch key; entries for other devices are ignored
+    p = tmp_path / "tune.txt"
+    p.write_text("# arch cus n variant\ngfx950 256 16 5\n 0 16 4\n")
+    assert L.tokuraLoadTuning(h, str(p).encode()) == 0
+    assert L.tokuraLoadTuning(h, b"/nonexistent/file") < 0
+    L.tokuraDestroy(h)
+
+
+def test_python_layer_refuses_host_tensors():
+    torch = pytest.importorskip("torch")
+    with pytest.raises(tk.TokuraError):
+        tk._dev_check(torch.zeros(2, 2, dtype=torch.complex128), "M", torch.complex128)

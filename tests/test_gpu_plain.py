@@ -1,0 +1,187 @@
+"""GPU parity of the same-size path (tokura_zgeev_batched_gpu) through the C ABI.
+
+Bar: max relative error <= 1e-10 against host LAPACK zgeev after optimal
+eigenvalue matching (BASELINE target), and <= 1e-11 against the oracle's
+restatement of the same method (zgehd2 + zlahqr without balancing).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import tokura_zgeev_batched_amd as tk
+
+pytestmark = pytest.mark.gpu
+
+TOL_LAPACK = 1e-10
+TOL_METHOD = 1e-11
+
+
+def rand(rng, b, n):
+    return rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n))
+
+
+def gpu_eigs(M, handle):
+    print(f"[case] n={M.shape[-1]} batch={M.shape[0]}", flush=True)
+    eig, flag = tk.eigvals(torch.tensor(M, device="cuda"), handle)
+    torch.cuda.synchronize()
+    return eig.cpu().numpy(), flag.cpu().numpy()
+
+
+@pytest.mark.parametrize("n", list(range(1, 65)))
+def test_every_size_default_config(handle, n):
+    rng = np.random.default_rng(1000 + n)
+    M = rand(rng, 33, n)
+    e, flag = gpu_eigs(M, handle)
+    assert not flag.any()
+    ref, info = oracle.lapack_zgeev(M)
+    assert oracle.max_relerr(ref, e) <= TOL_LAPACK
+    me, mflag = oracle.method_zgeev(M)
+    assert oracle.max_relerr(me, e) <= TOL_METHOD
+
+
+def variants_for(n):
+    import ctypes
+    out = []
+    for v in range(64):
+        h = tk.Handle()
+        rc = tk.lib().tokuraSetVariant(h.ptr, n, v)
+        h.close()
+        if rc == 0:
+            out.append(v)
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 8, 9, 16, 17, 31, 32, 33, 48, 49, 64])
+def test_every_variant(n):
+    rng = np.random.default_rng(2000 + n)
+    M = rand(rng, 70, n)             # 70: a partly filled last workgroup for every variant
+    ref, _ = oracle.lapack_zgeev(M)
+    vs = []
+    for v in range(40):
+        h = tk.Handle()
+        if tk.lib().tokuraSetVariant(h.ptr, n, v) != 0:
+            h.close()
+            continue
+        vs.append(v)
+        print(f"[case] variant={v}", flush=True)
+        e, flag = gpu_eigs(M, h)
+        h.close()
+        assert not flag.any(), v
+        assert oracle.max_relerr(ref, e) <= TOL_LAPACK, v
+    assert vs
+
+
+def test_golden(handle):
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "zgeev_golden.npz"))
+    for n in g["sizes"]:
+        e, flag = gpu_eigs(g[f"A_{n}"], handle)
+        assert not flag.any()
+        assert oracle.max_relerr(g[f"eig_zgeev_{n}"], e) <= TOL_LAPACK
+
+
+def test_closed_forms(handle):
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 7, 32, 64):
+        U = np.triu(rand(rng, 1, n)[0])
+        e, flag = gpu_eigs(U[None], handle)
+        assert flag[0] == 0
+        np.testing.assert_allclose(np.sort_complex(e[0]), np.sort_complex(np.diag(U)), rtol=1e-13, atol=1e-13)
+        D = np.diag(np.diag(U))
+        e, _ = gpu_eigs(D[None], handle)
+        np.testing.assert_array_equal(e[0], np.diag(D))
+    for n in (3, 16, 64):
+        C = np.zeros((n, n), complex)
+        C[1:, :-1] = np.eye(n - 1)
+        C[0, -1] = 1.0
+        e, flag = gpu_eigs(C[None], handle)
+        assert flag[0] == 0
+        assert oracle.max_relerr(np.exp(2j * np.pi * np.arange(n) / n)[None], e) < 1e-12
+    X = rand(rng, 8, 40)
+    Hm = X + np.conj(X.transpose(0, 2, 1))
+    e, _ = gpu_eigs(Hm, handle)
+    assert oracle.max_relerr(np.linalg.eigvalsh(Hm).astype(complex), e) < 1e-12
+    Z = np.zeros((2, 9, 9), complex)
+    e, flag = gpu_eigs(Z, handle)
+    assert not flag.any() and np.all(e == 0)
+
+
+def test_scaled_inputs(handle):
+    rng = np.random.default_rng(4)
+    M = rand(rng, 16, 24)
+    base, _ = oracle.lapack_zgeev(M)
+    for s in (1e-120, 1e-30, 1e30, 1e120):
+        e, flag = gpu_eigs(M * s, handle)
+        assert not flag.any()
+        assert oracle.max_relerr(base * s, e) <= TOL_LAPACK, s
+
+
+def test_flags(handle):
+    rng = np.random.default_rng(5)
+    M = rand(rng, 4, 12)
+    M[2, 5, 1] = np.inf
+    M[3, 0, 0] = np.nan
+    e, flag = gpu_eigs(M, handle)
+    assert list(flag) == [0, 0, 2, 2]
+    assert np.isnan(e[2]).all() and np.isnan(e[3]).all()
+    ref, _ = oracle.lapack_zgeev(M[:2])
+    assert oracle.max_relerr(ref, e[:2]) <= TOL_LAPACK
+    h = tk.Handle()
+    h.set_max_sweeps(0)
+    e, flag = gpu_eigs(rand(rng, 5, 12), h)
+    assert (flag == 1).all()
+    e, flag = gpu_eigs(np.array([[[3 - 2j]]]), h)   # n = 1 needs no sweep
+    assert flag[0] == 0 and e[0, 0] == 3 - 2j
+    h.close()
+
+
+def test_small_config_full_size(handle):
+    # 10,000 matrices n = 16, every one checked against host LAPACK
+    rng = np.random.default_rng(6)
+    M = rand(rng, 10000, 16)
+    e, flag = gpu_eigs(M, handle)
+    assert not flag.any()
+    ref, info = oracle.lapack_zgeev(M)
+    assert oracle.max_relerr(ref, e) <= TOL_LAPACK
+
+
+def test_n64_batch_trace_and_sample(handle):
+    # size-independent invariants on a large n = 64 batch: sum(lambda) = tr(A),
+    # sum(lambda^2) = tr(A^2); plus LAPACK on a sample
+    g = torch.Generator(device="cuda").manual_seed(7)
+    B, n = 20000, 64
+    A = torch.complex(torch.randn(B, n, n, generator=g, device="cuda", dtype=torch.float64),
+                      torch.randn(B, n, n, generator=g, device="cuda", dtype=torch.float64))
+    print("[case] n=64 batch=20000 invariants", flush=True)
+    eig, flag = tk.eigvals(A, handle)
+    torch.cuda.synchronize()
+    assert int(flag.abs().sum()) == 0
+    tr = torch.diagonal(A, dim1=-2, dim2=-1).sum(-1)
+    tr2 = torch.diagonal(A @ A, dim1=-2, dim2=-1).sum(-1)
+    nrm = torch.linalg.matrix_norm(A)
+    assert float(((eig.sum(-1) - tr).abs() / nrm).max()) < 1e-12
+    assert float(((eig.pow(2).sum(-1) - tr2).abs() / nrm ** 2).max()) < 1e-12
+    idx = torch.arange(0, B, 97, device="cuda")
+    Ms = A[idx].cpu().numpy()
+    ref, _ = oracle.lapack_zgeev(Ms)
+    assert oracle.max_relerr(ref, eig[idx].cpu().numpy()) <= TOL_LAPACK
+
+
+def test_stream_and_reuse(handle):
+    rng = np.random.default_rng(8)
+    M = rand(rng, 300, 20)
+    s = torch.cuda.Stream()
+    A = torch.tensor(M, device="cuda").transpose(-1, -2).contiguous()
+    eig = torch.empty((300, 20), dtype=torch.complex128, device="cuda")
+    flag = torch.empty(300, dtype=torch.int8, device="cuda")
+    work = torch.empty(tk.workspace_size(20, 300), dtype=torch.uint8, device="cuda")
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(3):
+            assert tk.zgeev_batched(A, 20, 300, eig, work, flag, handle, s) == 0
+    s.synchronize()
+    ref, _ = oracle.lapack_zgeev(M)
+    assert oracle.max_relerr(ref, eig.cpu().numpy()) <= TOL_LAPACK
+    # A is not clobbered
+    np.testing.assert_array_equal(A.transpose(-1, -2).cpu().numpy(), M)

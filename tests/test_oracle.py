@@ -1,0 +1,210 @@
+"""Pins the CPU oracle (test infrastructure) before anything is checked against it.
+
+* golden vectors written from host LAPACK (tests/golden/make_golden.py);
+* LAPACK zgehrd+zhseqr run live: the same method, implemented by OpenBLAS;
+* closed forms and invariants chosen so that plausible slips (a dropped
+  conjugate, a wrong shift sign, a transposed operand, a missing term in the
+  adjoint) fail at least one of them.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "zgeev_golden.npz")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD, allow_pickle=False)
+
+
+def rand(rng, b, n):
+    return rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n))
+
+
+def test_golden_lapack_reproduces(gold):
+    # the live LAPACK still gives the committed answers
+    for n in gold["sizes"]:
+        M = gold[f"A_{n}"]
+        e, info = oracle.lapack_zgeev(M)
+        assert not info.any()
+        assert oracle.max_relerr(gold[f"eig_zgeev_{n}"], e) < 1e-13
+
+
+def test_method_matches_golden(gold):
+    for n in gold["sizes"]:
+        M = gold[f"A_{n}"]
+        e, flag = oracle.method_zgeev(M)
+        assert not flag.any()
+        assert oracle.max_relerr(gold[f"eig_zgeev_{n}"], e) < 1e-11, n
+        assert oracle.max_relerr(gold[f"eig_hseqr_{n}"], e) < 1e-11, n
+
+
+@pytest.mark.parametrize("n", [2, 3, 6, 11, 20, 40, 64])
+def test_method_vs_lapack_same_method(n):
+    rng = np.random.default_rng(n)
+    M = rand(rng, 40, n)
+    e, flag = oracle.method_zgeev(M)
+    ref, info = oracle.lapack_hrd_hseqr(M)
+    assert not flag.any() and not info.any()
+    assert oracle.max_relerr(ref, e) < 1e-11
+
+
+def test_triangular_and_diagonal_closed_form():
+    rng = np.random.default_rng(1)
+    for n in (1, 2, 5, 17, 64):
+        U = np.triu(rand(rng, 1, n)[0])
+        e, flag, sw = oracle.method_zgeev(U, with_sweeps=True)
+        assert flag[0] == 0
+        np.testing.assert_allclose(np.sort_complex(e[0]), np.sort_complex(np.diag(U)), rtol=1e-13, atol=1e-13)
+        D = np.diag(np.diag(U))
+        e, flag, sw = oracle.method_zgeev(D, with_sweeps=True)
+        assert sw[0] == 0   # already deflated: no QR sweep
+        np.testing.assert_array_equal(e[0], np.diag(D))   # order of the diagonal
+
+
+def test_2x2_quadratic():
+    rng = np.random.default_rng(2)
+    M = rand(rng, 50, 2)
+    e, _ = oracle.method_zgeev(M)
+    a, b, c, d = M[:, 0, 0], M[:, 0, 1], M[:, 1, 0], M[:, 1, 1]
+    disc = np.sqrt((a - d) ** 2 + 4 * b * c)
+    ref = np.stack([(a + d + disc) / 2, (a + d - disc) / 2], 1)
+    assert oracle.max_relerr(ref, e) < 1e-12
+
+
+@pytest.mark.parametrize("n", [3, 8, 16, 33, 64])
+def test_companion_roots_of_unity(n):
+    C = np.zeros((n, n), complex)
+    C[1:, :-1] = np.eye(n - 1)
+    C[0, -1] = 1.0            # companion of z^n - 1
+    e, flag = oracle.method_zgeev(C)
+    assert flag[0] == 0
+    ref = np.exp(2j * np.pi * np.arange(n) / n)[None]
+    assert oracle.max_relerr(ref, e) < 1e-12
+
+
+def test_hermitian_real_spectrum():
+    rng = np.random.default_rng(3)
+    X = rand(rng, 10, 24)
+    Hm = X + np.conj(X.transpose(0, 2, 1))
+    e, _ = oracle.method_zgeev(Hm)
+    ref = np.linalg.eigvalsh(Hm)
+    assert oracle.max_relerr(ref.astype(complex), e) < 1e-12
+    assert np.abs(e.imag).max() < 1e-12 * np.abs(e).max()
+
+
+def test_known_spectrum_similarity():
+    rng = np.random.default_rng(4)
+    n = 20
+    lam = rand(rng, 1, n)[0][0]
+    V = rand(rng, 1, n)[0]
+    A = V @ np.diag(lam) @ np.linalg.inv(V)
+    e, _ = oracle.method_zgeev(A)
+    assert oracle.max_relerr(lam[None], e) < 1e-9
+
+
+def test_conjugate_transpose_invariant():
+    # eig(A^H) = conj(eig(A)): fails for a dropped conjugate in a reflector
+    rng = np.random.default_rng(5)
+    M = rand(rng, 20, 12)
+    e1, _ = oracle.method_zgeev(M)
+    e2, _ = oracle.method_zgeev(np.conj(M.transpose(0, 2, 1)))
+    assert oracle.max_relerr(np.conj(e1), e2) < 1e-11
+
+
+def test_matched_error_metric_detects_error():
+    rng = np.random.default_rng(6)
+    M = rand(rng, 4, 10)
+    e, _ = oracle.method_zgeev(M)
+    bad = e.copy()
+    bad[2, 3] *= (1 + 1e-8)
+    assert oracle.max_relerr(e, bad) > 5e-9
+    perm = e[:, ::-1]
+    assert oracle.max_relerr(e, perm) < 1e-15   # order does not matter
+
+
+def test_hessenberg_invariants():
+    rng = np.random.default_rng(7)
+    for n in (2, 5, 16, 64):
+        A = rand(rng, 1, n)[0]
+        H = oracle.hessenberg(A)
+        assert np.all(np.tril(H, -2) == 0)
+        sub = np.diag(H, -1)
+        assert np.all(sub.imag == 0)                      # LAPACK: real sub-diagonal
+        assert abs(np.linalg.norm(H) - np.linalg.norm(A)) < 1e-13 * np.linalg.norm(A)
+        assert abs(np.trace(H) - np.trace(A)) < 1e-12 * np.linalg.norm(A)
+
+
+def test_schur_factorisation():
+    rng = np.random.default_rng(8)
+    for n in (1, 3, 16, 40):
+        A = rand(rng, 1, n)[0]
+        T, U, w, info = oracle.method_schur(A)
+        assert info == 0
+        assert np.all(np.tril(T, -1) == 0)
+        assert np.linalg.norm(U.conj().T @ U - np.eye(n)) < 1e-13 * n
+        assert np.linalg.norm(U @ T @ U.conj().T - A) < 1e-13 * n * np.linalg.norm(A)
+        np.testing.assert_allclose(np.diag(T), w, rtol=0, atol=0)
+
+
+def test_flags():
+    rng = np.random.default_rng(9)
+    M = rand(rng, 3, 10)
+    M[1, 2, 3] = np.nan
+    e, flag = oracle.method_zgeev(M)
+    assert list(flag) == [0, 2, 0]
+    assert np.isnan(e[1]).all()
+    e, flag = oracle.method_zgeev(rand(rng, 3, 10), itmax=0)
+    assert (flag == 1).all()
+    e, flag = oracle.method_zgeev(np.array([[[2 + 1j]]]))
+    assert flag[0] == 0 and e[0, 0] == 2 + 1j     # n = 1: converged (reference returns 1)
+
+
+def _adjoint_torch(A, g):
+    torch = pytest.importorskip("torch")
+    At = torch.tensor(A, dtype=torch.complex128, requires_grad=True)
+    lam = torch.linalg.eigvals(At)
+    return lam.detach().numpy(), At, lam
+
+
+def test_adjoint_vs_torch_autograd():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(10)
+    for n in (2, 5, 12):
+        A = rand(rng, 1, n)[0]
+        g = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        G, w = oracle.method_adjoint(A, g)
+        At = torch.tensor(A, requires_grad=True)
+        lam = torch.linalg.eigvals(At)
+        # pair torch's order with the oracle's (Schur-diagonal) order
+        perm = oracle.match(w, lam.detach().numpy())
+        gt = torch.zeros(n, dtype=torch.complex128)
+        gt[torch.tensor(perm)] = torch.tensor(g)
+        loss = (torch.conj(gt) * lam).real.sum()
+        loss.backward()
+        assert np.abs(At.grad.numpy() - G).max() < 1e-10 * max(1.0, np.abs(G).max())
+
+
+def test_adjoint_vs_finite_differences():
+    rng = np.random.default_rng(11)
+    n, h = 6, 1e-6
+    A = rand(rng, 1, n)[0]
+    g = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    G, w = oracle.method_adjoint(A, g)
+
+    def loss(X):
+        e, _ = oracle.method_zgeev(X)
+        perm = oracle.match(w, e[0])
+        return float(np.real(np.sum(np.conj(g) * e[0][perm])))
+
+    for (r, c) in [(0, 0), (2, 3), (5, 1)]:
+        for unit in (1.0, 1j):
+            E = np.zeros_like(A)
+            E[r, c] = unit
+            fd = (loss(A + h * E) - loss(A - h * E)) / (2 * h)
+            an = np.real(np.conj(G[r, c]) * unit)   # dL = Re(conj(G) dA)
+            assert abs(fd - an) < 1e-6 * max(1.0, abs(an))

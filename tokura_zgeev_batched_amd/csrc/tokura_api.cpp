@@ -1,0 +1,280 @@
+// Host side of the C ABI declared in include/tokura_blas.h.
+//
+// Reference counterparts: handle life cycle tokura_zgeev_batched.cu:64-190,
+// workspace sizing :192-210, argument validation and stage dispatch
+// :212-285, per-n launch tables loaded from generated headers :13-60 and the
+// compile-time tuner TOKURA_ZHSEQR_BATCHED_GEHRD_TUNING_main.cu:89-242.
+// Here the launch table is a runtime object in the handle: defaults from a
+// size heuristic, optionally replaced by tokuraAutotune / a cache file keyed
+// by the device (gfx arch name and CU count).
+#include <hip/hip_runtime.h>
+
+#include <cerrno>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/tokura_blas.h"
+#include "tokura_internal.hpp"
+#include "zgeev_launch.hpp"
+
+using namespace tkz;
+
+namespace {
+
+// Default variant per n before any tuning (ids of zgeev_kernels.hip's table).
+int default_variant(int n)
+{
+    if (n <= 4) return 0;    // fused, 4 lanes/matrix, 16 matrices/wave
+    if (n <= 8) return 1;    // fused 8 x 4
+    if (n <= 16) return 4;   // fused 16 x 8
+    if (n <= 32) return 8;   // fused 32 x 32
+    if (n <= 48) return 16;  // split 48 x 64
+    return 18;               // split 64 x 64
+}
+
+bool variant_fits(int v, int n)
+{
+    const VariantInfo *vi = variant_info(v);
+    return vi && n <= vi->nb;
+}
+
+}  // namespace
+
+tokurablas_t::tokurablas_t()
+{
+    device = 0;
+    cu_count = 0;
+    arch[0] = '\0';
+    max_sweeps = -1;
+    for (int n = 0; n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE; ++n) variant_for_n[n] = default_variant(n < 1 ? 1 : n);
+}
+
+extern "C" int tokuraCreate(tokurablas_t **handle)
+{
+    if (handle == nullptr) return -1;
+    tokurablas_t *h = new (std::nothrow) tokurablas_t();
+    if (h == nullptr) return -1;
+    *handle = h;
+    if (hipGetDevice(&h->device) != hipSuccess) return 1;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) return 1;
+    h->cu_count = prop.multiProcessorCount;
+    std::snprintf(h->arch, sizeof(h->arch), "%s", prop.gcnArchName);
+    if (char *p = std::strchr(h->arch, ':')) *p = '\0';
+    if (const char *cache = std::getenv("TOKURA_TUNE_CACHE")) tokuraLoadTuning(h, cache);
+    return 0;
+}
+
+extern "C" int tokuraDestroy(tokurablas_t *handle)
+{
+    if (handle == nullptr) return 0;
+    ragged_release(handle);
+    for (auto &ev : handle->timed_calls)
+        for (auto &x : ev) hipEventDestroy(x);
+    for (auto &ev : handle->event_pool)
+        for (auto &x : ev) hipEventDestroy(x);
+    delete handle;
+    return 0;
+}
+
+extern "C" size_t tokura_get_zgeeveigenvaluesgetworspacesize(int n, int batch_count)
+{
+    if (!(0 < n && n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE)) return (size_t)-1;
+    if (!(0 < batch_count)) return (size_t)-2;
+    return sizeof(hipDoubleComplex) * (size_t)n * (size_t)n * (size_t)batch_count +
+           sizeof(hipDoubleComplex) * (size_t)n * (size_t)batch_count;
+}
+
+extern "C" int tokura_zgeev_batched_gpu(tokurablas_t *handle, int n, int batch_count, hipDoubleComplex *A,
+                                        hipDoubleComplex *eigenvalues, hipDoubleComplex *work, char *flag,
+                                        hipStream_t stream)
+{
+    if (handle == nullptr) return -1;
+    if (!(n > 0 && n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE)) return -2;
+    if (A == nullptr) return -3;
+    if (eigenvalues == nullptr) return -4;
+    if (work == nullptr) return -5;
+    if (!(batch_count > 0)) return -6;
+    if (flag == nullptr) return -7;
+    int v = handle->variant_for_n[n];
+    if (!variant_fits(v, n)) v = default_variant(n);
+    int stage = 1;
+    std::array<hipEvent_t, 3> *ev = handle->stage_timing ? handle->timing_begin(stream) : nullptr;
+    hipError_t e = launch_plain(v, n, batch_count, A, eigenvalues, flag, work, handle->max_sweeps, stream, &stage,
+                                ev ? (*ev)[1] : nullptr);
+    if (ev) {
+        if (variant_info(v)->kind == VK_FUSED) hipEventRecord((*ev)[1], stream);
+        handle->timing_end(ev, stream);
+    }
+    if (e != hipSuccess) return stage;
+    return 0;
+}
+
+std::array<hipEvent_t, 3> *tokurablas_t::timing_begin(hipStream_t s)
+{
+    std::array<hipEvent_t, 3> ev;
+    if (!event_pool.empty()) {
+        ev = event_pool.back();
+        event_pool.pop_back();
+    } else {
+        for (auto &x : ev) hipEventCreate(&x);
+    }
+    timed_calls.push_back(ev);
+    hipEventRecord(timed_calls.back()[0], s);
+    return &timed_calls.back();
+}
+
+void tokurablas_t::timing_end(std::array<hipEvent_t, 3> *ev, hipStream_t s) { hipEventRecord((*ev)[2], s); }
+
+extern "C" int tokuraSetStageTiming(tokurablas_t *handle, int enable)
+{
+    if (handle == nullptr) return -1;
+    handle->stage_timing = enable != 0;
+    return 0;
+}
+
+extern "C" int tokuraGetStageTimes(tokurablas_t *handle, int *calls, float *hessenberg_ms, float *qr_ms)
+{
+    if (handle == nullptr) return -1;
+    float s1 = 0.f, s2 = 0.f;
+    int rc = 0;
+    for (auto &ev : handle->timed_calls) {
+        if (hipEventSynchronize(ev[2]) != hipSuccess) rc = 1;
+        float a = 0.f, b = 0.f;
+        hipEventElapsedTime(&a, ev[0], ev[1]);
+        hipEventElapsedTime(&b, ev[1], ev[2]);
+        s1 += a;
+        s2 += b;
+        handle->event_pool.push_back(ev);
+    }
+    if (calls) *calls = (int)handle->timed_calls.size();
+    if (hessenberg_ms) *hessenberg_ms = s1;
+    if (qr_ms) *qr_ms = s2;
+    handle->timed_calls.clear();
+    return rc;
+}
+
+extern "C" int tokuraSetMaxSweeps(tokurablas_t *handle, int max_sweeps)
+{
+    if (handle == nullptr) return -1;
+    handle->max_sweeps = max_sweeps < 0 ? -1 : max_sweeps;
+    return 0;
+}
+
+extern "C" int tokuraGetLaunchConfig(tokurablas_t *handle, int n, int *variant, int *lanes_per_matrix,
+                                     int *matrices_per_block)
+{
+    if (handle == nullptr) return -1;
+    if (!(n > 0 && n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE)) return -2;
+    int v = handle->variant_for_n[n];
+    if (!variant_fits(v, n)) v = default_variant(n);
+    const VariantInfo *vi = variant_info(v);
+    if (variant) *variant = v;
+    if (lanes_per_matrix) *lanes_per_matrix = vi->lpm;
+    if (matrices_per_block) *matrices_per_block = vi->mpb;
+    return 0;
+}
+
+extern "C" int tokuraSetVariant(tokurablas_t *handle, int n, int variant)
+{
+    if (handle == nullptr) return -1;
+    if (!(n > 0 && n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE)) return -2;
+    if (!variant_fits(variant, n)) return -2;
+    handle->variant_for_n[n] = variant;
+    return 0;
+}
+
+extern "C" int tokuraLoadTuning(tokurablas_t *handle, const char *cache_path)
+{
+    if (handle == nullptr) return -1;
+    if (cache_path == nullptr) return -3;
+    FILE *f = std::fopen(cache_path, "r");
+    if (!f) return -3;
+    char line[256];
+    int count = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        char arch[64];
+        int cus, n, v;
+        if (line[0] == '#') continue;
+        if (std::sscanf(line, "%63s %d %d %d", arch, &cus, &n, &v) != 4) continue;
+        if (std::strcmp(arch, handle->arch) != 0 || cus != handle->cu_count) continue;
+        if (n < 1 || n > TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE || !variant_fits(v, n)) continue;
+        handle->variant_for_n[n] = v;
+        ++count;
+    }
+    std::fclose(f);
+    return count;
+}
+
+extern "C" int tokuraAutotune(tokurablas_t *handle, int n_lo, int n_hi, int batch_count, const char *cache_path,
+                              hipStream_t stream)
+{
+    if (handle == nullptr) return -1;
+    if (n_lo < 1) n_lo = 1;
+    if (n_hi > TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE) n_hi = TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE;
+    if (n_lo > n_hi || batch_count <= 0) return -2;
+    const size_t maxn = (size_t)n_hi;
+    const size_t nA = maxn * maxn * (size_t)batch_count;
+    hipDoubleComplex *dA = nullptr, *dE = nullptr, *dW = nullptr;
+    char *dF = nullptr;
+    if (hipMalloc(&dA, nA * sizeof(hipDoubleComplex)) != hipSuccess) return 3;
+    if (hipMalloc(&dE, maxn * batch_count * sizeof(hipDoubleComplex)) != hipSuccess ||
+        hipMalloc(&dW, tokura_get_zgeeveigenvaluesgetworspacesize((int)maxn, batch_count)) != hipSuccess ||
+        hipMalloc(&dF, (size_t)batch_count) != hipSuccess) {
+        hipFree(dA); hipFree(dE); hipFree(dW); hipFree(dF);
+        return 3;
+    }
+    // seeded N(0,1) entries, the distribution of the benchmark workloads
+    std::vector<hipDoubleComplex> hA(nA);
+    std::mt19937_64 rng(12345);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (auto &z : hA) z = make_hipDoubleComplex(nd(rng), nd(rng));
+    hipMemcpyAsync(dA, hA.data(), nA * sizeof(hipDoubleComplex), hipMemcpyHostToDevice, stream);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    int rc = 0;
+    for (int n = n_lo; n <= n_hi && rc == 0; ++n) {
+        float best = 1e30f;
+        int bestv = default_variant(n);
+        for (int v = 0; v < num_variants(); ++v) {
+            if (!variant_fits(v, n)) continue;
+            // skip variants whose LDS capacity is more than twice what n needs
+            // (they are never faster than a smaller-NB variant of the same lpm)
+            if (variant_info(v)->nb > 2 * ((n + 3) & ~3) && variant_info(v)->nb > 8) continue;
+            int stage;
+            float tms = 0.f;
+            for (int rep = 0; rep < 3; ++rep) {
+                hipEventRecord(e0, stream);
+                if (launch_plain(v, n, batch_count, dA, dE, dF, dW, -1, stream, &stage) != hipSuccess) { rc = 4; break; }
+                hipEventRecord(e1, stream);
+                hipEventSynchronize(e1);
+                float ms;
+                hipEventElapsedTime(&ms, e0, e1);
+                if (rep > 0) tms += ms;   // the first run warms up
+            }
+            if (rc) break;
+            if (tms < best) { best = tms; bestv = v; }
+        }
+        handle->variant_for_n[n] = bestv;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(dA); hipFree(dE); hipFree(dW); hipFree(dF);
+    if (rc == 0 && cache_path != nullptr) {
+        FILE *f = std::fopen(cache_path, "w");
+        if (!f) return 5;
+        std::fprintf(f, "# arch cus n variant  (tokura_zgeev_batched_amd autotune cache)\n");
+        for (int n = 1; n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE; ++n)
+            std::fprintf(f, "%s %d %d %d\n", handle->arch, handle->cu_count, n, handle->variant_for_n[n]);
+        std::fclose(f);
+    }
+    return rc;
+}

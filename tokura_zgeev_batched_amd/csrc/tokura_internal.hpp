@@ -1,0 +1,41 @@
+// Internal definition of the opaque handle shared by the host-side sources.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <array>
+#include <vector>
+
+#include "../../include/tokura_blas.h"
+
+struct RaggedGraphCache {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    const void *A = nullptr;
+    void *eig = nullptr, *work = nullptr;
+    char *flag = nullptr;
+    int batch = -1;
+    hipStream_t stream = nullptr;
+};
+
+struct tokurablas_t {
+    tokurablas_t();
+    int device;
+    int cu_count;
+    char arch[64];
+    int max_sweeps;                                           // < 0: LAPACK cap
+    int variant_for_n[TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE + 1];  // same-size path
+    int ragged_lpm_class[3] = {4, 16, 64};                    // ragged path tiles
+    RaggedGraphCache graph;
+    void *ragged_host_stage = nullptr;                        // pinned staging of the schedule
+    size_t ragged_host_bytes = 0;
+    // stage timing: events (start, between stages, end) per timed call
+    bool stage_timing = false;
+    std::vector<std::array<hipEvent_t, 3>> timed_calls;
+    std::vector<std::array<hipEvent_t, 3>> event_pool;
+    int timed_split = 0;
+    std::array<hipEvent_t, 3> *timing_begin(hipStream_t s);
+    void timing_end(std::array<hipEvent_t, 3> *ev, hipStream_t s);
+};
+
+void ragged_release(tokurablas_t *h);

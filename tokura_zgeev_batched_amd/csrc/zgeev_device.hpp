@@ -1,0 +1,511 @@
+// Device building blocks of the batched complex eigenvalue solver for CDNA4
+// (gfx950): complex FP64 helpers, lane-group reductions, Householder
+// Hessenberg reduction and single-shift complex QR, all on one matrix held in
+// LDS by a group of LPM lanes of one wavefront.
+//
+// Method (reference: tokura_zgeev_batched_library/src/tokura_zgeev_batched.cu:266-278,
+// Hessenberg tokura_zgehrd_MWB_kernel.cuh:159-287, QR
+// tokura_zhseqr_MWB_kernel.cuh:611-791), with the arithmetic of LAPACK's
+// ZGEHD2 / ZLARFG / ZLAHQR (eigenvalues only, no balancing):
+//  * Hessenberg: for i = 0..n-2 a reflector from column i, applied from the
+//    right to rows 0..n-1 and from the left to rows/cols i+1..n-1.
+//  * QR: Ahues-Kressner deflation test, Wilkinson shift with exceptional
+//    shifts every 10th/20th sweep without deflation, the two-consecutive-
+//    small-subdiagonals start, one 2-element reflector per chase step, a cap
+//    of 30*max(10,n) sweeps per eigenvalue.
+//
+// Work split of a chase step k (rows/cols k, k+1 of the active block [l, i]):
+//  * every lane of the group computes the reflector and the 2x2 "window"
+//    H(k:k+1, k:k+1) plus H(k+2, k+1) redundantly - this is the serial chain
+//    that carries the bulge to the next step;
+//  * lane position p >= k+2 applies the reflector from the left to column p
+//    (rows k, k+1), position p <= k-1 applies it from the right to row p
+//    (cols k, k+1).  Each position keeps the element it will touch again at
+//    step k+1 in a register ("carry"), so a step costs one LDS read and one
+//    or two LDS writes per position.
+// Only LDS traffic crosses lanes; a wavefront executes its LDS operations in
+// order, so ordering a write before a read of another lane of the same wave
+// needs only a compiler barrier (lds_order()).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+
+namespace tkz {
+
+struct __align__(16) zd {
+    double re, im;
+};
+
+__device__ __forceinline__ zd mk(double r, double i) { zd z; z.re = r; z.im = i; return z; }
+__device__ __forceinline__ zd zadd(zd a, zd b) { return mk(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ zd zsub(zd a, zd b) { return mk(a.re - b.re, a.im - b.im); }
+__device__ __forceinline__ zd zmul(zd a, zd b) { return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+// conj(a) * b
+__device__ __forceinline__ zd zmulc(zd a, zd b) { return mk(a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re); }
+__device__ __forceinline__ zd zconj(zd a) { return mk(a.re, -a.im); }
+__device__ __forceinline__ zd zscale(double s, zd a) { return mk(s * a.re, s * a.im); }
+__device__ __forceinline__ double cabs1(zd a) { return fabs(a.re) + fabs(a.im); }
+__device__ __forceinline__ double zabs(zd a) { return hypot(a.re, a.im); }
+__device__ __forceinline__ bool zfinite(zd a) { return isfinite(a.re) && isfinite(a.im); }
+
+// Robust complex division x / y (Smith's method with Stewart's ordering).
+__device__ __forceinline__ zd zdiv(zd x, zd y)
+{
+    if (fabs(y.re) >= fabs(y.im)) {
+        double r = y.im / y.re, den = y.re + y.im * r;
+        return mk((x.re + x.im * r) / den, (x.im - x.re * r) / den);
+    } else {
+        double r = y.re / y.im, den = y.im + y.re * r;
+        return mk((x.re * r + x.im) / den, (x.im * r - x.re) / den);
+    }
+}
+
+// Principal complex square root.
+__device__ __forceinline__ zd zsqrt(zd z)
+{
+    if (z.re == 0.0 && z.im == 0.0) return mk(0.0, z.im);
+    double r = zabs(z);
+    if (z.re >= 0.0) {
+        double s = sqrt(0.5 * (r + z.re));
+        return mk(s, z.im / (2.0 * s));
+    } else {
+        double s = sqrt(0.5 * (r - z.re));
+        return mk(fabs(z.im) / (2.0 * s), copysign(s, z.im));
+    }
+}
+
+// Compiler-level ordering of LDS accesses between lanes of one wavefront.
+__device__ __forceinline__ void lds_order() { __asm__ __volatile__("" ::: "memory"); }
+
+// ---- lane groups: LPM consecutive lanes of a wavefront own one matrix ----
+template <int LPM>
+__device__ __forceinline__ double group_sum(double v)
+{
+#pragma unroll
+    for (int o = LPM / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, LPM);
+    return v;
+}
+template <int LPM>
+__device__ __forceinline__ double group_max(double v)
+{
+#pragma unroll
+    for (int o = LPM / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, LPM));
+    return v;
+}
+// Bits of `p` for the lanes of the caller's group, bit t = group lane t.
+template <int LPM>
+__device__ __forceinline__ uint64_t group_ballot(bool p)
+{
+    uint64_t b = __ballot(p);
+    if constexpr (LPM == 64) {
+        return b;
+    } else {
+        const int g = (int)(threadIdx.x & 63) / LPM;
+        return (b >> (g * LPM)) & ((1ull << LPM) - 1ull);
+    }
+}
+template <int LPM>
+__device__ __forceinline__ bool group_any(bool p) { return group_ballot<LPM>(p) != 0ull; }
+
+// ---- matrix layouts in LDS ----
+struct FullLayout {   // column-major with odd leading dimension (bank spread)
+    int ld;
+    __device__ __forceinline__ int operator()(int r, int c) const { return r + c * ld; }
+};
+struct PackedLayout { // upper Hessenberg, column c holds rows 0..c+1
+    __device__ __forceinline__ int operator()(int r, int c) const { return ((c * (c + 3)) >> 1) + r; }
+};
+__host__ __device__ constexpr int packed_size(int n) { return n <= 0 ? 0 : ((n - 1) * (n + 2)) / 2 + n; }
+
+// ---- ZLARFG for a 2-vector (x1, x2): H^H [x1; x2] = [beta; 0] ----
+// On return t1 = tau, v2 = scaled x2, beta; ident = true when tau = 0
+// (then LAPACK leaves alpha = x1 and x2 untouched).  The rescaling loop of
+// ZLARFG for |beta| < safmin/eps (~1e-292) is not reproduced.
+__device__ __forceinline__ void zlarfg2(zd x1, zd x2, zd &t1, zd &v2, double &beta, bool &ident)
+{
+    ident = (x2.re == 0.0 && x2.im == 0.0 && x1.im == 0.0);
+    double ss = x1.re * x1.re + x1.im * x1.im + x2.re * x2.re + x2.im * x2.im;
+    double nrm;
+    if (ss > 1e-290 && ss < 1e290) {
+        nrm = sqrt(ss);
+    } else {
+        double w = fmax(fmax(fabs(x1.re), fabs(x1.im)), fmax(fabs(x2.re), fabs(x2.im)));
+        double iw = w > 0.0 ? 1.0 / w : 0.0;
+        double a = x1.re * iw, b = x1.im * iw, c = x2.re * iw, d = x2.im * iw;
+        nrm = w * sqrt(a * a + b * b + c * c + d * d);
+    }
+    beta = -copysign(nrm, x1.re);
+    double ib = 1.0 / beta;
+    t1 = mk((beta - x1.re) * ib, -x1.im * ib);
+    zd z = mk(x1.re - beta, x1.im);
+    zd rz;
+    double mz = fmax(fabs(z.re), fabs(z.im));
+    if (mz > 1e-150 && mz < 1e150) {
+        double iz = 1.0 / (z.re * z.re + z.im * z.im);
+        rz = mk(z.re * iz, -z.im * iz);
+    } else {
+        rz = zdiv(mk(1.0, 0.0), z);
+    }
+    v2 = zmul(x2, rz);
+    if (ident) { t1 = mk(0.0, 0.0); v2 = x2; beta = x1.re; }
+}
+
+// ---- ZGEHD2 on an n x n matrix in LDS (full layout), LPM lanes ----
+// Entries below the sub-diagonal are zeroed (LAPACK stores the reflectors
+// there; nothing later reads them).
+template <int LPM>
+__device__ void hessenberg(zd *A, const FullLayout lay, int n, int t)
+{
+    for (int i = 0; i + 2 <= n; ++i) {
+        const int m = n - 1 - i;      // reflector length, rows i+1..n-1
+        double ssq = 0.0, amx = 0.0;
+        for (int r = i + 2 + t; r < n; r += LPM) {
+            zd v = A[lay(r, i)];
+            ssq += v.re * v.re + v.im * v.im;
+            amx = fmax(amx, fmax(fabs(v.re), fabs(v.im)));
+        }
+        ssq = group_sum<LPM>(ssq);
+        amx = group_max<LPM>(amx);
+        zd alpha = A[lay(i + 1, i)];
+        double xnorm = 0.0;
+        if (amx > 0.0) {
+            if (ssq > 1e-290 && ssq < 1e290) {
+                xnorm = sqrt(ssq);
+            } else {
+                double ia = 1.0 / amx, s2 = 0.0;
+                for (int r = i + 2 + t; r < n; r += LPM) {
+                    zd v = A[lay(r, i)];
+                    s2 += (v.re * ia) * (v.re * ia) + (v.im * ia) * (v.im * ia);
+                }
+                xnorm = amx * sqrt(group_sum<LPM>(s2));
+            }
+        }
+        if (xnorm == 0.0 && alpha.im == 0.0) continue;   // tau = 0, H(i) = I
+        // beta = -sign(alpha_r) * |(alpha, xnorm)|  (DLAPY3 with scaling)
+        double w = fmax(fmax(fabs(alpha.re), fabs(alpha.im)), xnorm);
+        double iw = 1.0 / w;
+        double ar = alpha.re * iw, ai = alpha.im * iw, xn = xnorm * iw;
+        double beta = -copysign(w * sqrt(ar * ar + ai * ai + xn * xn), alpha.re);
+        zd tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
+        zd scal = zdiv(mk(1.0, 0.0), mk(alpha.re - beta, alpha.im));
+        for (int r = i + 2 + t; r < n; r += LPM) A[lay(r, i)] = zmul(A[lay(r, i)], scal);
+        lds_order();
+        // right: A(0:n-1, i+1:n-1) -= tau (A v) v^H   (v_0 = 1 at row i+1)
+        for (int r = t; r < n; r += LPM) {
+            zd u = A[lay(r, i + 1)];
+            for (int c = 1; c < m; ++c) {
+                zd a = A[lay(r, i + 1 + c)], v = A[lay(i + 1 + c, i)];
+                u = zadd(u, zmul(a, v));
+            }
+            u = zmul(u, tau);
+            A[lay(r, i + 1)] = zsub(A[lay(r, i + 1)], u);
+            for (int c = 1; c < m; ++c) {
+                zd v = A[lay(i + 1 + c, i)];
+                A[lay(r, i + 1 + c)] = zsub(A[lay(r, i + 1 + c)], zmul(u, zconj(v)));
+            }
+        }
+        lds_order();
+        // left: A(i+1:n-1, i+1:n-1) -= conj(tau) v (v^H A)
+        for (int c = i + 1 + t; c < n; c += LPM) {
+            zd s = A[lay(i + 1, c)];
+            for (int r = 1; r < m; ++r) {
+                zd v = A[lay(i + 1 + r, i)];
+                s = zadd(s, zmulc(v, A[lay(i + 1 + r, c)]));
+            }
+            s = zmulc(tau, s);
+            A[lay(i + 1, c)] = zsub(A[lay(i + 1, c)], s);
+            for (int r = 1; r < m; ++r) {
+                zd v = A[lay(i + 1 + r, i)];
+                A[lay(i + 1 + r, c)] = zsub(A[lay(i + 1 + r, c)], zmul(v, s));
+            }
+        }
+        lds_order();
+        if (t == 0) A[lay(i + 1, i)] = mk(beta, 0.0);
+        for (int r = i + 2 + t; r < n; r += LPM) A[lay(r, i)] = mk(0.0, 0.0);
+        lds_order();
+    }
+}
+
+// Ahues-Kressner negligibility test of H(k, k-1) (ZLAHQR loop 40).
+template <class Lay>
+__device__ __forceinline__ bool subdiag_negligible(const zd *H, const Lay &lay, int k, int n,
+                                                   double smlnum, double ulp)
+{
+    zd h = H[lay(k, k - 1)];
+    if (cabs1(h) <= smlnum) return true;
+    zd hkm = H[lay(k - 1, k - 1)], hkk = H[lay(k, k)];
+    double tst = cabs1(hkm) + cabs1(hkk);
+    if (tst == 0.0) {
+        if (k - 2 >= 0) tst += fabs(H[lay(k - 1, k - 2)].re);
+        if (k + 1 <= n - 1) tst += fabs(H[lay(k + 1, k)].re);
+    }
+    if (fabs(h.re) <= ulp * tst) {
+        zd hup = H[lay(k - 1, k)];
+        double ab = fmax(cabs1(h), cabs1(hup)), ba = fmin(cabs1(h), cabs1(hup));
+        zd dd = zsub(hkm, hkk);
+        double aa = fmax(cabs1(hkk), cabs1(dd)), bb = fmin(cabs1(hkk), cabs1(dd));
+        double s = aa + ab;
+        if (ba * (ab / s) <= fmax(smlnum, ulp * (bb * (aa / s)))) return true;
+    }
+    return false;
+}
+
+// Largest position in [lo, hi] (positions p = t + s*LPM) whose predicate is
+// true, or `none`.  pred(p) is evaluated only for p in [lo, hi].
+template <int LPM, int S, class F>
+__device__ __forceinline__ int group_last_true(int t, int lo, int hi, int none, F pred)
+{
+    int found = none;
+#pragma unroll
+    for (int s = S - 1; s >= 0; --s) {
+        int p = t + s * LPM;
+        bool q = (p >= lo && p <= hi) ? pred(p) : false;
+        uint64_t bm = group_ballot<LPM>(q);
+        if (found == none && bm != 0ull) found = s * LPM + (63 - __clzll((long long)bm));
+    }
+    return found;
+}
+
+// ---- ZLAHQR, eigenvalues only (WANTT = WANTZ = false), ilo = 0, ihi = n-1 ----
+// Returns 0 or 1 + the 0-based index of the first eigenvalue that did not
+// converge (LAPACK INFO).  On return the eigenvalues are H(i,i).
+template <int LPM, int S, class Lay>
+__device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_override, int *nsweeps)
+{
+    const double ulp = DBL_EPSILON, safmin = DBL_MIN;
+    const double smlnum = safmin * ((double)n / ulp);
+    const int itmax = itmax_override >= 0 ? itmax_override : 30 * (n > 10 ? n : 10);
+    int kdefl = 0, sweeps = 0;
+    int i = n - 1;
+    while (i >= 1) {
+        int l = 0;
+        bool conv = false;
+        for (int its = 0; its <= itmax; ++its) {
+            // ---- look for a single small sub-diagonal element
+            const int k = group_last_true<LPM, S>(t, l + 1, i, l, [&](int p) {
+                return subdiag_negligible(H, lay, p, n, smlnum, ulp);
+            });
+            l = k;
+            if (l > 0 && t == 0) H[lay(l, l - 1)] = mk(0.0, 0.0);
+            lds_order();
+            if (l >= i) { conv = true; break; }
+            ++kdefl;
+            // ---- shift
+            zd sh;
+            if (kdefl % 20 == 0) {
+                double s = 0.75 * fabs(H[lay(i, i - 1)].re);
+                sh = zadd(H[lay(i, i)], mk(s, 0.0));
+            } else if (kdefl % 10 == 0) {
+                double s = 0.75 * fabs(H[lay(l + 1, l)].re);
+                sh = zadd(H[lay(l, l)], mk(s, 0.0));
+            } else {
+                sh = H[lay(i, i)];
+                zd u = zmul(zsqrt(H[lay(i - 1, i)]), zsqrt(H[lay(i, i - 1)]));
+                double s = cabs1(u);
+                if (s != 0.0) {
+                    zd x = zscale(0.5, zsub(H[lay(i - 1, i - 1)], sh));
+                    double sx = cabs1(x);
+                    s = fmax(s, sx);
+                    double is = 1.0 / s;
+                    zd xs = zscale(is, x), us = zscale(is, u);
+                    zd y = zscale(s, zsqrt(zadd(zmul(xs, xs), zmul(us, us))));
+                    if (sx > 0.0) {
+                        zd xn = zscale(1.0 / sx, x);
+                        if (xn.re * y.re + xn.im * y.im < 0.0) y = mk(-y.re, -y.im);
+                    }
+                    sh = zsub(sh, zmul(u, zdiv(u, zadd(x, y))));
+                }
+            }
+            // ---- look for two consecutive small sub-diagonal elements
+            const int m = group_last_true<LPM, S>(t, l + 1, i - 1, l, [&](int p) {
+                zd h11 = H[lay(p, p)], h22 = H[lay(p + 1, p + 1)];
+                zd h11s = zsub(h11, sh);
+                double h21 = H[lay(p + 1, p)].re;
+                double s = cabs1(h11s) + fabs(h21);
+                h11s = zscale(1.0 / s, h11s);
+                h21 = h21 / s;
+                double h10 = H[lay(p, p - 1)].re;
+                return fabs(h10) * fabs(h21) <= ulp * (cabs1(h11s) * (cabs1(h11) + cabs1(h22)));
+            });
+            zd x1, x2;
+            {
+                zd h11s = zsub(H[lay(m, m)], sh);
+                double h21 = H[lay(m + 1, m)].re;
+                double s = cabs1(h11s) + fabs(h21);
+                x1 = mk(h11s.re / s, h11s.im / s);
+                x2 = mk(h21 / s, 0.0);
+            }
+            ++sweeps;
+            // ---- single-shift sweep k = m .. i-1 over the active block [l, i]
+            zd carry[S], ypre[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int p = t + s * LPM;
+                carry[s] = mk(0.0, 0.0);
+                ypre[s] = mk(0.0, 0.0);
+                if (p >= m + 2 && p <= i) { carry[s] = H[lay(m, p)]; ypre[s] = H[lay(m + 1, p)]; }
+                else if (p >= l && p <= m - 1) { carry[s] = H[lay(p, m)]; ypre[s] = H[lay(p, m + 1)]; }
+            }
+            zd a = H[lay(m, m)], b = H[lay(m, m + 1)], c = H[lay(m + 1, m)], d = H[lay(m + 1, m + 1)];
+            zd e = (m + 2 <= i) ? H[lay(m + 2, m + 1)] : mk(0.0, 0.0);
+            for (int kk = m; kk < i; ++kk) {
+                zd t1, v2;
+                double beta;
+                bool ident;
+                zlarfg2(x1, x2, t1, v2, beta, ident);
+                const double t2 = t1.re * v2.re - t1.im * v2.im;   // DBLE(T1*V2)
+                // bulk: left on columns p >= kk+2, right on rows l <= p <= kk-1
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int p = t + s * LPM;
+                    const bool rowop = (p >= kk + 2 && p <= i);
+                    const bool colop = (p >= l && p <= kk - 1);
+                    if (rowop || colop) {
+                        zd c1 = rowop ? zconj(t1) : t1;
+                        zd vv = rowop ? v2 : zconj(v2);
+                        zd x = carry[s], y = ypre[s];
+                        zd sm = zadd(zmul(c1, x), zscale(t2, y));
+                        x = zsub(x, sm);
+                        y = zsub(y, zmul(sm, vv));
+                        if (rowop) {
+                            H[lay(kk, p)] = x;
+                            H[lay(kk + 1, p)] = y;
+                        } else {
+                            H[lay(p, kk)] = x;
+                        }
+                        carry[s] = y;
+                    }
+                }
+                // window: rows/cols kk, kk+1 and row kk+2
+                if (kk > m && t == 0) H[lay(kk, kk - 1)] = ident ? x1 : mk(beta, 0.0);
+                zd tc = zconj(t1), vc = zconj(v2);
+                zd s1 = zadd(zmul(tc, a), zscale(t2, c));
+                a = zsub(a, s1);
+                c = zsub(c, zmul(s1, v2));
+                zd s2 = zadd(zmul(tc, b), zscale(t2, d));
+                b = zsub(b, s2);
+                d = zsub(d, zmul(s2, v2));
+                zd s3 = zadd(zmul(t1, a), zscale(t2, b));
+                a = zsub(a, s3);
+                b = zsub(b, zmul(s3, vc));
+                zd s4 = zadd(zmul(t1, c), zscale(t2, d));
+                c = zsub(c, s4);
+                d = zsub(d, zmul(s4, vc));
+                zd f = mk(0.0, 0.0);
+                if (kk + 2 <= i) {
+                    zd s5 = zscale(t2, e);
+                    f = mk(-s5.re, -s5.im);
+                    e = zsub(e, zmul(s5, vc));
+                }
+                if (t == 0) H[lay(kk, kk)] = a;
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                    if (t + s * LPM == kk) carry[s] = b;
+                lds_order();
+                if (kk == m && m > l) {
+                    // ZLAHQR keeps H(m, m-1) real after a start at m > l:
+                    // flush the step's state, scale in LDS, reload.
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const int p = t + s * LPM;
+                        if (p >= l && p <= m - 1) H[lay(p, m + 1)] = carry[s];
+                    }
+                    if (t == 0) {
+                        H[lay(m, m + 1)] = b;
+                        H[lay(m + 1, m)] = c;
+                        H[lay(m + 1, m + 1)] = d;
+                        if (m + 2 <= i) H[lay(m + 2, m + 1)] = e;
+                    }
+                    lds_order();
+                    zd temp = zsub(mk(1.0, 0.0), t1);
+                    temp = zscale(1.0 / zabs(temp), temp);
+                    zd tempc = zconj(temp);
+                    if (t == 0) {
+                        H[lay(m + 1, m)] = zmul(H[lay(m + 1, m)], tempc);
+                        if (m + 2 <= i) H[lay(m + 2, m + 1)] = zmul(H[lay(m + 2, m + 1)], temp);
+                    }
+                    lds_order();
+                    for (int j = m + t; j <= i; j += LPM)
+                        if (j != m + 1)
+                            for (int cc = j + 1; cc <= i; ++cc) H[lay(j, cc)] = zmul(H[lay(j, cc)], temp);
+                    lds_order();
+                    for (int j = m + t; j <= i; j += LPM)
+                        if (j != m + 1)
+                            for (int r = l; r <= j - 1; ++r) H[lay(r, j)] = zmul(H[lay(r, j)], tempc);
+                    lds_order();
+                    const int k1 = kk + 1;
+                    if (k1 < i) {
+                        x1 = H[lay(k1, kk)];
+                        x2 = f;
+                        a = H[lay(k1, k1)];
+                        b = H[lay(k1, k1 + 1)];
+                        c = H[lay(k1 + 1, k1)];
+                        d = H[lay(k1 + 1, k1 + 1)];
+                        e = (k1 + 2 <= i) ? H[lay(k1 + 2, k1 + 1)] : mk(0.0, 0.0);
+#pragma unroll
+                        for (int s = 0; s < S; ++s) {
+                            const int p = t + s * LPM;
+                            if (p >= k1 + 2 && p <= i) { carry[s] = H[lay(k1, p)]; ypre[s] = H[lay(k1 + 1, p)]; }
+                            else if (p >= l && p <= k1 - 1) { carry[s] = H[lay(p, k1)]; ypre[s] = H[lay(p, k1 + 1)]; }
+                        }
+                    } else {
+                        c = H[lay(i, i - 1)];
+                        d = H[lay(i, i)];
+#pragma unroll
+                        for (int s = 0; s < S; ++s) {
+                            const int p = t + s * LPM;
+                            if (p >= l && p <= i - 1) carry[s] = H[lay(p, i)];
+                        }
+                    }
+                    continue;
+                }
+                if (kk + 1 < i) {
+                    const int k1 = kk + 1;
+                    x1 = c;
+                    x2 = f;
+                    a = d;
+                    c = e;
+                    b = H[lay(k1, k1 + 1)];
+                    d = H[lay(k1 + 1, k1 + 1)];
+                    e = (k1 + 2 <= i) ? H[lay(k1 + 2, k1 + 1)] : mk(0.0, 0.0);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const int p = t + s * LPM;
+                        if (p >= k1 + 2 && p <= i) ypre[s] = H[lay(k1 + 1, p)];
+                        else if (p >= l && p <= k1 - 1) ypre[s] = H[lay(p, k1 + 1)];
+                    }
+                }
+            }
+            // ---- end of sweep: c = H(i,i-1), d = H(i,i); make H(i,i-1) real
+            zd hsub = c;
+            bool rot = (c.im != 0.0);
+            zd temp = mk(1.0, 0.0);
+            if (rot) {
+                double r = zabs(c);
+                hsub = mk(r, 0.0);
+                temp = zscale(1.0 / r, c);
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int p = t + s * LPM;
+                if (p >= l && p <= i - 1) H[lay(p, i)] = rot ? zmul(carry[s], temp) : carry[s];
+            }
+            if (t == 0) {
+                H[lay(i, i - 1)] = hsub;
+                H[lay(i, i)] = d;
+            }
+            lds_order();
+        }
+        if (!conv) {
+            if (nsweeps) *nsweeps = sweeps;
+            return i + 1;
+        }
+        kdefl = 0;
+        i = l - 1;
+    }
+    if (nsweeps) *nsweeps = sweeps;
+    return 0;
+}
+
+}  // namespace tkz

@@ -1,0 +1,27 @@
+// Internal interface between the HIP kernels and the host-side C ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace tkz {
+
+enum VariantKind { VK_FUSED = 0, VK_SPLIT = 1 };
+
+// One compiled launch configuration of the same-size path.
+//   nb   largest n the LDS allocation holds
+//   lpm  lanes per matrix (a wavefront holds 64/lpm matrices)
+//   mpb  matrices per workgroup (one wavefront per workgroup)
+struct VariantInfo {
+    int id, kind, nb, lpm, mpb;
+};
+
+int num_variants();
+const VariantInfo *variant_info(int v);
+
+// Launches variant `variant` on `s`; *stage reports 1 (Hessenberg) or 2 (QR)
+// for the launch that failed.
+// When `mid` is non-null it is recorded on `s` between the Hessenberg and the
+// QR kernel of a split variant (fused variants record nothing).
+hipError_t launch_plain(int variant, int n, int batch, const void *A, void *eig, char *flag, void *work,
+                        int itmax, hipStream_t s, int *stage, hipEvent_t mid = nullptr);
+
+}  // namespace tkz
