@@ -118,20 +118,25 @@ int tokuraSetVariant(tokurablas_t *handle, int n, int variant);
 /* ---- ragged batches: n varies per matrix --------------------------------
  * Matrix b is stored column-major at A + offsets[b] and is n_b x n_b with
  * n_b*n_b = offsets[b+1] - offsets[b], 1 <= n_b <= 64 (offsets are int64,
- * counted in complex elements, batch_count+1 entries, non-decreasing; matrices
- * need not be contiguous with one another).  Eigenvalues of matrix b are
- * written to eigenvalues[e_b + i], e_b = n_0 + ... + n_{b-1}; flag[b] as above.
+ * counted in complex elements, batch_count+1 entries, so matrix b+1 starts
+ * where matrix b ends).  Eigenvalues of matrix b are written to
+ * eigenvalues[e_b + i], e_b = n_0 + ... + n_{b-1}; flag[b] as above.
  *
+ * tokura_get_zgeev_ragged_worksize: bytes of `work` for this batch ((size_t)-2
+ * for batch_count <= 0, (size_t)-3 for invalid offsets).
  * tokura_zgeev_ragged_prepare reads the host copy of offsets, builds the
- * schedule (cost-descending work queue, eigenvalue offsets) and copies it
- * into `work` with an asynchronous copy on `stream`; it must run before the
- * solves that use that `work`, and the schedule stays valid for any number of
- * solves of batches with the same offsets.
- * tokura_zgeev_ragged_gpu then runs one persistent kernel that pulls matrices
- * from the queue; tokura_zgeev_ragged_graph does the same through a hipGraph
- * that the handle captures on first use and replays afterwards (re-captured
- * when the pointers or batch change). */
-size_t tokura_get_zgeev_ragged_worksize(int batch_count);
+ * schedule (size-class tiles in decreasing size, eigenvalue offsets, packed
+ * Hessenberg offsets) and copies it into `work` with an asynchronous copy on
+ * `stream`.  The schedule stays valid for any number of solves of batches
+ * with the same offsets; it uses the handle's current class configuration.
+ * tokura_zgeev_ragged_gpu launches the Hessenberg pass and one persistent QR
+ * kernel whose wavefronts pull tiles from an atomic queue;
+ * tokura_zgeev_ragged_graph does the same through a hipGraph that the handle
+ * captures on first use and replays while pointers, batch, class
+ * configuration and sweep cap are unchanged.
+ * Return codes as tokura_zgeev_batched_gpu, plus -8 when `work` has not been
+ * prepared for this batch_count. */
+size_t tokura_get_zgeev_ragged_worksize(int batch_count, const int64_t *offsets_host);
 int tokura_zgeev_ragged_prepare(tokurablas_t *handle, int batch_count, const int64_t *offsets_host,
                                 void *work, size_t work_bytes, hipStream_t stream);
 int tokura_zgeev_ragged_gpu(tokurablas_t *handle, int batch_count, const hipDoubleComplex *A,
@@ -140,6 +145,17 @@ int tokura_zgeev_ragged_gpu(tokurablas_t *handle, int batch_count, const hipDoub
 int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, const hipDoubleComplex *A,
                               hipDoubleComplex *eigenvalues, void *work, char *flag,
                               hipStream_t stream);
+
+/* Class configuration of the ragged path: lanes per matrix of class A
+ * (n 33..64), B (17..32), C (1..16); a wavefront holds 64/lanes matrices of
+ * a class.  tokuraAutotuneRagged times every configuration on the caller's
+ * prepared batch (buffers as for tokura_zgeev_ragged_gpu; the stream is
+ * synchronised), keeps the fastest and re-prepares `work` for it. */
+int tokuraSetRaggedConfig(tokurablas_t *handle, int config);
+int tokuraGetRaggedConfig(tokurablas_t *handle, int *config, int *lanes_a, int *lanes_b, int *lanes_c);
+int tokuraAutotuneRagged(tokurablas_t *handle, int batch_count, const int64_t *offsets_host,
+                         const hipDoubleComplex *A, hipDoubleComplex *eigenvalues, void *work,
+                         size_t work_bytes, char *flag, hipStream_t stream);
 
 /* ---- differentiation ----------------------------------------------------
  * Forward with Schur vectors: besides eigenvalues[i + j*n] and flag, writes

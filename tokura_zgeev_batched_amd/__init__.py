@@ -49,7 +49,10 @@ _SIGNATURES = {
     "tokuraSetVariant": ([_P, _I, _I], _I),
     "tokuraSetStageTiming": ([_P, _I], _I),
     "tokuraGetStageTimes": ([_P, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)], _I),
-    "tokura_get_zgeev_ragged_worksize": ([_I], _SZ),
+    "tokura_get_zgeev_ragged_worksize": ([_I, _P], _SZ),
+    "tokuraSetRaggedConfig": ([_P, _I], _I),
+    "tokuraGetRaggedConfig": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
+    "tokuraAutotuneRagged": ([_P, _I, _P, _P, _P, _P, _SZ, _P, _P], _I),
     "tokura_zgeev_ragged_prepare": ([_P, _I, _P, _P, _SZ, _P], _I),
     "tokura_zgeev_ragged_gpu": ([_P, _I, _P, _P, _P, _P, _P], _I),
     "tokura_zgeev_ragged_graph": ([_P, _I, _P, _P, _P, _P, _P], _I),

@@ -15,7 +15,14 @@ struct RaggedGraphCache {
     void *eig = nullptr, *work = nullptr;
     char *flag = nullptr;
     int batch = -1;
-    hipStream_t stream = nullptr;
+    int cfg = -1;
+    int max_sweeps = -1;
+};
+
+struct RaggedPrepared {
+    void *work;
+    int cfg;
+    int batch;
 };
 
 struct tokurablas_t {
@@ -25,10 +32,12 @@ struct tokurablas_t {
     char arch[64];
     int max_sweeps;                                           // < 0: LAPACK cap
     int variant_for_n[TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE + 1];  // same-size path
-    int ragged_lpm_class[3] = {4, 16, 64};                    // ragged path tiles
+    int ragged_cfg = 0;                                       // ragged class configuration
     RaggedGraphCache graph;
     void *ragged_host_stage = nullptr;                        // pinned staging of the schedule
     size_t ragged_host_bytes = 0;
+    hipStream_t ragged_stage_stream = nullptr;
+    std::vector<RaggedPrepared> prepared;
     // stage timing: events (start, between stages, end) per timed call
     bool stage_timing = false;
     std::vector<std::array<hipEvent_t, 3>> timed_calls;
@@ -39,3 +48,4 @@ struct tokurablas_t {
 };
 
 void ragged_release(tokurablas_t *h);
+int ragged_num_configs();

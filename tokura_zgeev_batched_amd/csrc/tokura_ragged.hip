@@ -1,6 +1,270 @@
-// Ragged batches (n varies per matrix): persistent work-queue kernel.
+// Ragged batches: n varies per matrix (1..64).  No counterpart in the
+// reference, which requires one n per call (README.md:5, tokura_blas.h:12).
+//
+// Pipeline (both launches are captured together by tokura_zgeev_ragged_graph):
+//  1. k_ragged_reset    zeroes the queue head (a fill kernel, graph-safe).
+//  2. k_ragged_hess     Householder Hessenberg of every matrix, by tiles,
+//                       full-storage LDS, packed result written to `work`.
+//  3. k_ragged_qr       ONE persistent kernel: every wavefront pulls the next
+//                       tile from an atomic queue until the queue is empty, so
+//                       matrices that need more QR sweeps never hold up a
+//                       whole launch; tiles are queued in decreasing size
+//                       (longest-processing-time first).
+// A tile is the work of one wavefront: one matrix of class A (33..64), or
+// 64/LPM_B matrices of class B (17..32), or 64/LPM_C of class C (1..16); the
+// lanes-per-matrix of each class form a "class configuration" that the
+// autotuner picks.
+//
+// `work` layout (built on the host by tokura_zgeev_ragged_prepare):
+//   RaggedHeader | tiles int4[ntiles] | order int[B] | n int[B] |
+//   moff int64[B] | eoff int64[B] | hoff int64[B] | packed H (complex)
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
 #include "tokura_internal.hpp"
 #include "zgeev_device.hpp"
+
+namespace tkz {
+
+struct RaggedHeader {
+    int counter;   // QR queue head
+    int ntiles;
+    int batch;
+    int cfg;
+    long long off_tiles, off_order, off_n, off_moff, off_eoff, off_hoff, off_H;
+    long long total_bytes;
+    long long pad[6];
+};
+static_assert(sizeof(RaggedHeader) == 128, "header size");
+
+// lanes per matrix of classes A (33..64), B (17..32), C (1..16)
+#define TKZ_RAGGED_CFGS(X) \
+    X(0, 64, 16, 4)        \
+    X(1, 64, 32, 8)        \
+    X(2, 64, 16, 8)        \
+    X(3, 64, 32, 4)
+static const int g_cfg_lpm[][3] = {
+#define X(id, a, b, c) {a, b, c},
+    TKZ_RAGGED_CFGS(X)
+#undef X
+};
+static constexpr int kNumRaggedCfg = 4;
+
+__host__ __device__ constexpr int cls_of(int n) { return n > 32 ? 0 : (n > 16 ? 1 : 2); }
+__host__ __device__ constexpr int cls_nb(int c) { return c == 0 ? 64 : (c == 1 ? 32 : 16); }
+
+struct RaggedView {
+    RaggedHeader *h;
+    const int4 *tiles;
+    const int *order, *nsz;
+    const long long *moff, *eoff, *hoff;
+    zd *H;
+    __device__ RaggedView(char *w)
+    {
+        h = (RaggedHeader *)w;
+        tiles = (const int4 *)(w + h->off_tiles);
+        order = (const int *)(w + h->off_order);
+        nsz = (const int *)(w + h->off_n);
+        moff = (const long long *)(w + h->off_moff);
+        eoff = (const long long *)(w + h->off_eoff);
+        hoff = (const long long *)(w + h->off_hoff);
+        H = (zd *)(w + h->off_H);
+    }
+};
+
+__global__ void k_ragged_reset(char *work) { ((RaggedHeader *)work)->counter = 0; }
+
+template <int NB, int LPM>
+__device__ __forceinline__ void hess_tile(zd *lds, const RaggedView &V, int4 td, const zd *__restrict__ A,
+                                          int8_t *__restrict__ flag)
+{
+    constexpr int LDP = NB | 1;
+    const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
+    if (g >= td.y) return;
+    const int b = V.order[td.z + g];
+    const int n = V.nsz[b];
+    zd *H = lds + g * NB * LDP;
+    const FullLayout lay{n | 1};
+    const bool ok = load_full<LPM>(H, lay, A + V.moff[b], n, t);
+    lds_order();
+    if (ok) hessenberg<LPM>(H, lay, n, t);
+    store_packed<LPM>(H, lay, V.H + V.hoff[b], n, t);
+    if (t == 0) flag[b] = ok ? 0 : 2;
+}
+
+template <int NB, int LPM>
+__device__ __forceinline__ void qr_tile(zd *lds, const RaggedView &V, int4 td, zd *__restrict__ eig,
+                                        int8_t *__restrict__ flag, int itmax)
+{
+    constexpr int PS = packed_size(NB), S = (NB + LPM - 1) / LPM;
+    const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
+    if (g >= td.y) return;
+    const int b = V.order[td.z + g];
+    const int n = V.nsz[b];
+    zd *out = eig + V.eoff[b];
+    if (flag[b] == 2) {
+        for (int r = t; r < n; r += LPM) out[r] = mk(__builtin_nan(""), __builtin_nan(""));
+        return;
+    }
+    zd *H = lds + g * PS;
+    const zd *src = V.H + V.hoff[b];
+    const int ps = packed_size(n);
+    for (int e = t; e < ps; e += LPM) H[e] = src[e];
+    lds_order();
+    const PackedLayout pk;
+    int info = 0;
+    if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax, nullptr);
+    for (int r = t; r < n; r += LPM) out[r] = H[pk(r, r)];
+    if (t == 0) flag[b] = info != 0 ? 1 : 0;
+    lds_order();
+}
+
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+template <int LA, int LB, int LC>
+__global__ __launch_bounds__(64) void k_ragged_hess(const zd *__restrict__ A, char *work, int8_t *__restrict__ flag)
+{
+    constexpr int SZ = cmax(cmax((64 / LA) * 64 * 65, (64 / LB) * 32 * 33), (64 / LC) * 16 * 17);
+    __shared__ zd lds[SZ];
+    const RaggedView V(work);
+    const int nt = V.h->ntiles;
+    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+        const int4 td = V.tiles[tile];
+        if (td.x == 0) hess_tile<64, LA>(lds, V, td, A, flag);
+        else if (td.x == 1) hess_tile<32, LB>(lds, V, td, A, flag);
+        else hess_tile<16, LC>(lds, V, td, A, flag);
+        lds_order();
+    }
+}
+
+template <int LA, int LB, int LC>
+__global__ __launch_bounds__(64) void k_ragged_qr(char *work, zd *__restrict__ eig, int8_t *__restrict__ flag, int itmax)
+{
+    constexpr int SZ = cmax(cmax((64 / LA) * packed_size(64), (64 / LB) * packed_size(32)),
+                            (64 / LC) * packed_size(16));
+    __shared__ zd lds[SZ];
+    const RaggedView V(work);
+    const int nt = V.h->ntiles;
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        int tile = 0;
+        if (lane == 0) tile = atomicAdd(&V.h->counter, 1);
+        tile = __shfl(tile, 0);
+        if (tile >= nt) break;
+        const int4 td = V.tiles[tile];
+        if (td.x == 0) qr_tile<64, LA>(lds, V, td, eig, flag, itmax);
+        else if (td.x == 1) qr_tile<32, LB>(lds, V, td, eig, flag, itmax);
+        else qr_tile<16, LC>(lds, V, td, eig, flag, itmax);
+    }
+}
+
+template <int LA, int LB, int LC>
+static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int cus,
+                                    hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(1), 0, s, work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // resident workgroups per CU, queried once (outside any graph capture the
+    // first call is made from tokura_zgeev_ragged_graph's capture stream, where
+    // occupancy queries are not stream operations and are allowed)
+    static int occ_h = 0, occ_q = 0;
+    if (occ_h == 0) {
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_h, k_ragged_hess<LA, LB, LC>, 64, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_q, k_ragged_qr<LA, LB, LC>, 64, 0);
+        if (occ_h < 1) occ_h = 1;
+        if (occ_q < 1) occ_q = 1;
+    }
+    hipLaunchKernelGGL((k_ragged_hess<LA, LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ragged_qr<LA, LB, LC>), dim3(cus * occ_q), dim3(64), 0, s, work, eig, flag, itmax);
+    return hipGetLastError();
+}
+
+static hipError_t launch_ragged(int cfg, const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int cus,
+                                hipStream_t s)
+{
+    switch (cfg) {
+#define X(id, a, b, c) \
+    case id: return launch_ragged_cfg<a, b, c>(A, work, eig, flag, itmax, cus, s);
+        TKZ_RAGGED_CFGS(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---- host-side schedule ---------------------------------------------------
+struct Schedule {
+    RaggedHeader hdr;
+    std::vector<int> tiles;   // 4 ints per tile
+    std::vector<int> order, nsz;
+    std::vector<long long> moff, eoff, hoff;
+    long long nH = 0;
+};
+
+static long long align256(long long x) { return (x + 255) & ~255LL; }
+
+static int build_schedule(int batch, const int64_t *offsets, int cfg, Schedule &S)
+{
+    S.nsz.resize(batch);
+    S.moff.resize(batch);
+    S.eoff.resize(batch);
+    S.hoff.resize(batch);
+    long long e = 0;
+    for (int b = 0; b < batch; ++b) {
+        long long cnt = (long long)offsets[b + 1] - (long long)offsets[b];
+        long long n = (long long)std::llround(std::sqrt((double)cnt));
+        if (offsets[b] < 0 || cnt < 1 || n * n != cnt || n > TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE) return -3;
+        S.nsz[b] = (int)n;
+        S.moff[b] = offsets[b];
+        S.eoff[b] = e;
+        e += n;
+    }
+    // order: by size class, then n descending (largest work first)
+    S.order.resize(batch);
+    std::iota(S.order.begin(), S.order.end(), 0);
+    std::stable_sort(S.order.begin(), S.order.end(), [&](int a, int b) { return S.nsz[a] > S.nsz[b]; });
+    long long h = 0;
+    for (int i = 0; i < batch; ++i) {
+        int b = S.order[i];
+        S.hoff[b] = h;
+        h += packed_size(S.nsz[b]);
+    }
+    S.nH = h;
+    S.tiles.clear();
+    int i = 0;
+    while (i < batch) {
+        const int c = cls_of(S.nsz[S.order[i]]);
+        const int per = 64 / g_cfg_lpm[cfg][c];
+        int j = i;
+        while (j < batch && j - i < per && cls_of(S.nsz[S.order[j]]) == c) ++j;
+        S.tiles.insert(S.tiles.end(), {c, j - i, i, S.nsz[S.order[i]]});
+        i = j;
+    }
+    const int ntiles = (int)(S.tiles.size() / 4);
+    std::memset(&S.hdr, 0, sizeof S.hdr);
+    S.hdr.ntiles = ntiles;
+    S.hdr.batch = batch;
+    S.hdr.cfg = cfg;
+    long long o = align256(sizeof(RaggedHeader));
+    S.hdr.off_tiles = o;  o = align256(o + 16LL * ntiles);
+    S.hdr.off_order = o;  o = align256(o + 4LL * batch);
+    S.hdr.off_n = o;      o = align256(o + 4LL * batch);
+    S.hdr.off_moff = o;   o = align256(o + 8LL * batch);
+    S.hdr.off_eoff = o;   o = align256(o + 8LL * batch);
+    S.hdr.off_hoff = o;   o = align256(o + 8LL * batch);
+    S.hdr.off_H = o;      o = align256(o + 16LL * S.nH);
+    S.hdr.total_bytes = o;
+    return 0;
+}
+
+}  // namespace tkz
+
+using namespace tkz;
 
 void ragged_release(tokurablas_t *h)
 {
@@ -10,9 +274,177 @@ void ragged_release(tokurablas_t *h)
     if (h->ragged_host_stage) hipHostFree(h->ragged_host_stage);
     h->ragged_host_stage = nullptr;
     h->ragged_host_bytes = 0;
+    h->prepared.clear();
 }
 
-extern "C" size_t tokura_get_zgeev_ragged_worksize(int batch_count) { return batch_count > 0 ? 0 : (size_t)-2; }
-extern "C" int tokura_zgeev_ragged_prepare(tokurablas_t *, int, const int64_t *, void *, size_t, hipStream_t) { return -100; }
-extern "C" int tokura_zgeev_ragged_gpu(tokurablas_t *, int, const hipDoubleComplex *, hipDoubleComplex *, void *, char *, hipStream_t) { return -100; }
-extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *, int, const hipDoubleComplex *, hipDoubleComplex *, void *, char *, hipStream_t) { return -100; }
+int ragged_num_configs() { return kNumRaggedCfg; }
+
+extern "C" size_t tokura_get_zgeev_ragged_worksize(int batch_count, const int64_t *offsets_host)
+{
+    if (batch_count <= 0) return (size_t)-2;
+    if (offsets_host == nullptr) return (size_t)-3;
+    Schedule S;
+    // the byte count does not depend on the class configuration except for
+    // the tile array; size it for the configuration with the most tiles
+    long long worst = 0;
+    for (int c = 0; c < kNumRaggedCfg; ++c) {
+        if (build_schedule(batch_count, offsets_host, c, S) != 0) return (size_t)-3;
+        worst = std::max(worst, S.hdr.total_bytes);
+    }
+    return (size_t)worst;
+}
+
+extern "C" int tokura_zgeev_ragged_prepare(tokurablas_t *handle, int batch_count, const int64_t *offsets_host,
+                                           void *work, size_t work_bytes, hipStream_t stream)
+{
+    if (handle == nullptr) return -1;
+    if (batch_count <= 0) return -6;
+    if (offsets_host == nullptr) return -3;
+    if (work == nullptr) return -5;
+    Schedule S;
+    const int cfg = handle->ragged_cfg;
+    if (build_schedule(batch_count, offsets_host, cfg, S) != 0) return -3;
+    if ((long long)work_bytes < S.hdr.total_bytes) return -5;
+    const size_t meta = (size_t)S.hdr.off_H;
+    // the staging buffer may still feed an earlier copy: wait for it
+    if (handle->ragged_host_stage) hipStreamSynchronize(handle->ragged_stage_stream);
+    if (handle->ragged_host_bytes < meta) {
+        if (handle->ragged_host_stage) hipHostFree(handle->ragged_host_stage);
+        handle->ragged_host_stage = nullptr;
+        handle->ragged_host_bytes = 0;
+        if (hipHostMalloc(&handle->ragged_host_stage, meta, hipHostMallocDefault) != hipSuccess) return 3;
+        handle->ragged_host_bytes = meta;
+    }
+    char *st = (char *)handle->ragged_host_stage;
+    std::memset(st, 0, meta);
+    std::memcpy(st, &S.hdr, sizeof S.hdr);
+    std::memcpy(st + S.hdr.off_tiles, S.tiles.data(), 4 * S.tiles.size());
+    std::memcpy(st + S.hdr.off_order, S.order.data(), 4LL * batch_count);
+    std::memcpy(st + S.hdr.off_n, S.nsz.data(), 4LL * batch_count);
+    std::memcpy(st + S.hdr.off_moff, S.moff.data(), 8LL * batch_count);
+    std::memcpy(st + S.hdr.off_eoff, S.eoff.data(), 8LL * batch_count);
+    std::memcpy(st + S.hdr.off_hoff, S.hoff.data(), 8LL * batch_count);
+    if (hipMemcpyAsync(work, st, meta, hipMemcpyHostToDevice, stream) != hipSuccess) return 3;
+    handle->ragged_stage_stream = stream;
+    // remember which configuration this work buffer was prepared for
+    bool found = false;
+    for (auto &p : handle->prepared)
+        if (p.work == work) { p.cfg = cfg; p.batch = batch_count; found = true; }
+    if (!found) handle->prepared.push_back({work, cfg, batch_count});
+    return 0;
+}
+
+static int prepared_cfg(tokurablas_t *h, void *work, int batch)
+{
+    for (auto &p : h->prepared)
+        if (p.work == work && p.batch == batch) return p.cfg;
+    return -1;
+}
+
+extern "C" int tokura_zgeev_ragged_gpu(tokurablas_t *handle, int batch_count, const hipDoubleComplex *A,
+                                       hipDoubleComplex *eigenvalues, void *work, char *flag, hipStream_t stream)
+{
+    if (handle == nullptr) return -1;
+    if (A == nullptr) return -3;
+    if (eigenvalues == nullptr) return -4;
+    if (work == nullptr) return -5;
+    if (batch_count <= 0) return -6;
+    if (flag == nullptr) return -7;
+    const int cfg = prepared_cfg(handle, work, batch_count);
+    if (cfg < 0) return -8;
+    hipError_t e = launch_ragged(cfg, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
+                                 handle->max_sweeps, handle->cu_count > 0 ? handle->cu_count : 1, stream);
+    return e == hipSuccess ? 0 : 2;
+}
+
+extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, const hipDoubleComplex *A,
+                                         hipDoubleComplex *eigenvalues, void *work, char *flag, hipStream_t stream)
+{
+    if (handle == nullptr) return -1;
+    if (A == nullptr) return -3;
+    if (eigenvalues == nullptr) return -4;
+    if (work == nullptr) return -5;
+    if (batch_count <= 0) return -6;
+    if (flag == nullptr) return -7;
+    const int cfg = prepared_cfg(handle, work, batch_count);
+    if (cfg < 0) return -8;
+    RaggedGraphCache &G = handle->graph;
+    const bool hit = G.exec && G.A == (const void *)A && G.eig == (void *)eigenvalues && G.work == work &&
+                     G.flag == flag && G.batch == batch_count && G.cfg == cfg && G.max_sweeps == handle->max_sweeps;
+    if (!hit) {
+        if (G.exec) hipGraphExecDestroy(G.exec);
+        if (G.graph) hipGraphDestroy(G.graph);
+        G = RaggedGraphCache();
+        hipStream_t cs;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return 3;
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            hipError_t el = launch_ragged(cfg, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
+                                          handle->max_sweeps, handle->cu_count > 0 ? handle->cu_count : 1, cs);
+            e = hipStreamEndCapture(cs, &G.graph);
+            if (el != hipSuccess) e = el;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0);
+        hipStreamDestroy(cs);
+        if (e != hipSuccess) {
+            if (G.graph) hipGraphDestroy(G.graph);
+            G = RaggedGraphCache();
+            return 2;
+        }
+        G.A = A; G.eig = eigenvalues; G.work = work; G.flag = flag; G.batch = batch_count; G.cfg = cfg;
+        G.max_sweeps = handle->max_sweeps;
+    }
+    return hipGraphLaunch(G.exec, stream) == hipSuccess ? 0 : 2;
+}
+
+extern "C" int tokuraSetRaggedConfig(tokurablas_t *handle, int cfg)
+{
+    if (handle == nullptr) return -1;
+    if (cfg < 0 || cfg >= kNumRaggedCfg) return -2;
+    handle->ragged_cfg = cfg;
+    return 0;
+}
+
+extern "C" int tokuraGetRaggedConfig(tokurablas_t *handle, int *cfg, int *lpm_a, int *lpm_b, int *lpm_c)
+{
+    if (handle == nullptr) return -1;
+    const int c = handle->ragged_cfg;
+    if (cfg) *cfg = c;
+    if (lpm_a) *lpm_a = g_cfg_lpm[c][0];
+    if (lpm_b) *lpm_b = g_cfg_lpm[c][1];
+    if (lpm_c) *lpm_c = g_cfg_lpm[c][2];
+    return 0;
+}
+
+extern "C" int tokuraAutotuneRagged(tokurablas_t *handle, int batch_count, const int64_t *offsets_host,
+                                    const hipDoubleComplex *A, hipDoubleComplex *eigenvalues, void *work,
+                                    size_t work_bytes, char *flag, hipStream_t stream)
+{
+    if (handle == nullptr) return -1;
+    const int keep = handle->ragged_cfg;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    float best = 1e30f;
+    int bestc = keep, rc = 0;
+    for (int c = 0; c < kNumRaggedCfg && rc == 0; ++c) {
+        handle->ragged_cfg = c;
+        rc = tokura_zgeev_ragged_prepare(handle, batch_count, offsets_host, work, work_bytes, stream);
+        if (rc) break;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2 && rc == 0; ++rep) {
+            hipEventRecord(e0, stream);
+            rc = tokura_zgeev_ragged_gpu(handle, batch_count, A, eigenvalues, work, flag, stream);
+            hipEventRecord(e1, stream);
+            hipEventSynchronize(e1);
+            hipEventElapsedTime(&ms, e0, e1);
+        }
+        if (rc == 0 && ms < best) { best = ms; bestc = c; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    handle->ragged_cfg = rc == 0 ? bestc : keep;
+    int rc2 = tokura_zgeev_ragged_prepare(handle, batch_count, offsets_host, work, work_bytes, stream);
+    hipStreamSynchronize(stream);
+    return rc ? rc : rc2;
+}

@@ -227,6 +227,38 @@ __device__ void hessenberg(zd *A, const FullLayout lay, int n, int t)
     }
 }
 
+// Coalesced load of matrix `src` (n*n column-major) into LDS; returns
+// whether all entries are finite (group-uniform).
+template <int LPM>
+__device__ __forceinline__ bool load_full(zd *H, const FullLayout lay, const zd *__restrict__ src, int n, int t)
+{
+    const int nn = n * n;
+    bool bad = false;
+    int r = t % n, c = t / n;
+    const int dr = LPM % n, dc = LPM / n;
+    for (int e = t; e < nn; e += LPM) {
+        zd v = src[e];
+        bad |= !zfinite(v);
+        H[lay(r, c)] = v;
+        r += dr;
+        c += dc;
+        if (r >= n) { r -= n; ++c; }
+    }
+    return !group_any<LPM>(bad);
+}
+
+// Store the upper Hessenberg part of an LDS matrix in packed order
+// (column c: rows 0..min(c+1, n-1)); consecutive lanes write consecutive rows.
+template <int LPM>
+__device__ __forceinline__ void store_packed(const zd *H, const FullLayout lay, zd *__restrict__ dst, int n, int t)
+{
+    const PackedLayout pk;
+    for (int c = 0; c < n; ++c) {
+        const int rows = (c + 2 < n ? c + 2 : n);
+        for (int r = t; r < rows; r += LPM) dst[pk(r, c)] = H[lay(r, c)];
+    }
+}
+
 // Ahues-Kressner negligibility test of H(k, k-1) (ZLAHQR loop 40).
 template <class Lay>
 __device__ __forceinline__ bool subdiag_negligible(const zd *H, const Lay &lay, int k, int n,

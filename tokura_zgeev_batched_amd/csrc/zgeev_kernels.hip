@@ -29,26 +29,6 @@ struct Geometry {
     static constexpr int S = (NB + LPM - 1) / LPM; // positions per lane
 };
 
-// Coalesced load of matrix `src` (n*n column-major) into LDS; returns
-// whether all entries are finite (group-uniform).
-template <int LPM>
-__device__ __forceinline__ bool load_full(zd *H, const FullLayout lay, const zd *__restrict__ src, int n, int t)
-{
-    const int nn = n * n;
-    bool bad = false;
-    int r = t % n, c = t / n;
-    const int dr = LPM % n, dc = LPM / n;
-    for (int e = t; e < nn; e += LPM) {
-        zd v = src[e];
-        bad |= !zfinite(v);
-        H[lay(r, c)] = v;
-        r += dr;
-        c += dc;
-        if (r >= n) { r -= n; ++c; }
-    }
-    return !group_any<LPM>(bad);
-}
-
 template <int NB, int LPM>
 __global__ __launch_bounds__(64) void k_zgeev_fused(int n, int batch, const zd *__restrict__ A,
                                                     zd *__restrict__ eig, int8_t *__restrict__ flag,
@@ -93,12 +73,7 @@ __global__ __launch_bounds__(64) void k_hessenberg_packed(int n, int batch, cons
     const bool ok = load_full<LPM>(H, lay, A + mat * (long long)n * n, n, t);
     lds_order();
     if (ok) hessenberg<LPM>(H, lay, n, t);
-    zd *dst = work + mat * (long long)packed_size(n);
-    const PackedLayout pk;
-    for (int c = 0; c < n; ++c) {
-        const int rows = (c + 2 < n ? c + 2 : n);
-        for (int r = t; r < rows; r += LPM) dst[pk(r, c)] = H[lay(r, c)];
-    }
+    store_packed<LPM>(H, lay, work + mat * (long long)packed_size(n), n, t);
     if (t == 0) flag[mat] = ok ? 0 : 2;
 }
 

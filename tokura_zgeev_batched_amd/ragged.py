@@ -1,7 +1,94 @@
-"""Ragged (mixed-size) batches through the persistent work-queue kernel."""
+"""Ragged (mixed-size) batches through the persistent work-queue kernel.
+
+Wraps tokura_get_zgeev_ragged_worksize / _prepare / _gpu / _graph and
+tokuraAutotuneRagged (include/tokura_blas.h).  The reference has no ragged
+mode (one n per call, README.md:5).
+"""
 from __future__ import annotations
 
+import ctypes
 
-class RaggedBatch:  # filled in with the ragged kernel
-    def __init__(self, *a, **k):
-        raise NotImplementedError("ragged path not built yet")
+import numpy as np
+import torch
+
+from . import Handle, TokuraError, _P, _check, _dev_check, _stream_ptr, default_handle, lib
+
+
+def pack(mats):
+    """Packs a list of square complex arrays (row/column sense) into the
+    ragged layout: (flat complex128 array, int64 offsets)."""
+    sizes = [int(m.shape[0]) for m in mats]
+    offs = np.zeros(len(mats) + 1, np.int64)
+    offs[1:] = np.cumsum(np.array(sizes, np.int64) ** 2)
+    flat = np.empty(int(offs[-1]), np.complex128)
+    for b, m in enumerate(mats):
+        flat[offs[b]:offs[b + 1]] = np.asarray(m, np.complex128).T.reshape(-1)   # column-major
+    return flat, offs
+
+
+class RaggedBatch:
+    """A prepared ragged schedule bound to one set of offsets.
+
+    ``offsets``: int64 host array (batch+1), offsets[b] is the start of matrix
+    b (column-major) in the flat complex buffer, offsets[b+1]-offsets[b] = n_b^2.
+    """
+
+    def __init__(self, offsets, handle: Handle | None = None, device="cuda", stream=None):
+        self.h = handle or default_handle()
+        self.offsets = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        self.batch = len(self.offsets) - 1
+        if self.batch <= 0:
+            raise TokuraError("empty ragged batch")
+        d = np.diff(self.offsets)
+        self.sizes = np.rint(np.sqrt(d)).astype(np.int64)
+        if np.any(self.sizes ** 2 != d) or np.any(self.sizes < 1) or np.any(self.sizes > 64):
+            raise TokuraError("offsets must describe n x n matrices with 1 <= n <= 64")
+        self.eig_offsets = np.zeros(self.batch + 1, np.int64)
+        self.eig_offsets[1:] = np.cumsum(self.sizes)
+        ws = int(lib().tokura_get_zgeev_ragged_worksize(self.batch, self.offsets.ctypes.data_as(_P)))
+        if ws >= (1 << 63):
+            raise TokuraError("tokura_get_zgeev_ragged_worksize rejected the offsets")
+        self.work = torch.empty(ws, dtype=torch.uint8, device=device)
+        self.prepare(stream)
+
+    def prepare(self, stream=None):
+        rc = lib().tokura_zgeev_ragged_prepare(self.h.ptr, self.batch, self.offsets.ctypes.data_as(_P),
+                                               _P(self.work.data_ptr()), self.work.numel(), _stream_ptr(stream))
+        _check(rc, "tokura_zgeev_ragged_prepare")
+
+    def _bufs(self, A, eig, flag):
+        _dev_check(A, "A", torch.complex128)
+        if A.numel() < int(self.offsets.max()):
+            raise TokuraError("A is smaller than the offsets require")
+        if eig is None:
+            eig = torch.empty(int(self.eig_offsets[-1]), dtype=torch.complex128, device=A.device)
+        if flag is None:
+            flag = torch.empty(self.batch, dtype=torch.int8, device=A.device)
+        return eig, flag
+
+    def solve(self, A: torch.Tensor, eig=None, flag=None, graph: bool = False, stream=None):
+        """Eigenvalues of every matrix: returns (eig flat (sum n), flag (batch,))."""
+        eig, flag = self._bufs(A, eig, flag)
+        f = lib().tokura_zgeev_ragged_graph if graph else lib().tokura_zgeev_ragged_gpu
+        rc = f(self.h.ptr, self.batch, _P(A.data_ptr()), _P(eig.data_ptr()), _P(self.work.data_ptr()),
+               _P(flag.data_ptr()), _stream_ptr(stream))
+        _check(rc, "tokura_zgeev_ragged_graph" if graph else "tokura_zgeev_ragged_gpu")
+        return eig, flag
+
+    def autotune(self, A: torch.Tensor, stream=None):
+        eig, flag = self._bufs(A, None, None)
+        rc = lib().tokuraAutotuneRagged(self.h.ptr, self.batch, self.offsets.ctypes.data_as(_P), _P(A.data_ptr()),
+                                        _P(eig.data_ptr()), _P(self.work.data_ptr()), self.work.numel(),
+                                        _P(flag.data_ptr()), _stream_ptr(stream))
+        _check(rc, "tokuraAutotuneRagged")
+        return self.config()
+
+    def config(self):
+        c, a, b, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().tokuraGetRaggedConfig(self.h.ptr, ctypes.byref(c), ctypes.byref(a), ctypes.byref(b),
+                                           ctypes.byref(d)), "tokuraGetRaggedConfig")
+        return c.value, (a.value, b.value, d.value)
+
+    def split(self, eig_flat: torch.Tensor):
+        """List of per-matrix eigenvalue tensors."""
+        return [eig_flat[self.eig_offsets[b]:self.eig_offsets[b + 1]] for b in range(self.batch)]
