@@ -1,0 +1,158 @@
+"""GPU checks of the differentiation path through the C ABI.
+
+* forward with Schur vectors: A = U T U^H, U unitary, T upper triangular,
+  diag(T) = eigenvalues (<= 1e-10 vs LAPACK zgeev), same order as the
+  eigenvalue-only kernel with matching lanes per matrix;
+* adjoint: vs the oracle's dL/dA (same method on the CPU), vs CPU
+  torch.linalg.eigvals autograd in complex128, and vs central finite
+  differences (h = 1e-6) with gradient relative error <= 1e-7.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import tokura_zgeev_batched_amd as tk
+from tokura_zgeev_batched_amd.autograd import backward_batched, eigvals_autograd, schur_batched
+
+pytestmark = pytest.mark.gpu
+
+
+def rand(rng, b, n):
+    return rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n))
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 16, 17, 32, 33, 64])
+def test_schur_forward(n):
+    rng = np.random.default_rng(300 + n)
+    M = rand(rng, 20, n)
+    A = torch.tensor(M, device="cuda").transpose(-1, -2).contiguous()
+    print(f"[case] schur n={n}", flush=True)
+    eig, schur, flag = schur_batched(A, n, 20)
+    torch.cuda.synchronize()
+    assert int(flag.abs().sum()) == 0
+    T = schur[:, 0].transpose(-1, -2).cpu().numpy()
+    U = schur[:, 1].transpose(-1, -2).cpu().numpy()
+    e = eig.cpu().numpy()
+    for b in range(20):
+        assert np.all(np.tril(T[b], -1) == 0)
+        assert np.linalg.norm(U[b].conj().T @ U[b] - np.eye(n)) < 1e-13 * n
+        assert np.linalg.norm(U[b] @ T[b] @ U[b].conj().T - M[b]) < 1e-13 * n * np.linalg.norm(M[b])
+        np.testing.assert_array_equal(np.diag(T[b]), e[b])
+    ref, _ = oracle.lapack_zgeev(M)
+    assert oracle.max_relerr(ref, e) <= 1e-10
+
+
+def test_recompute_forward_order_matches_schur():
+    rng = np.random.default_rng(5)
+    for n in (3, 16, 24, 32, 48, 64):
+        M = torch.tensor(rand(rng, 30, n), device="cuda")
+        e1, f1 = eigvals_autograd(M, mode="store")
+        e2, f2 = eigvals_autograd(M, mode="recompute")
+        assert torch.equal(e1, e2), n
+
+
+def oracle_grad_in_order(Mb, g, order_eigs):
+    """Oracle dL/dA with g given in the order of `order_eigs`."""
+    G, w = oracle.method_adjoint(Mb, np.zeros(Mb.shape[0]))
+    perm = oracle.match(order_eigs, w)   # w[perm] pairs with order_eigs
+    g_o = np.zeros_like(g)
+    g_o[perm] = g
+    G, w2 = oracle.method_adjoint(Mb, g_o)
+    return G
+
+
+@pytest.mark.parametrize("mode", ["store", "recompute"])
+@pytest.mark.parametrize("n", [2, 7, 16, 32, 45, 64])
+def test_adjoint_vs_oracle_and_torch(n, mode):
+    rng = np.random.default_rng(400 + n)
+    B = 6
+    M = rand(rng, B, n)
+    g = rng.standard_normal((B, n)) + 1j * rng.standard_normal((B, n))
+    Mt = torch.tensor(M, device="cuda", requires_grad=True)
+    print(f"[case] adjoint n={n} mode={mode}", flush=True)
+    eig, flag = eigvals_autograd(Mt, mode=mode)
+    loss = (torch.conj(torch.tensor(g, device="cuda")) * eig).real.sum()
+    loss.backward()
+    G = Mt.grad.cpu().numpy()
+    e = eig.detach().cpu().numpy()
+    for b in range(B):
+        Go = oracle_grad_in_order(M[b], g[b], e[b])
+        scale = max(1.0, np.abs(Go).max())
+        assert np.abs(G[b] - Go).max() <= 1e-9 * scale, b
+        # CPU torch autograd, complex128
+        At = torch.tensor(M[b], requires_grad=True)
+        lam = torch.linalg.eigvals(At)
+        perm = oracle.match(e[b], lam.detach().numpy())   # lam[perm] pairs with e
+        gt = torch.zeros(n, dtype=torch.complex128)
+        gt[torch.tensor(perm)] = torch.tensor(g[b])
+        (torch.conj(gt) * lam).real.sum().backward()
+        assert np.abs(At.grad.numpy() - G[b]).max() <= 1e-8 * scale, b
+
+
+def test_adjoint_vs_finite_differences():
+    # 100 sampled matrices of the n = 32 configuration, directional central
+    # differences with h = 1e-6 through the eigenvalue-only GPU kernel
+    rng = np.random.default_rng(11)
+    n, B, h = 32, 100, 1e-6
+    M = rand(rng, B, n)
+    g = rng.standard_normal((B, n)) + 1j * rng.standard_normal((B, n))
+    Mt = torch.tensor(M, device="cuda", requires_grad=True)
+    eig, flag = eigvals_autograd(Mt)
+    (torch.conj(torch.tensor(g, device="cuda")) * eig).real.sum().backward()
+    G = Mt.grad.cpu().numpy()
+    e0 = eig.detach().cpu().numpy()
+    E = rand(rng, B, n)
+    E /= np.linalg.norm(E, axis=(1, 2), keepdims=True)
+    ep, _ = tk.eigvals(torch.tensor(M + h * E, device="cuda"))
+    em, _ = tk.eigvals(torch.tensor(M - h * E, device="cuda"))
+    ep, em = ep.cpu().numpy(), em.cpu().numpy()
+    worst = 0.0
+    for b in range(B):
+        pp = oracle.match(e0[b], ep[b])
+        pm = oracle.match(e0[b], em[b])
+        Lp = np.real(np.sum(np.conj(g[b]) * ep[b][pp]))
+        Lm = np.real(np.sum(np.conj(g[b]) * em[b][pm]))
+        fd = (Lp - Lm) / (2 * h)
+        an = np.real(np.sum(np.conj(G[b]) * E[b]))
+        rel = abs(fd - an) / max(abs(an), 1e-3)
+        worst = max(worst, rel)
+    print(f"FD worst rel err {worst:.2e}", flush=True)
+    assert worst <= 1e-7
+
+
+def test_adjoint_flags_nan():
+    rng = np.random.default_rng(12)
+    M = rand(rng, 3, 9)
+    M[1, 4, 4] = np.nan
+    Mt = torch.tensor(M, device="cuda", requires_grad=True)
+    eig, flag = eigvals_autograd(Mt)
+    assert list(flag.cpu().numpy()) == [0, 2, 0]
+    (eig[[0, 2]].real.sum()).backward()
+    G = Mt.grad.cpu().numpy()
+    assert np.isnan(G[1]).all() and np.isfinite(G[[0, 2]]).all()
+
+
+def test_diff_config_sample_full_size():
+    # 50,000 matrices n = 32: forward with Schur vectors + adjoint; a sample
+    # checked against the oracle
+    n, B = 32, 50000
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    M = torch.complex(torch.randn(B, n, n, generator=gen, device="cuda", dtype=torch.float64),
+                      torch.randn(B, n, n, generator=gen, device="cuda", dtype=torch.float64))
+    gv = torch.complex(torch.randn(B, n, generator=gen, device="cuda", dtype=torch.float64),
+                       torch.randn(B, n, generator=gen, device="cuda", dtype=torch.float64))
+    M.requires_grad_(True)
+    print("[case] diff config 50000 x 32", flush=True)
+    eig, flag = eigvals_autograd(M)
+    (torch.conj(gv) * eig).real.sum().backward()
+    torch.cuda.synchronize()
+    assert int(flag.abs().sum()) == 0
+    idx = list(range(0, B, 2500))
+    Mh = M.detach()[idx].cpu().numpy()
+    Gh = M.grad[idx].cpu().numpy()
+    eh = eig.detach()[idx].cpu().numpy()
+    gh = gv[idx].cpu().numpy()
+    for k in range(len(idx)):
+        Go = oracle_grad_in_order(Mh[k], gh[k], eh[k])
+        assert np.abs(Gh[k] - Go).max() <= 1e-9 * max(1.0, np.abs(Go).max())

@@ -40,18 +40,6 @@ def test_every_size_default_config(handle, n):
     assert oracle.max_relerr(me, e) <= TOL_METHOD
 
 
-def variants_for(n):
-    import ctypes
-    out = []
-    for v in range(64):
-        h = tk.Handle()
-        rc = tk.lib().tokuraSetVariant(h.ptr, n, v)
-        h.close()
-        if rc == 0:
-            out.append(v)
-    return out
-
-
 @pytest.mark.parametrize("n", [1, 3, 4, 5, 8, 9, 16, 17, 31, 32, 33, 48, 49, 64])
 def test_every_variant(n):
     rng = np.random.default_rng(2000 + n)

@@ -116,7 +116,7 @@ __device__ __forceinline__ void qr_tile(zd *lds, const RaggedView &V, int4 td, z
     lds_order();
     const PackedLayout pk;
     int info = 0;
-    if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax, nullptr);
+    if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
     for (int r = t; r < n; r += LPM) out[r] = H[pk(r, r)];
     if (t == 0) flag[b] = info != 0 ? 1 : 0;
     lds_order();

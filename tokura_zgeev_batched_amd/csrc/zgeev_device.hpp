@@ -154,8 +154,11 @@ __device__ __forceinline__ void zlarfg2(zd x1, zd x2, zd &t1, zd &v2, double &be
 // ---- ZGEHD2 on an n x n matrix in LDS (full layout), LPM lanes ----
 // Entries below the sub-diagonal are zeroed (LAPACK stores the reflectors
 // there; nothing later reads them).
-template <int LPM>
-__device__ void hessenberg(zd *A, const FullLayout lay, int n, int t)
+// With KEEPV the reflector vectors stay below the sub-diagonal (LAPACK's
+// storage, v_0 = 1 implied) and taus[i] receives tau of step i, so that
+// form_q can accumulate Q = H(0) H(1) ... H(n-2).
+template <int LPM, bool KEEPV = false>
+__device__ void hessenberg(zd *A, const FullLayout lay, int n, int t, zd *taus = nullptr)
 {
     for (int i = 0; i + 2 <= n; ++i) {
         const int m = n - 1 - i;      // reflector length, rows i+1..n-1
@@ -181,7 +184,12 @@ __device__ void hessenberg(zd *A, const FullLayout lay, int n, int t)
                 xnorm = amx * sqrt(group_sum<LPM>(s2));
             }
         }
-        if (xnorm == 0.0 && alpha.im == 0.0) continue;   // tau = 0, H(i) = I
+        if (xnorm == 0.0 && alpha.im == 0.0) {   // tau = 0, H(i) = I
+            if constexpr (KEEPV) {
+                if (t == 0) taus[i] = mk(0.0, 0.0);
+            }
+            continue;
+        }
         // beta = -sign(alpha_r) * |(alpha, xnorm)|  (DLAPY3 with scaling)
         double w = fmax(fmax(fabs(alpha.re), fabs(alpha.im)), xnorm);
         double iw = 1.0 / w;
@@ -222,9 +230,40 @@ __device__ void hessenberg(zd *A, const FullLayout lay, int n, int t)
         }
         lds_order();
         if (t == 0) A[lay(i + 1, i)] = mk(beta, 0.0);
-        for (int r = i + 2 + t; r < n; r += LPM) A[lay(r, i)] = mk(0.0, 0.0);
+        if constexpr (KEEPV) {
+            if (t == 0) taus[i] = tau;
+        } else {
+            for (int r = i + 2 + t; r < n; r += LPM) A[lay(r, i)] = mk(0.0, 0.0);
+        }
         lds_order();
     }
+}
+
+// Q = H(0) H(1) ... H(n-2) from the reflectors kept by hessenberg<LPM, true>
+// (LAPACK ZUNGHR), accumulated backwards: Q <- H(i) Q acts on rows and
+// columns i+1..n-1 only.  Afterwards the reflector storage in A is zeroed.
+template <int LPM>
+__device__ void form_q(zd *Q, const FullLayout ql, zd *A, const FullLayout lay, const zd *taus, int n, int t)
+{
+    for (int c = t; c < n; c += LPM)
+        for (int r = 0; r < n; ++r) Q[ql(r, c)] = mk(r == c ? 1.0 : 0.0, 0.0);
+    lds_order();
+    for (int i = n - 2; i >= 0; --i) {
+        const zd tau = taus[i];
+        if (tau.re == 0.0 && tau.im == 0.0) continue;
+        const int m = n - 1 - i;
+        for (int c = i + 1 + t; c < n; c += LPM) {
+            zd s = Q[ql(i + 1, c)];
+            for (int r = 1; r < m; ++r) s = zadd(s, zmulc(A[lay(i + 1 + r, i)], Q[ql(i + 1 + r, c)]));
+            s = zmul(tau, s);
+            Q[ql(i + 1, c)] = zsub(Q[ql(i + 1, c)], s);
+            for (int r = 1; r < m; ++r) Q[ql(i + 1 + r, c)] = zsub(Q[ql(i + 1 + r, c)], zmul(A[lay(i + 1 + r, i)], s));
+        }
+        lds_order();
+    }
+    for (int c = 0; c + 2 < n; ++c)
+        for (int r = c + 2 + t; r < n; r += LPM) A[lay(r, c)] = mk(0.0, 0.0);
+    lds_order();
 }
 
 // Coalesced load of matrix `src` (n*n column-major) into LDS; returns
@@ -299,16 +338,26 @@ __device__ __forceinline__ int group_last_true(int t, int lo, int hi, int none, 
     return found;
 }
 
-// ---- ZLAHQR, eigenvalues only (WANTT = WANTZ = false), ilo = 0, ihi = n-1 ----
-// Returns 0 or 1 + the 0-based index of the first eigenvalue that did not
-// converge (LAPACK INFO).  On return the eigenvalues are H(i,i).
-template <int LPM, int S, class Lay>
-__device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_override, int *nsweeps)
+// ---- ZLAHQR with ilo = 0, ihi = n-1 ----
+// WANTT = false: eigenvalues only (transformations restricted to the active
+// block [l, i], as ZLAHQR with WANTT = WANTZ = false).  WANTT = true: the full
+// Schur form T (rows 0..min(k+2,i), columns k..n-1 per step) and, when Z is
+// non-null, Z <- Z * G for every reflector (WANTZ).  Returns 0 or 1 + the
+// 0-based index of the first eigenvalue that did not converge (LAPACK INFO).
+// On return the eigenvalues are H(i,i).
+struct NoZ {
+    __device__ __forceinline__ int operator()(int, int) const { return 0; }
+};
+
+template <int LPM, int S, bool WANTT, class Lay, class ZLay = NoZ>
+__device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_override, zd *Z = nullptr,
+                           const ZLay zlay = ZLay())
 {
     const double ulp = DBL_EPSILON, safmin = DBL_MIN;
     const double smlnum = safmin * ((double)n / ulp);
     const int itmax = itmax_override >= 0 ? itmax_override : 30 * (n > 10 ? n : 10);
-    int kdefl = 0, sweeps = 0;
+    const bool wantz = WANTT && Z != nullptr;
+    int kdefl = 0;
     int i = n - 1;
     while (i >= 1) {
         int l = 0;
@@ -323,6 +372,7 @@ __device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_overrid
             lds_order();
             if (l >= i) { conv = true; break; }
             ++kdefl;
+            const int I1 = WANTT ? 0 : l, I2 = WANTT ? n - 1 : i;
             // ---- shift
             zd sh;
             if (kdefl % 20 == 0) {
@@ -368,16 +418,20 @@ __device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_overrid
                 x1 = mk(h11s.re / s, h11s.im / s);
                 x2 = mk(h21 / s, 0.0);
             }
-            ++sweeps;
-            // ---- single-shift sweep k = m .. i-1 over the active block [l, i]
-            zd carry[S], ypre[S];
+            // ---- single-shift sweep k = m .. i-1; H rows I1.., columns ..I2
+            zd carry[S], ypre[S], zc[S], zy[S];
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const int p = t + s * LPM;
                 carry[s] = mk(0.0, 0.0);
                 ypre[s] = mk(0.0, 0.0);
-                if (p >= m + 2 && p <= i) { carry[s] = H[lay(m, p)]; ypre[s] = H[lay(m + 1, p)]; }
-                else if (p >= l && p <= m - 1) { carry[s] = H[lay(p, m)]; ypre[s] = H[lay(p, m + 1)]; }
+                if (p >= m + 2 && p <= I2) { carry[s] = H[lay(m, p)]; ypre[s] = H[lay(m + 1, p)]; }
+                else if (p >= I1 && p <= m - 1) { carry[s] = H[lay(p, m)]; ypre[s] = H[lay(p, m + 1)]; }
+                if constexpr (WANTT) {
+                    zc[s] = mk(0.0, 0.0);
+                    zy[s] = mk(0.0, 0.0);
+                    if (wantz && p < n) { zc[s] = Z[zlay(p, m)]; zy[s] = Z[zlay(p, m + 1)]; }
+                }
             }
             zd a = H[lay(m, m)], b = H[lay(m, m + 1)], c = H[lay(m + 1, m)], d = H[lay(m + 1, m + 1)];
             zd e = (m + 2 <= i) ? H[lay(m + 2, m + 1)] : mk(0.0, 0.0);
@@ -387,12 +441,12 @@ __device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_overrid
                 bool ident;
                 zlarfg2(x1, x2, t1, v2, beta, ident);
                 const double t2 = t1.re * v2.re - t1.im * v2.im;   // DBLE(T1*V2)
-                // bulk: left on columns p >= kk+2, right on rows l <= p <= kk-1
+                // bulk: left on columns kk+2 <= p <= I2, right on rows I1 <= p <= kk-1
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
                     const int p = t + s * LPM;
-                    const bool rowop = (p >= kk + 2 && p <= i);
-                    const bool colop = (p >= l && p <= kk - 1);
+                    const bool rowop = (p >= kk + 2 && p <= I2);
+                    const bool colop = (p >= I1 && p <= kk - 1);
                     if (rowop || colop) {
                         zd c1 = rowop ? zconj(t1) : t1;
                         zd vv = rowop ? v2 : zconj(v2);
@@ -407,6 +461,13 @@ __device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_overrid
                             H[lay(p, kk)] = x;
                         }
                         carry[s] = y;
+                    }
+                    if constexpr (WANTT) {
+                        if (wantz && p < n) {
+                            zd sm = zadd(zmul(t1, zc[s]), zscale(t2, zy[s]));
+                            Z[zlay(p, kk)] = zsub(zc[s], sm);
+                            zc[s] = zsub(zy[s], zmul(sm, zconj(v2)));
+                        }
                     }
                 }
                 // window: rows/cols kk, kk+1 and row kk+2
@@ -441,7 +502,10 @@ __device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_overrid
 #pragma unroll
                     for (int s = 0; s < S; ++s) {
                         const int p = t + s * LPM;
-                        if (p >= l && p <= m - 1) H[lay(p, m + 1)] = carry[s];
+                        if (p >= I1 && p <= m - 1) H[lay(p, m + 1)] = carry[s];
+                        if constexpr (WANTT) {
+                            if (wantz && p < n) Z[zlay(p, m + 1)] = zc[s];
+                        }
                     }
                     if (t == 0) {
                         H[lay(m, m + 1)] = b;
@@ -460,11 +524,17 @@ __device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_overrid
                     lds_order();
                     for (int j = m + t; j <= i; j += LPM)
                         if (j != m + 1)
-                            for (int cc = j + 1; cc <= i; ++cc) H[lay(j, cc)] = zmul(H[lay(j, cc)], temp);
+                            for (int cc = j + 1; cc <= I2; ++cc) H[lay(j, cc)] = zmul(H[lay(j, cc)], temp);
                     lds_order();
                     for (int j = m + t; j <= i; j += LPM)
                         if (j != m + 1)
-                            for (int r = l; r <= j - 1; ++r) H[lay(r, j)] = zmul(H[lay(r, j)], tempc);
+                            for (int r = I1; r <= j - 1; ++r) H[lay(r, j)] = zmul(H[lay(r, j)], tempc);
+                    if constexpr (WANTT) {
+                        if (wantz)
+                            for (int j = m; j <= i; ++j)
+                                if (j != m + 1)
+                                    for (int r = t; r < n; r += LPM) Z[zlay(r, j)] = zmul(Z[zlay(r, j)], tempc);
+                    }
                     lds_order();
                     const int k1 = kk + 1;
                     if (k1 < i) {
@@ -478,8 +548,11 @@ __device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_overrid
 #pragma unroll
                         for (int s = 0; s < S; ++s) {
                             const int p = t + s * LPM;
-                            if (p >= k1 + 2 && p <= i) { carry[s] = H[lay(k1, p)]; ypre[s] = H[lay(k1 + 1, p)]; }
-                            else if (p >= l && p <= k1 - 1) { carry[s] = H[lay(p, k1)]; ypre[s] = H[lay(p, k1 + 1)]; }
+                            if (p >= k1 + 2 && p <= I2) { carry[s] = H[lay(k1, p)]; ypre[s] = H[lay(k1 + 1, p)]; }
+                            else if (p >= I1 && p <= k1 - 1) { carry[s] = H[lay(p, k1)]; ypre[s] = H[lay(p, k1 + 1)]; }
+                            if constexpr (WANTT) {
+                                if (wantz && p < n) { zc[s] = Z[zlay(p, k1)]; zy[s] = Z[zlay(p, k1 + 1)]; }
+                            }
                         }
                     } else {
                         c = H[lay(i, i - 1)];
@@ -487,7 +560,11 @@ __device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_overrid
 #pragma unroll
                         for (int s = 0; s < S; ++s) {
                             const int p = t + s * LPM;
-                            if (p >= l && p <= i - 1) carry[s] = H[lay(p, i)];
+                            if (p >= I1 && p <= i - 1) carry[s] = H[lay(p, i)];
+                            else if (p >= i + 1 && p <= I2) carry[s] = H[lay(i, p)];
+                            if constexpr (WANTT) {
+                                if (wantz && p < n) zc[s] = Z[zlay(p, i)];
+                            }
                         }
                     }
                     continue;
@@ -504,8 +581,11 @@ __device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_overrid
 #pragma unroll
                     for (int s = 0; s < S; ++s) {
                         const int p = t + s * LPM;
-                        if (p >= k1 + 2 && p <= i) ypre[s] = H[lay(k1 + 1, p)];
-                        else if (p >= l && p <= k1 - 1) ypre[s] = H[lay(p, k1 + 1)];
+                        if (p >= k1 + 2 && p <= I2) ypre[s] = H[lay(k1 + 1, p)];
+                        else if (p >= I1 && p <= k1 - 1) ypre[s] = H[lay(p, k1 + 1)];
+                        if constexpr (WANTT) {
+                            if (wantz && p < n) zy[s] = Z[zlay(p, k1 + 1)];
+                        }
                     }
                 }
             }
@@ -521,7 +601,11 @@ __device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_overrid
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const int p = t + s * LPM;
-                if (p >= l && p <= i - 1) H[lay(p, i)] = rot ? zmul(carry[s], temp) : carry[s];
+                if (p >= I1 && p <= i - 1) H[lay(p, i)] = rot ? zmul(carry[s], temp) : carry[s];
+                if constexpr (WANTT) {
+                    if (p >= i + 1 && p <= I2) H[lay(i, p)] = rot ? zmul(carry[s], zconj(temp)) : carry[s];
+                    if (wantz && p < n) Z[zlay(p, i)] = rot ? zmul(zc[s], temp) : zc[s];
+                }
             }
             if (t == 0) {
                 H[lay(i, i - 1)] = hsub;
@@ -529,15 +613,17 @@ __device__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_overrid
             }
             lds_order();
         }
-        if (!conv) {
-            if (nsweeps) *nsweeps = sweeps;
-            return i + 1;
-        }
+        if (!conv) return i + 1;
         kdefl = 0;
         i = l - 1;
     }
-    if (nsweeps) *nsweeps = sweeps;
     return 0;
+}
+
+template <int LPM, int S, class Lay>
+__device__ __forceinline__ int zlahqr_eigs(zd *H, const Lay lay, int n, int t, int itmax_override)
+{
+    return zlahqr_core<LPM, S, false>(H, lay, n, t, itmax_override);
 }
 
 }  // namespace tkz

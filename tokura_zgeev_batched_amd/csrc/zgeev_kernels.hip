@@ -52,7 +52,7 @@ __global__ __launch_bounds__(64) void k_zgeev_fused(int n, int batch, const zd *
     }
     hessenberg<LPM>(H, lay, n, t);
     int info = 0;
-    if (n > 1) info = zlahqr_eigs<LPM, G::S>(H, lay, n, t, itmax, nullptr);
+    if (n > 1) info = zlahqr_eigs<LPM, G::S>(H, lay, n, t, itmax);
     for (int r = t; r < n; r += LPM) out[r] = H[lay(r, r)];
     if (t == 0) flag[mat] = info != 0 ? 1 : 0;
 }
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
     lds_order();
     const PackedLayout pk;
     int info = 0;
-    if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax, nullptr);
+    if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
     for (int r = t; r < n; r += LPM) out[r] = H[pk(r, r)];
     if (t == 0) flag[mat] = info != 0 ? 1 : 0;
 }
