@@ -49,7 +49,8 @@ def test_recompute_forward_order_matches_schur():
         M = torch.tensor(rand(rng, 30, n), device="cuda")
         e1, f1 = eigvals_autograd(M, mode="store")
         e2, f2 = eigvals_autograd(M, mode="recompute")
-        assert torch.equal(e1, e2), n
+        # same arithmetic in both kernels: identical values in identical order
+        assert torch.equal(e1, e2), (n, float((e1 - e2).abs().max()))
 
 
 def oracle_grad_in_order(Mb, g, order_eigs):
@@ -90,9 +91,21 @@ def test_adjoint_vs_oracle_and_torch(n, mode):
         assert np.abs(At.grad.numpy() - G[b]).max() <= 1e-8 * scale, b
 
 
+def _loss_fd(M, g, e0, E, h, handle=None):
+    """L(M + h E) per matrix, eigenvalues paired with e0 by optimal matching."""
+    ep, _ = tk.eigvals(torch.tensor(M + h * E, device="cuda"), handle)
+    ep = ep.cpu().numpy()
+    out = np.zeros(M.shape[0])
+    for b in range(M.shape[0]):
+        pp = oracle.match(e0[b], ep[b])
+        out[b] = np.real(np.sum(np.conj(g[b]) * ep[b][pp]))
+    return out
+
+
 def test_adjoint_vs_finite_differences():
-    # 100 sampled matrices of the n = 32 configuration, directional central
-    # differences with h = 1e-6 through the eigenvalue-only GPU kernel
+    # 100 sampled matrices of the n = 32 configuration; for each a random unit
+    # direction E.  Gradient relative error = |FD - <G, E>| / ||G||_F, with FD
+    # the central difference at h = 1e-6 through the eigenvalue-only kernel.
     rng = np.random.default_rng(11)
     n, B, h = 32, 100, 1e-6
     M = rand(rng, B, n)
@@ -104,21 +117,19 @@ def test_adjoint_vs_finite_differences():
     e0 = eig.detach().cpu().numpy()
     E = rand(rng, B, n)
     E /= np.linalg.norm(E, axis=(1, 2), keepdims=True)
-    ep, _ = tk.eigvals(torch.tensor(M + h * E, device="cuda"))
-    em, _ = tk.eigvals(torch.tensor(M - h * E, device="cuda"))
-    ep, em = ep.cpu().numpy(), em.cpu().numpy()
-    worst = 0.0
-    for b in range(B):
-        pp = oracle.match(e0[b], ep[b])
-        pm = oracle.match(e0[b], em[b])
-        Lp = np.real(np.sum(np.conj(g[b]) * ep[b][pp]))
-        Lm = np.real(np.sum(np.conj(g[b]) * em[b][pm]))
-        fd = (Lp - Lm) / (2 * h)
-        an = np.real(np.sum(np.conj(G[b]) * E[b]))
-        rel = abs(fd - an) / max(abs(an), 1e-3)
-        worst = max(worst, rel)
-    print(f"FD worst rel err {worst:.2e}", flush=True)
-    assert worst <= 1e-7
+    an = np.real(np.sum(np.conj(G) * E, axis=(1, 2)))
+    gnorm = np.linalg.norm(G, axis=(1, 2))
+    fd2 = (_loss_fd(M, g, e0, E, h) - _loss_fd(M, g, e0, E, -h)) / (2 * h)
+    err2 = np.abs(fd2 - an) / gnorm
+    # fourth-order stencil at a larger step: truncation O(h^4), less round-off
+    h4 = 1e-3
+    fd4 = (-_loss_fd(M, g, e0, E, 2 * h4) + 8 * _loss_fd(M, g, e0, E, h4)
+           - 8 * _loss_fd(M, g, e0, E, -h4) + _loss_fd(M, g, e0, E, -2 * h4)) / (12 * h4)
+    err4 = np.abs(fd4 - an) / gnorm
+    print(f"FD gradient rel err: central h=1e-6 max {err2.max():.2e} median {np.median(err2):.2e}; "
+          f"4th-order h=1e-3 max {err4.max():.2e} median {np.median(err4):.2e}", flush=True)
+    assert err2.max() <= 1e-7
+    assert err4.max() <= 1e-8
 
 
 def test_adjoint_flags_nan():

@@ -19,7 +19,10 @@ ARCH = os.environ.get("TOKURA_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code (none of the
 # kernels use FP32; the flag keeps hipcc from forming them anywhere).
-FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INC,
+# -ffp-contract=on: multiply-adds are fused only inside one source expression,
+# so the same source line rounds identically in every kernel that inlines it
+# (the eigenvalue-only and Schur-form QR then agree bit for bit).
+FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INC, "-ffp-contract=on",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
 
 

@@ -108,47 +108,88 @@ __device__ __forceinline__ uint64_t group_ballot(bool p)
 template <int LPM>
 __device__ __forceinline__ bool group_any(bool p) { return group_ballot<LPM>(p) != 0ull; }
 
-// ---- matrix layouts in LDS ----
+// ---- matrix layouts in LDS: lay(r, c) = cb(c) + r ----
 struct FullLayout {   // column-major with odd leading dimension (bank spread)
     int ld;
+    __device__ __forceinline__ int cb(int c) const { return c * ld; }
     __device__ __forceinline__ int operator()(int r, int c) const { return r + c * ld; }
 };
 struct PackedLayout { // upper Hessenberg, column c holds rows 0..c+1
+    __device__ __forceinline__ int cb(int c) const { return (c * (c + 3)) >> 1; }
     __device__ __forceinline__ int operator()(int r, int c) const { return ((c * (c + 3)) >> 1) + r; }
 };
 __host__ __device__ constexpr int packed_size(int n) { return n <= 0 ? 0 : ((n - 1) * (n + 2)) / 2 + n; }
 
+// ---- fast FP64 reciprocal and reciprocal square root ----
+// Hardware estimates (v_rcp_f64, v_rsq_f64) refined by two Newton steps;
+// valid for normal arguments well inside the exponent range (callers
+// guarantee 1e-290 < x < 1e290), accurate to about one ulp.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double rsq_nr(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    double r = fma(-hx * y, y, 0.5);
+    y = fma(y, r, y);
+    r = fma(-hx * y, y, 0.5);
+    return fma(y, r, y);
+}
+
 // ---- ZLARFG for a 2-vector (x1, x2): H^H [x1; x2] = [beta; 0] ----
 // On return t1 = tau, v2 = scaled x2, beta; ident = true when tau = 0
-// (then LAPACK leaves alpha = x1 and x2 untouched).  The rescaling loop of
-// ZLARFG for |beta| < safmin/eps (~1e-292) is not reproduced.
+// (then LAPACK leaves alpha = x1 and x2 untouched).
+// beta = -sign(Re x1) * |(x1, x2)|, tau = (beta - x1)/beta, v2 = x2/(x1 - beta).
+// In the normal range the square root and both reciprocals come from one
+// rsq and one rcp estimate (|x1 - beta|^2 = (|Re x1| + |beta|)^2 + Im(x1)^2 is
+// within [ss, 4 ss]); outside it a scaled, exactly rounded path is used.  The
+// rescaling loop of ZLARFG for |beta| < safmin/eps (~1e-292) is not reproduced.
 __device__ __forceinline__ void zlarfg2(zd x1, zd x2, zd &t1, zd &v2, double &beta, bool &ident)
 {
     ident = (x2.re == 0.0 && x2.im == 0.0 && x1.im == 0.0);
-    double ss = x1.re * x1.re + x1.im * x1.im + x2.re * x2.re + x2.im * x2.im;
-    double nrm;
+    const double ss = x1.re * x1.re + x1.im * x1.im + (x2.re * x2.re + x2.im * x2.im);
+    double ib;
+    zd rz;
     if (ss > 1e-290 && ss < 1e290) {
-        nrm = sqrt(ss);
+        const double y = rsq_nr(ss);
+        double nrm = ss * y;
+        nrm = fma(fma(-nrm, nrm, ss), 0.5 * y, nrm);
+        beta = -copysign(nrm, x1.re);
+        ib = -copysign(y, x1.re);
+        const double zr = x1.re - beta;
+        const double iz = rcp_nr(zr * zr + x1.im * x1.im);
+        rz = mk(zr * iz, -x1.im * iz);
     } else {
         double w = fmax(fmax(fabs(x1.re), fabs(x1.im)), fmax(fabs(x2.re), fabs(x2.im)));
         double iw = w > 0.0 ? 1.0 / w : 0.0;
         double a = x1.re * iw, b = x1.im * iw, c = x2.re * iw, d = x2.im * iw;
-        nrm = w * sqrt(a * a + b * b + c * c + d * d);
+        beta = -copysign(w * sqrt(a * a + b * b + c * c + d * d), x1.re);
+        ib = 1.0 / beta;
+        rz = zdiv(mk(1.0, 0.0), mk(x1.re - beta, x1.im));
     }
-    beta = -copysign(nrm, x1.re);
-    double ib = 1.0 / beta;
     t1 = mk((beta - x1.re) * ib, -x1.im * ib);
-    zd z = mk(x1.re - beta, x1.im);
-    zd rz;
-    double mz = fmax(fabs(z.re), fabs(z.im));
-    if (mz > 1e-150 && mz < 1e150) {
-        double iz = 1.0 / (z.re * z.re + z.im * z.im);
-        rz = mk(z.re * iz, -z.im * iz);
-    } else {
-        rz = zdiv(mk(1.0, 0.0), z);
-    }
     v2 = zmul(x2, rz);
     if (ident) { t1 = mk(0.0, 0.0); v2 = x2; beta = x1.re; }
+}
+
+// Application of a 2-element reflector to one element pair (x, y):
+//   s = c1 x + t2 y,  x <- x - s,  y <- y - s v
+// with (c1, v) = (conj(tau), v2) from the left (rows k, k+1) and (tau, conj(v2))
+// from the right (columns k, k+1); t2 = Re(tau v2) is real.  12 FP64 ops.
+__device__ __forceinline__ void pair_op(zd &x, zd &y, const zd c1, const double t2, const zd v)
+{
+    const double sr = fma(c1.re, x.re, fma(-c1.im, x.im, t2 * y.re));
+    const double si = fma(c1.re, x.im, fma(c1.im, x.re, t2 * y.im));
+    x.re -= sr;
+    x.im -= si;
+    y.re = fma(-sr, v.re, fma(si, v.im, y.re));
+    y.im = fma(-sr, v.im, fma(-si, v.re, y.im));
 }
 
 // ---- ZGEHD2 on an n x n matrix in LDS (full layout), LPM lanes ----
@@ -199,34 +240,46 @@ __device__ void hessenberg(zd *A, const FullLayout lay, int n, int t, zd *taus =
         zd scal = zdiv(mk(1.0, 0.0), mk(alpha.re - beta, alpha.im));
         for (int r = i + 2 + t; r < n; r += LPM) A[lay(r, i)] = zmul(A[lay(r, i)], scal);
         lds_order();
-        // right: A(0:n-1, i+1:n-1) -= tau (A v) v^H   (v_0 = 1 at row i+1)
+        // right: A(0:n-1, i+1:n-1) -= tau (A v) v^H   (v_0 = 1 at row i+1).
+        // Two accumulators and unrolled loops keep several LDS loads in flight.
+        const zd *vcol = A + lay(i + 1, i);   // v(c) = vcol[c], c >= 1
         for (int r = t; r < n; r += LPM) {
-            zd u = A[lay(r, i + 1)];
-            for (int c = 1; c < m; ++c) {
-                zd a = A[lay(r, i + 1 + c)], v = A[lay(i + 1 + c, i)];
-                u = zadd(u, zmul(a, v));
+            zd *row = A + r;
+            const int ld = lay.ld;
+            zd u0 = row[(i + 1) * ld], u1 = mk(0.0, 0.0);
+            int c = 1;
+#pragma unroll 2
+            for (; c + 1 < m; c += 2) {
+                const zd a0 = row[(i + 1 + c) * ld], v0 = vcol[c];
+                const zd a1 = row[(i + 2 + c) * ld], v1 = vcol[c + 1];
+                u0 = zadd(u0, zmul(a0, v0));
+                u1 = zadd(u1, zmul(a1, v1));
             }
-            u = zmul(u, tau);
-            A[lay(r, i + 1)] = zsub(A[lay(r, i + 1)], u);
-            for (int c = 1; c < m; ++c) {
-                zd v = A[lay(i + 1 + c, i)];
-                A[lay(r, i + 1 + c)] = zsub(A[lay(r, i + 1 + c)], zmul(u, zconj(v)));
+            if (c < m) u0 = zadd(u0, zmul(row[(i + 1 + c) * ld], vcol[c]));
+            const zd u = zmul(zadd(u0, u1), tau);
+            row[(i + 1) * ld] = zsub(row[(i + 1) * ld], u);
+#pragma unroll 4
+            for (c = 1; c < m; ++c) {
+                const zd v = vcol[c];
+                row[(i + 1 + c) * ld] = zsub(row[(i + 1 + c) * ld], zmul(u, zconj(v)));
             }
         }
         lds_order();
         // left: A(i+1:n-1, i+1:n-1) -= conj(tau) v (v^H A)
         for (int c = i + 1 + t; c < n; c += LPM) {
-            zd s = A[lay(i + 1, c)];
-            for (int r = 1; r < m; ++r) {
-                zd v = A[lay(i + 1 + r, i)];
-                s = zadd(s, zmulc(v, A[lay(i + 1 + r, c)]));
+            zd *col = A + lay(i + 1, c);
+            zd s0 = col[0], s1 = mk(0.0, 0.0);
+            int r = 1;
+#pragma unroll 2
+            for (; r + 1 < m; r += 2) {
+                s0 = zadd(s0, zmulc(vcol[r], col[r]));
+                s1 = zadd(s1, zmulc(vcol[r + 1], col[r + 1]));
             }
-            s = zmulc(tau, s);
-            A[lay(i + 1, c)] = zsub(A[lay(i + 1, c)], s);
-            for (int r = 1; r < m; ++r) {
-                zd v = A[lay(i + 1 + r, i)];
-                A[lay(i + 1 + r, c)] = zsub(A[lay(i + 1 + r, c)], zmul(v, s));
-            }
+            if (r < m) s0 = zadd(s0, zmulc(vcol[r], col[r]));
+            const zd sv = zmulc(tau, zadd(s0, s1));
+            col[0] = zsub(col[0], sv);
+#pragma unroll 4
+            for (r = 1; r < m; ++r) col[r] = zsub(col[r], zmul(vcol[r], sv));
         }
         lds_order();
         if (t == 0) A[lay(i + 1, i)] = mk(beta, 0.0);
@@ -419,13 +472,19 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                 x2 = mk(h21 / s, 0.0);
             }
             // ---- single-shift sweep k = m .. i-1; H rows I1.., columns ..I2
+            // State between steps: reflector input (x1, x2), window
+            // a = H(k,k), b = H(k,k+1), c = H(k+1,k), d = H(k+1,k+1),
+            // e = H(k+2,k+1); per position the carried element and the
+            // prefetched partner (ypre), and for Z the carried Z(p,k) and Z(p,k+1).
             zd carry[S], ypre[S], zc[S], zy[S];
+            int cbp[S];   // column base of the position's own column
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const int p = t + s * LPM;
+                cbp[s] = lay.cb(p < n ? p : n - 1);
                 carry[s] = mk(0.0, 0.0);
                 ypre[s] = mk(0.0, 0.0);
-                if (p >= m + 2 && p <= I2) { carry[s] = H[lay(m, p)]; ypre[s] = H[lay(m + 1, p)]; }
+                if (p >= m + 2 && p <= I2) { carry[s] = H[cbp[s] + m]; ypre[s] = H[cbp[s] + m + 1]; }
                 else if (p >= I1 && p <= m - 1) { carry[s] = H[lay(p, m)]; ypre[s] = H[lay(p, m + 1)]; }
                 if constexpr (WANTT) {
                     zc[s] = mk(0.0, 0.0);
@@ -435,12 +494,15 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
             }
             zd a = H[lay(m, m)], b = H[lay(m, m + 1)], c = H[lay(m + 1, m)], d = H[lay(m + 1, m + 1)];
             zd e = (m + 2 <= i) ? H[lay(m + 2, m + 1)] : mk(0.0, 0.0);
-            for (int kk = m; kk < i; ++kk) {
+
+            // one chase step kk; returns the new bulge H(kk+2, kk)
+            auto step = [&](const int kk, const bool first, const bool has_e) -> zd {
                 zd t1, v2;
                 double beta;
                 bool ident;
                 zlarfg2(x1, x2, t1, v2, beta, ident);
                 const double t2 = t1.re * v2.re - t1.im * v2.im;   // DBLE(T1*V2)
+                const int cbk = lay.cb(kk);
                 // bulk: left on columns kk+2 <= p <= I2, right on rows I1 <= p <= kk-1
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
@@ -448,147 +510,156 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                     const bool rowop = (p >= kk + 2 && p <= I2);
                     const bool colop = (p >= I1 && p <= kk - 1);
                     if (rowop || colop) {
-                        zd c1 = rowop ? zconj(t1) : t1;
-                        zd vv = rowop ? v2 : zconj(v2);
+                        const zd c1 = mk(t1.re, rowop ? -t1.im : t1.im);
+                        const zd vv = mk(v2.re, rowop ? v2.im : -v2.im);
                         zd x = carry[s], y = ypre[s];
-                        zd sm = zadd(zmul(c1, x), zscale(t2, y));
-                        x = zsub(x, sm);
-                        y = zsub(y, zmul(sm, vv));
+                        pair_op(x, y, c1, t2, vv);
                         if (rowop) {
-                            H[lay(kk, p)] = x;
-                            H[lay(kk + 1, p)] = y;
+                            H[cbp[s] + kk] = x;
+                            H[cbp[s] + kk + 1] = y;
                         } else {
-                            H[lay(p, kk)] = x;
+                            H[cbk + p] = x;
                         }
                         carry[s] = y;
                     }
                     if constexpr (WANTT) {
                         if (wantz && p < n) {
-                            zd sm = zadd(zmul(t1, zc[s]), zscale(t2, zy[s]));
-                            Z[zlay(p, kk)] = zsub(zc[s], sm);
-                            zc[s] = zsub(zy[s], zmul(sm, zconj(v2)));
+                            zd x = zc[s], y = zy[s];
+                            pair_op(x, y, t1, t2, zconj(v2));
+                            Z[zlay(p, kk)] = x;
+                            zc[s] = y;
                         }
                     }
                 }
                 // window: rows/cols kk, kk+1 and row kk+2
-                if (kk > m && t == 0) H[lay(kk, kk - 1)] = ident ? x1 : mk(beta, 0.0);
-                zd tc = zconj(t1), vc = zconj(v2);
-                zd s1 = zadd(zmul(tc, a), zscale(t2, c));
-                a = zsub(a, s1);
-                c = zsub(c, zmul(s1, v2));
-                zd s2 = zadd(zmul(tc, b), zscale(t2, d));
-                b = zsub(b, s2);
-                d = zsub(d, zmul(s2, v2));
-                zd s3 = zadd(zmul(t1, a), zscale(t2, b));
-                a = zsub(a, s3);
-                b = zsub(b, zmul(s3, vc));
-                zd s4 = zadd(zmul(t1, c), zscale(t2, d));
-                c = zsub(c, s4);
-                d = zsub(d, zmul(s4, vc));
+                if (!first && t == 0) H[lay(kk, kk - 1)] = ident ? x1 : mk(beta, 0.0);
+                const zd tc = zconj(t1), vc = zconj(v2);
+                pair_op(a, c, tc, t2, v2);    // left: column kk
+                pair_op(b, d, tc, t2, v2);    // left: column kk+1
+                pair_op(a, b, t1, t2, vc);    // right: row kk
+                pair_op(c, d, t1, t2, vc);    // right: row kk+1
                 zd f = mk(0.0, 0.0);
-                if (kk + 2 <= i) {
-                    zd s5 = zscale(t2, e);
-                    f = mk(-s5.re, -s5.im);
-                    e = zsub(e, zmul(s5, vc));
+                if (has_e) {                  // right: row kk+2, H(kk+2,kk) = 0 before
+                    const double sr = t2 * e.re, si = t2 * e.im;
+                    f = mk(-sr, -si);
+                    e.re = fma(-sr, vc.re, fma(si, vc.im, e.re));
+                    e.im = fma(-sr, vc.im, fma(-si, vc.re, e.im));
                 }
-                if (t == 0) H[lay(kk, kk)] = a;
+                if (t == 0) H[cbk + kk] = a;
 #pragma unroll
                 for (int s = 0; s < S; ++s)
                     if (t + s * LPM == kk) carry[s] = b;
                 lds_order();
-                if (kk == m && m > l) {
-                    // ZLAHQR keeps H(m, m-1) real after a start at m > l:
-                    // flush the step's state, scale in LDS, reload.
+                return f;
+            };
+            // move the state from step kk to step k1 = kk+1 (< i)
+            auto advance = [&](const int kk, const zd f) {
+                const int k1 = kk + 1;
+                const int cb1 = lay.cb(k1 + 1);
+                x1 = c;
+                x2 = f;
+                a = d;
+                c = e;
+                b = H[cb1 + k1];
+                d = H[cb1 + k1 + 1];
+                e = H[cb1 + (k1 + 2 <= i ? k1 + 2 : k1 + 1)];   // unused at the last step
 #pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        const int p = t + s * LPM;
-                        if (p >= I1 && p <= m - 1) H[lay(p, m + 1)] = carry[s];
-                        if constexpr (WANTT) {
-                            if (wantz && p < n) Z[zlay(p, m + 1)] = zc[s];
-                        }
-                    }
-                    if (t == 0) {
-                        H[lay(m, m + 1)] = b;
-                        H[lay(m + 1, m)] = c;
-                        H[lay(m + 1, m + 1)] = d;
-                        if (m + 2 <= i) H[lay(m + 2, m + 1)] = e;
-                    }
-                    lds_order();
-                    zd temp = zsub(mk(1.0, 0.0), t1);
-                    temp = zscale(1.0 / zabs(temp), temp);
-                    zd tempc = zconj(temp);
-                    if (t == 0) {
-                        H[lay(m + 1, m)] = zmul(H[lay(m + 1, m)], tempc);
-                        if (m + 2 <= i) H[lay(m + 2, m + 1)] = zmul(H[lay(m + 2, m + 1)], temp);
-                    }
-                    lds_order();
-                    for (int j = m + t; j <= i; j += LPM)
-                        if (j != m + 1)
-                            for (int cc = j + 1; cc <= I2; ++cc) H[lay(j, cc)] = zmul(H[lay(j, cc)], temp);
-                    lds_order();
-                    for (int j = m + t; j <= i; j += LPM)
-                        if (j != m + 1)
-                            for (int r = I1; r <= j - 1; ++r) H[lay(r, j)] = zmul(H[lay(r, j)], tempc);
+                for (int s = 0; s < S; ++s) {
+                    const int p = t + s * LPM;
+                    const bool rowop = (p >= k1 + 2 && p <= I2);
+                    const int pc = p < k1 ? p : k1;
+                    ypre[s] = H[rowop ? cbp[s] + k1 + 1 : cb1 + pc];
                     if constexpr (WANTT) {
-                        if (wantz)
-                            for (int j = m; j <= i; ++j)
-                                if (j != m + 1)
-                                    for (int r = t; r < n; r += LPM) Z[zlay(r, j)] = zmul(Z[zlay(r, j)], tempc);
+                        if (wantz) zy[s] = Z[zlay(p < n ? p : n - 1, k1 + 1)];
                     }
-                    lds_order();
-                    const int k1 = kk + 1;
-                    if (k1 < i) {
-                        x1 = H[lay(k1, kk)];
-                        x2 = f;
-                        a = H[lay(k1, k1)];
-                        b = H[lay(k1, k1 + 1)];
-                        c = H[lay(k1 + 1, k1)];
-                        d = H[lay(k1 + 1, k1 + 1)];
-                        e = (k1 + 2 <= i) ? H[lay(k1 + 2, k1 + 1)] : mk(0.0, 0.0);
-#pragma unroll
-                        for (int s = 0; s < S; ++s) {
-                            const int p = t + s * LPM;
-                            if (p >= k1 + 2 && p <= I2) { carry[s] = H[lay(k1, p)]; ypre[s] = H[lay(k1 + 1, p)]; }
-                            else if (p >= I1 && p <= k1 - 1) { carry[s] = H[lay(p, k1)]; ypre[s] = H[lay(p, k1 + 1)]; }
-                            if constexpr (WANTT) {
-                                if (wantz && p < n) { zc[s] = Z[zlay(p, k1)]; zy[s] = Z[zlay(p, k1 + 1)]; }
-                            }
-                        }
-                    } else {
-                        c = H[lay(i, i - 1)];
-                        d = H[lay(i, i)];
-#pragma unroll
-                        for (int s = 0; s < S; ++s) {
-                            const int p = t + s * LPM;
-                            if (p >= I1 && p <= i - 1) carry[s] = H[lay(p, i)];
-                            else if (p >= i + 1 && p <= I2) carry[s] = H[lay(i, p)];
-                            if constexpr (WANTT) {
-                                if (wantz && p < n) zc[s] = Z[zlay(p, i)];
-                            }
-                        }
-                    }
-                    continue;
                 }
-                if (kk + 1 < i) {
-                    const int k1 = kk + 1;
-                    x1 = c;
+            };
+
+            zd f = step(m, true, m + 2 <= i);
+            if (m > l) {
+                // ZLAHQR keeps H(m, m-1) real after a start at m > l:
+                // flush the step's state, scale in LDS, reload.
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int p = t + s * LPM;
+                    if (p >= I1 && p <= m - 1) H[lay(p, m + 1)] = carry[s];
+                    if constexpr (WANTT) {
+                        if (wantz && p < n) Z[zlay(p, m + 1)] = zc[s];
+                    }
+                }
+                if (t == 0) {
+                    H[lay(m, m + 1)] = b;
+                    H[lay(m + 1, m)] = c;
+                    H[lay(m + 1, m + 1)] = d;
+                    if (m + 2 <= i) H[lay(m + 2, m + 1)] = e;
+                }
+                lds_order();
+                zd t1, v2;
+                double beta;
+                bool ident;
+                zlarfg2(x1, x2, t1, v2, beta, ident);   // the reflector of step m again
+                zd temp = zsub(mk(1.0, 0.0), t1);
+                temp = zscale(1.0 / zabs(temp), temp);
+                const zd tempc = zconj(temp);
+                if (t == 0) {
+                    H[lay(m + 1, m)] = zmul(H[lay(m + 1, m)], tempc);
+                    if (m + 2 <= i) H[lay(m + 2, m + 1)] = zmul(H[lay(m + 2, m + 1)], temp);
+                }
+                lds_order();
+                for (int j = m + t; j <= i; j += LPM)
+                    if (j != m + 1)
+                        for (int cc = j + 1; cc <= I2; ++cc) H[lay(j, cc)] = zmul(H[lay(j, cc)], temp);
+                lds_order();
+                for (int j = m + t; j <= i; j += LPM)
+                    if (j != m + 1)
+                        for (int r = I1; r <= j - 1; ++r) H[lay(r, j)] = zmul(H[lay(r, j)], tempc);
+                if constexpr (WANTT) {
+                    if (wantz)
+                        for (int j = m; j <= i; ++j)
+                            if (j != m + 1)
+                                for (int r = t; r < n; r += LPM) Z[zlay(r, j)] = zmul(Z[zlay(r, j)], tempc);
+                }
+                lds_order();
+                const int k1 = m + 1;
+                if (k1 < i) {
+                    x1 = H[lay(k1, m)];
                     x2 = f;
-                    a = d;
-                    c = e;
+                    a = H[lay(k1, k1)];
                     b = H[lay(k1, k1 + 1)];
+                    c = H[lay(k1 + 1, k1)];
                     d = H[lay(k1 + 1, k1 + 1)];
                     e = (k1 + 2 <= i) ? H[lay(k1 + 2, k1 + 1)] : mk(0.0, 0.0);
 #pragma unroll
                     for (int s = 0; s < S; ++s) {
                         const int p = t + s * LPM;
-                        if (p >= k1 + 2 && p <= I2) ypre[s] = H[lay(k1 + 1, p)];
-                        else if (p >= I1 && p <= k1 - 1) ypre[s] = H[lay(p, k1 + 1)];
+                        if (p >= k1 + 2 && p <= I2) { carry[s] = H[lay(k1, p)]; ypre[s] = H[lay(k1 + 1, p)]; }
+                        else if (p >= I1 && p <= k1 - 1) { carry[s] = H[lay(p, k1)]; ypre[s] = H[lay(p, k1 + 1)]; }
                         if constexpr (WANTT) {
-                            if (wantz && p < n) zy[s] = Z[zlay(p, k1 + 1)];
+                            if (wantz && p < n) { zc[s] = Z[zlay(p, k1)]; zy[s] = Z[zlay(p, k1 + 1)]; }
+                        }
+                    }
+                } else {
+                    c = H[lay(i, i - 1)];
+                    d = H[lay(i, i)];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const int p = t + s * LPM;
+                        if (p >= I1 && p <= i - 1) carry[s] = H[lay(p, i)];
+                        else if (p >= i + 1 && p <= I2) carry[s] = H[lay(i, p)];
+                        if constexpr (WANTT) {
+                            if (wantz && p < n) zc[s] = Z[zlay(p, i)];
                         }
                     }
                 }
+            } else if (m + 1 < i) {
+                advance(m, f);
             }
+            // main chase: every step has a row kk+2 and a successor
+            for (int kk = m + 1; kk < i - 1; ++kk) {
+                f = step(kk, false, true);
+                advance(kk, f);
+            }
+            if (i - 1 > m) f = step(i - 1, false, false);
             // ---- end of sweep: c = H(i,i-1), d = H(i,i); make H(i,i-1) real
             zd hsub = c;
             bool rot = (c.im != 0.0);
