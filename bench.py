@@ -74,6 +74,115 @@ def cpu_baseline(A_host: np.ndarray, n: int, budget_s: float):
     return done / dt, done, np.concatenate(eig_chunks), oracle.max_threads()
 
 
+def timed(fn, steps: int, warmup: int) -> float:
+    """Seconds per call of fn, HIP events on the current stream."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / steps
+
+
+def bench_ragged(dev, B=200_000, lo=4, hi=64, steps=3, seed=1):
+    """Mixed sizes n ~ U{lo..hi}: persistent work-queue kernel (hipGraph) vs
+    the same-size path called once per size bucket on the same matrices."""
+    from tokura_zgeev_batched_amd.ragged import RaggedBatch
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(lo, hi + 1, B)
+    offs = np.zeros(B + 1, np.int64)
+    offs[1:] = np.cumsum(sizes.astype(np.int64) ** 2)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    flat = torch.empty(int(offs[-1]), dtype=torch.complex128, device=dev)
+    buckets = {}
+    offs_t = torch.tensor(offs[:-1], device=dev)
+    sizes_t = torch.tensor(sizes, device=dev)
+    for n in range(lo, hi + 1):
+        idx = torch.nonzero(sizes_t == n).flatten()
+        if idx.numel() == 0:
+            continue
+        cnt = int(idx.numel())
+        A = torch.complex(torch.randn(cnt, n * n, generator=g, device=dev, dtype=torch.float64),
+                          torch.randn(cnt, n * n, generator=g, device=dev, dtype=torch.float64))
+        pos = (offs_t[idx][:, None] + torch.arange(n * n, device=dev)[None]).flatten()
+        flat[pos] = A.flatten()
+        buckets[n] = (A, torch.empty((cnt, n), dtype=torch.complex128, device=dev),
+                      torch.empty(cnt, dtype=torch.int8, device=dev),
+                      torch.empty(tk.workspace_size(n, cnt), dtype=torch.uint8, device=dev))
+    h = tk.Handle()
+    rb = RaggedBatch(offs, handle=h)
+    cfg, lanes = rb.autotune(flat)
+    eig = torch.empty(int(rb.eig_offsets[-1]), dtype=torch.complex128, device=dev)
+    flag = torch.empty(B, dtype=torch.int8, device=dev)
+    t_rag = timed(lambda: rb.solve(flat, eig, flag, graph=True), steps, 1)
+    nbad = int((flag != 0).sum())
+
+    def plain_buckets():
+        for n, (A, e, f, w) in buckets.items():
+            rc = tk.zgeev_batched(A, n, A.shape[0], e, w, f, h)
+            if rc != 0:
+                raise RuntimeError(f"plain call n={n} returned {rc}")
+    t_plain = timed(plain_buckets, steps, 1)
+    # the two paths agree on every matrix of one bucket
+    n0 = hi
+    idx = np.nonzero(sizes == n0)[0]
+    got = np.stack([eig[rb.eig_offsets[b]:rb.eig_offsets[b + 1]].cpu().numpy() for b in idx[:200]])
+    ref = buckets[n0][1][:200].cpu().numpy()
+    import oracle
+    agree = oracle.max_relerr(ref, got)
+    flops = float(np.sum(40.0 * sizes.astype(np.float64) ** 3))
+    return {"workload": f"ragged, batch={B}, n~U{{{lo}..{hi}}} seed {seed}, int64 offsets",
+            "ragged_matrices_per_s": B / t_rag, "ragged_ms": t_rag * 1e3,
+            "plain_per_bucket_matrices_per_s": B / t_plain, "plain_per_bucket_ms": t_plain * 1e3,
+            "speedup_vs_plain_per_bucket": t_plain / t_rag,
+            "ragged_config": {"id": cfg, "lanes_per_matrix_A_B_C": lanes},
+            "model_tflops": flops / t_rag / 1e12, "flags_nonzero": nbad,
+            "max_rel_diff_ragged_vs_plain_n64_sample": agree}
+
+
+def bench_diff(dev, B=50_000, n=32, steps=3, seed=3):
+    """Forward (eigenvalues only) vs forward with Schur vectors + adjoint."""
+    from tokura_zgeev_batched_amd.autograd import backward_batched, schur_batched
+    g = torch.Generator(device=dev).manual_seed(seed)
+    A = make_batch(B, n, seed, dev)
+    gbar = torch.complex(torch.randn(B, n, generator=g, device=dev, dtype=torch.float64),
+                         torch.randn(B, n, generator=g, device=dev, dtype=torch.float64))
+    h = tk.Handle()
+    eig = torch.empty((B, n), dtype=torch.complex128, device=dev)
+    flag = torch.empty(B, dtype=torch.int8, device=dev)
+    work = torch.empty(tk.workspace_size(n, B), dtype=torch.uint8, device=dev)
+    t_fwd = timed(lambda: tk.zgeev_batched(A, n, B, eig, work, flag, h), steps, 1)
+    state = {}
+
+    def fwd_schur():
+        state["r"] = schur_batched(A, n, B, h)
+
+    t_schur = timed(fwd_schur, steps, 1)
+    e_s, schur, fl = state["r"]
+
+    def bwd():
+        backward_batched(n, B, gbar, schur=schur, flag=fl, handle=h)
+
+    t_bwd = timed(bwd, steps, 1)
+
+    def bwd_recompute():
+        backward_batched(n, B, gbar, A_colmajor=A, flag=flag, handle=h)
+
+    t_bwd_rc = timed(bwd_recompute, steps, 1)
+    return {"workload": f"differentiation, batch={B}, n={n}, loss Re sum conj(g) lambda",
+            "fwd_eigvals_ms": t_fwd * 1e3, "fwd_schur_ms": t_schur * 1e3, "bwd_ms": t_bwd * 1e3,
+            "bwd_recompute_ms": t_bwd_rc * 1e3,
+            "fwd_bwd_solves_per_s": B / (t_schur + t_bwd),
+            "fwd_bwd_over_fwd_eigvals": (t_schur + t_bwd) / t_fwd,
+            "fwd_bwd_over_fwd_schur": (t_schur + t_bwd) / t_schur,
+            "recompute_fwd_bwd_over_fwd_eigvals": (t_fwd + t_bwd_rc) / t_fwd}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +192,7 @@ def main():
     ap.add_argument("--batch", type=int, default=100_000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the ragged and differentiation legs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the batched eigenvalue path shards by matrices "
@@ -165,6 +275,10 @@ def main():
                                             f"(OpenBLAS, 1 thread/matrix, OpenMP over {cores} threads)"}
         result["max_rel_err_vs_lapack"] = err
         result["accuracy_sample"] = done
+    if not args.no_extra:
+        del A, eig, flag, work
+        torch.cuda.empty_cache()
+        result["extra"] = {"ragged": bench_ragged(dev), "differentiation": bench_diff(dev)}
     print(json.dumps(result), flush=True)
 
 
