@@ -115,6 +115,7 @@ def bench_ragged(dev, B=200_000, lo=4, hi=64, steps=3, seed=1):
                       torch.empty(cnt, dtype=torch.int8, device=dev),
                       torch.empty(tk.workspace_size(n, cnt), dtype=torch.uint8, device=dev))
     h = tk.Handle()
+    h.autotune(lo, hi, 4096)            # the per-bucket baseline gets tuned variants too
     rb = RaggedBatch(offs, handle=h)
     cfg, lanes = rb.autotune(flat)
     eig = torch.empty(int(rb.eig_offsets[-1]), dtype=torch.complex128, device=dev)
@@ -140,7 +141,7 @@ def bench_ragged(dev, B=200_000, lo=4, hi=64, steps=3, seed=1):
             "ragged_matrices_per_s": B / t_rag, "ragged_ms": t_rag * 1e3,
             "plain_per_bucket_matrices_per_s": B / t_plain, "plain_per_bucket_ms": t_plain * 1e3,
             "speedup_vs_plain_per_bucket": t_plain / t_rag,
-            "ragged_config": {"id": cfg, "lanes_per_matrix_A_B_C": lanes},
+            "ragged_config": {"id": cfg, "lanes_per_matrix_A1_A2_B_C": lanes},
             "model_tflops": flops / t_rag / 1e12, "flags_nonzero": nbad,
             "max_rel_diff_ragged_vs_plain_n64_sample": agree}
 
@@ -193,6 +194,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the ragged and differentiation legs")
+    ap.add_argument("--no-tune", action="store_true", help="use the default launch table")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the batched eigenvalue path shards by matrices "
@@ -207,6 +209,8 @@ def main():
     flag = torch.empty((B,), dtype=torch.int8, device=dev)
     work = torch.empty((tk.workspace_size(n, B),), dtype=torch.uint8, device=dev)
     h = tk.Handle()
+    if not args.no_tune:
+        h.autotune(n, n, 8192)          # runtime autotuner picks the kernel variant for n
     variant, lpm, mpb = h.launch_config(n)
     stream = torch.cuda.current_stream()
 

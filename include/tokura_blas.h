@@ -146,13 +146,14 @@ int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, const hipDo
                               hipDoubleComplex *eigenvalues, void *work, char *flag,
                               hipStream_t stream);
 
-/* Class configuration of the ragged path: lanes per matrix of class A
- * (n 33..64), B (17..32), C (1..16); a wavefront holds 64/lanes matrices of
- * a class.  tokuraAutotuneRagged times every configuration on the caller's
+/* Class configuration of the ragged path: lanes per matrix of class A1
+ * (n 49..64), A2 (33..48), B (17..32), C (1..16); a wavefront holds
+ * 64/lanes matrices of a class.  tokuraAutotuneRagged times every configuration on the caller's
  * prepared batch (buffers as for tokura_zgeev_ragged_gpu; the stream is
  * synchronised), keeps the fastest and re-prepares `work` for it. */
 int tokuraSetRaggedConfig(tokurablas_t *handle, int config);
-int tokuraGetRaggedConfig(tokurablas_t *handle, int *config, int *lanes_a, int *lanes_b, int *lanes_c);
+int tokuraGetRaggedConfig(tokurablas_t *handle, int *config, int *lanes_a1, int *lanes_a2, int *lanes_b,
+                          int *lanes_c);
 int tokuraAutotuneRagged(tokurablas_t *handle, int batch_count, const int64_t *offsets_host,
                          const hipDoubleComplex *A, hipDoubleComplex *eigenvalues, void *work,
                          size_t work_bytes, char *flag, hipStream_t stream);

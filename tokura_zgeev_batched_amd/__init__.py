@@ -51,7 +51,7 @@ _SIGNATURES = {
     "tokuraGetStageTimes": ([_P, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)], _I),
     "tokura_get_zgeev_ragged_worksize": ([_I, _P], _SZ),
     "tokuraSetRaggedConfig": ([_P, _I], _I),
-    "tokuraGetRaggedConfig": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
+    "tokuraGetRaggedConfig": ([_P] + [ctypes.POINTER(_I)] * 5, _I),
     "tokuraAutotuneRagged": ([_P, _I, _P, _P, _P, _P, _SZ, _P, _P], _I),
     "tokura_zgeev_ragged_prepare": ([_P, _I, _P, _P, _SZ, _P], _I),
     "tokura_zgeev_ragged_gpu": ([_P, _I, _P, _P, _P, _P, _P], _I),

@@ -53,7 +53,7 @@ __device__ __forceinline__ void fill_nan(zd *__restrict__ dst, int count, int t)
 
 // Schur form of the matrix at src: T into H (upper triangular), U into Z.
 // Returns -1 for a non-finite input, else the QR INFO.
-template <int NB, int LPM>
+template <int NB, int LPM, bool ZH = false>
 __device__ int schur_in_lds(zd *H, zd *Z, zd *taus, const FullLayout lay, const zd *__restrict__ src, int n, int t,
                             int itmax)
 {
@@ -64,14 +64,17 @@ __device__ int schur_in_lds(zd *H, zd *Z, zd *taus, const FullLayout lay, const 
     hessenberg<LPM, true>(H, lay, n, t, taus);
     form_q<LPM>(Z, lay, H, lay, taus, n, t);
     int info = 0;
-    if (n > 1) info = zlahqr_core<LPM, S, true>(H, lay, n, t, itmax, Z, lay);
+    if (n > 1) info = zlahqr_core<LPM, S, true, ZH>(H, lay, n, t, itmax, Z, lay);
     for (int c = 0; c < n; ++c)
         for (int r = c + 1 + t; r < n; r += LPM) H[lay(r, c)] = mk(0.0, 0.0);
     lds_order();
     return info;
 }
 
-template <int NB, int LPM>
+// ZH: the upper half of each lane group applies the reflectors to Z in the
+// same instructions that update H (n <= LPM/2), doubling the matrices per CU
+// compared with one group of n lanes doing both.
+template <int NB, int LPM, bool ZH>
 __global__ __launch_bounds__(64) void k_zgeev_schur(int n, int batch, const zd *__restrict__ A, zd *__restrict__ eig,
                                                     zd *__restrict__ schur, int8_t *__restrict__ flag, int itmax)
 {
@@ -84,7 +87,7 @@ __global__ __launch_bounds__(64) void k_zgeev_schur(int n, int batch, const zd *
     zd *H = lds + g * G::PER, *Z = H + G::MAT, *taus = Z + G::MAT;
     const FullLayout lay{n | 1};
     const long long nn = (long long)n * n;
-    const int info = schur_in_lds<NB, LPM>(H, Z, taus, lay, A + mat * nn, n, t, itmax);
+    const int info = schur_in_lds<NB, LPM, ZH>(H, Z, taus, lay, A + mat * nn, n, t, itmax);
     zd *out = eig + mat * n;
     zd *T = schur + mat * 2 * nn, *U = T + nn;
     if (info < 0) {
@@ -237,16 +240,19 @@ __global__ __launch_bounds__(64) void k_zgeev_adjoint(int n, int batch, const zd
 
 static int adj_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 
+// Schur forward configurations (NB, lanes per matrix, Z on the upper half-wave)
+#define TKZ_SCHUR_CFG(X) X(16, 32, true) X(32, 64, true) X(64, 64, false)
+
 static hipError_t launch_schur(int n, int batch, const zd *A, zd *eig, zd *schur, int8_t *flag, int itmax,
                                hipStream_t s)
 {
     switch (adj_nb(n)) {
-#define X(nb, lpm)                                                                                          \
-    case nb:                                                                                                \
-        hipLaunchKernelGGL((k_zgeev_schur<nb, lpm>), dim3((batch + 64 / lpm - 1) / (64 / lpm)), dim3(64), 0, s, \
-                           n, batch, A, eig, schur, flag, itmax);                                           \
+#define X(nb, lpm, zh)                                                                                       \
+    case nb:                                                                                                 \
+        hipLaunchKernelGGL((k_zgeev_schur<nb, lpm, zh>), dim3((batch + 64 / lpm - 1) / (64 / lpm)), dim3(64), 0, \
+                           s, n, batch, A, eig, schur, flag, itmax);                                         \
         return hipGetLastError();
-        TKZ_ADJ_NB(X)
+        TKZ_SCHUR_CFG(X)
 #undef X
     }
     return hipErrorInvalidValue;

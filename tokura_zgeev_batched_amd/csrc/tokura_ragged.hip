@@ -10,10 +10,11 @@
 //                       matrices that need more QR sweeps never hold up a
 //                       whole launch; tiles are queued in decreasing size
 //                       (longest-processing-time first).
-// A tile is the work of one wavefront: one matrix of class A (33..64), or
-// 64/LPM_B matrices of class B (17..32), or 64/LPM_C of class C (1..16); the
-// lanes-per-matrix of each class form a "class configuration" that the
-// autotuner picks.
+// A tile is the work of one wavefront: 64/LPM matrices of one size class,
+// A1 (n 49..64), A2 (33..48), B (17..32) or C (1..16); the lanes-per-matrix of
+// each class form a "class configuration" that the autotuner picks.  Smaller
+// classes put more matrices on a wavefront, so the serial reflector chain of
+// every chase step serves several matrices at once.
 //
 // `work` layout (built on the host by tokura_zgeev_ragged_prepare):
 //   RaggedHeader | tiles int4[ntiles] | order int[B] | n int[B] |
@@ -40,21 +41,20 @@ struct RaggedHeader {
 };
 static_assert(sizeof(RaggedHeader) == 128, "header size");
 
-// lanes per matrix of classes A (33..64), B (17..32), C (1..16)
+// lanes per matrix of classes A1 (49..64), A2 (33..48), B (17..32), C (1..16)
 #define TKZ_RAGGED_CFGS(X) \
-    X(0, 64, 16, 4)        \
-    X(1, 64, 32, 8)        \
-    X(2, 64, 16, 8)        \
-    X(3, 64, 32, 4)
-static const int g_cfg_lpm[][3] = {
-#define X(id, a, b, c) {a, b, c},
+    X(0, 64, 32, 16, 4)    \
+    X(1, 64, 64, 32, 8)    \
+    X(2, 64, 32, 16, 8)    \
+    X(3, 64, 32, 32, 4)
+static const int g_cfg_lpm[][4] = {
+#define X(id, a1, a2, b, c) {a1, a2, b, c},
     TKZ_RAGGED_CFGS(X)
 #undef X
 };
 static constexpr int kNumRaggedCfg = 4;
 
-__host__ __device__ constexpr int cls_of(int n) { return n > 32 ? 0 : (n > 16 ? 1 : 2); }
-__host__ __device__ constexpr int cls_nb(int c) { return c == 0 ? 64 : (c == 1 ? 32 : 16); }
+__host__ __device__ constexpr int cls_of(int n) { return n > 48 ? 0 : (n > 32 ? 1 : (n > 16 ? 2 : 3)); }
 
 struct RaggedView {
     RaggedHeader *h;
@@ -124,27 +124,29 @@ __device__ __forceinline__ void qr_tile(zd *lds, const RaggedView &V, int4 td, z
 
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
-template <int LA, int LB, int LC>
+template <int LA1, int LA2, int LB, int LC>
 __global__ __launch_bounds__(64) void k_ragged_hess(const zd *__restrict__ A, char *work, int8_t *__restrict__ flag)
 {
-    constexpr int SZ = cmax(cmax((64 / LA) * 64 * 65, (64 / LB) * 32 * 33), (64 / LC) * 16 * 17);
+    constexpr int SZ = cmax(cmax((64 / LA1) * 64 * 65, (64 / LA2) * 48 * 49),
+                            cmax((64 / LB) * 32 * 33, (64 / LC) * 16 * 17));
     __shared__ zd lds[SZ];
     const RaggedView V(work);
     const int nt = V.h->ntiles;
     for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
         const int4 td = V.tiles[tile];
-        if (td.x == 0) hess_tile<64, LA>(lds, V, td, A, flag);
-        else if (td.x == 1) hess_tile<32, LB>(lds, V, td, A, flag);
+        if (td.x == 0) hess_tile<64, LA1>(lds, V, td, A, flag);
+        else if (td.x == 1) hess_tile<48, LA2>(lds, V, td, A, flag);
+        else if (td.x == 2) hess_tile<32, LB>(lds, V, td, A, flag);
         else hess_tile<16, LC>(lds, V, td, A, flag);
         lds_order();
     }
 }
 
-template <int LA, int LB, int LC>
+template <int LA1, int LA2, int LB, int LC>
 __global__ __launch_bounds__(64) void k_ragged_qr(char *work, zd *__restrict__ eig, int8_t *__restrict__ flag, int itmax)
 {
-    constexpr int SZ = cmax(cmax((64 / LA) * packed_size(64), (64 / LB) * packed_size(32)),
-                            (64 / LC) * packed_size(16));
+    constexpr int SZ = cmax(cmax((64 / LA1) * packed_size(64), (64 / LA2) * packed_size(48)),
+                            cmax((64 / LB) * packed_size(32), (64 / LC) * packed_size(16)));
     __shared__ zd lds[SZ];
     const RaggedView V(work);
     const int nt = V.h->ntiles;
@@ -155,13 +157,14 @@ __global__ __launch_bounds__(64) void k_ragged_qr(char *work, zd *__restrict__ e
         tile = __shfl(tile, 0);
         if (tile >= nt) break;
         const int4 td = V.tiles[tile];
-        if (td.x == 0) qr_tile<64, LA>(lds, V, td, eig, flag, itmax);
-        else if (td.x == 1) qr_tile<32, LB>(lds, V, td, eig, flag, itmax);
+        if (td.x == 0) qr_tile<64, LA1>(lds, V, td, eig, flag, itmax);
+        else if (td.x == 1) qr_tile<48, LA2>(lds, V, td, eig, flag, itmax);
+        else if (td.x == 2) qr_tile<32, LB>(lds, V, td, eig, flag, itmax);
         else qr_tile<16, LC>(lds, V, td, eig, flag, itmax);
     }
 }
 
-template <int LA, int LB, int LC>
+template <int LA1, int LA2, int LB, int LC>
 static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int cus,
                                     hipStream_t s)
 {
@@ -173,15 +176,15 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     // occupancy queries are not stream operations and are allowed)
     static int occ_h = 0, occ_q = 0;
     if (occ_h == 0) {
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_h, k_ragged_hess<LA, LB, LC>, 64, 0);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_q, k_ragged_qr<LA, LB, LC>, 64, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_h, k_ragged_hess<LA1, LA2, LB, LC>, 64, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_q, k_ragged_qr<LA1, LA2, LB, LC>, 64, 0);
         if (occ_h < 1) occ_h = 1;
         if (occ_q < 1) occ_q = 1;
     }
-    hipLaunchKernelGGL((k_ragged_hess<LA, LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag);
+    hipLaunchKernelGGL((k_ragged_hess<LA1, LA2, LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ragged_qr<LA, LB, LC>), dim3(cus * occ_q), dim3(64), 0, s, work, eig, flag, itmax);
+    hipLaunchKernelGGL((k_ragged_qr<LA1, LA2, LB, LC>), dim3(cus * occ_q), dim3(64), 0, s, work, eig, flag, itmax);
     return hipGetLastError();
 }
 
@@ -189,8 +192,8 @@ static hipError_t launch_ragged(int cfg, const zd *A, char *work, zd *eig, int8_
                                 hipStream_t s)
 {
     switch (cfg) {
-#define X(id, a, b, c) \
-    case id: return launch_ragged_cfg<a, b, c>(A, work, eig, flag, itmax, cus, s);
+#define X(id, a1, a2, b, c) \
+    case id: return launch_ragged_cfg<a1, a2, b, c>(A, work, eig, flag, itmax, cus, s);
         TKZ_RAGGED_CFGS(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -405,14 +408,16 @@ extern "C" int tokuraSetRaggedConfig(tokurablas_t *handle, int cfg)
     return 0;
 }
 
-extern "C" int tokuraGetRaggedConfig(tokurablas_t *handle, int *cfg, int *lpm_a, int *lpm_b, int *lpm_c)
+extern "C" int tokuraGetRaggedConfig(tokurablas_t *handle, int *cfg, int *lpm_a1, int *lpm_a2, int *lpm_b,
+                                     int *lpm_c)
 {
     if (handle == nullptr) return -1;
     const int c = handle->ragged_cfg;
     if (cfg) *cfg = c;
-    if (lpm_a) *lpm_a = g_cfg_lpm[c][0];
-    if (lpm_b) *lpm_b = g_cfg_lpm[c][1];
-    if (lpm_c) *lpm_c = g_cfg_lpm[c][2];
+    if (lpm_a1) *lpm_a1 = g_cfg_lpm[c][0];
+    if (lpm_a2) *lpm_a2 = g_cfg_lpm[c][1];
+    if (lpm_b) *lpm_b = g_cfg_lpm[c][2];
+    if (lpm_c) *lpm_c = g_cfg_lpm[c][3];
     return 0;
 }
 

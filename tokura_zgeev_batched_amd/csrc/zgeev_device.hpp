@@ -142,18 +142,20 @@ __device__ __forceinline__ double rsq_nr(double x)
     return fma(y, r, y);
 }
 
-// ---- ZLARFG for a 2-vector (x1, x2): H^H [x1; x2] = [beta; 0] ----
-// On return t1 = tau, v2 = scaled x2, beta; ident = true when tau = 0
-// (then LAPACK leaves alpha = x1 and x2 untouched).
+// ---- ZLARFG for a 2-vector (x1, x2), x2 real: H^H [x1; x2] = [beta; 0] ----
+// In the single-shift sweep the second entry is always real (it is a bulge
+// created from a real sub-diagonal), as ZLAHQR relies on.  On return
+// t1 = tau, v2 = scaled x2, t2 = Re(tau v2) = -x2/beta, beta; ident = true
+// when tau = 0 (then LAPACK leaves alpha = x1 and x2 untouched).
 // beta = -sign(Re x1) * |(x1, x2)|, tau = (beta - x1)/beta, v2 = x2/(x1 - beta).
 // In the normal range the square root and both reciprocals come from one
 // rsq and one rcp estimate (|x1 - beta|^2 = (|Re x1| + |beta|)^2 + Im(x1)^2 is
 // within [ss, 4 ss]); outside it a scaled, exactly rounded path is used.  The
 // rescaling loop of ZLARFG for |beta| < safmin/eps (~1e-292) is not reproduced.
-__device__ __forceinline__ void zlarfg2(zd x1, zd x2, zd &t1, zd &v2, double &beta, bool &ident)
+__device__ __forceinline__ void zlarfg2(zd x1, double x2, zd &t1, zd &v2, double &t2, double &beta, bool &ident)
 {
-    ident = (x2.re == 0.0 && x2.im == 0.0 && x1.im == 0.0);
-    const double ss = x1.re * x1.re + x1.im * x1.im + (x2.re * x2.re + x2.im * x2.im);
+    ident = (x2 == 0.0 && x1.im == 0.0);
+    const double ss = fma(x1.re, x1.re, fma(x1.im, x1.im, x2 * x2));
     double ib;
     zd rz;
     if (ss > 1e-290 && ss < 1e290) {
@@ -166,16 +168,17 @@ __device__ __forceinline__ void zlarfg2(zd x1, zd x2, zd &t1, zd &v2, double &be
         const double iz = rcp_nr(zr * zr + x1.im * x1.im);
         rz = mk(zr * iz, -x1.im * iz);
     } else {
-        double w = fmax(fmax(fabs(x1.re), fabs(x1.im)), fmax(fabs(x2.re), fabs(x2.im)));
+        double w = fmax(fmax(fabs(x1.re), fabs(x1.im)), fabs(x2));
         double iw = w > 0.0 ? 1.0 / w : 0.0;
-        double a = x1.re * iw, b = x1.im * iw, c = x2.re * iw, d = x2.im * iw;
-        beta = -copysign(w * sqrt(a * a + b * b + c * c + d * d), x1.re);
+        double a = x1.re * iw, b = x1.im * iw, c = x2 * iw;
+        beta = -copysign(w * sqrt(a * a + b * b + c * c), x1.re);
         ib = 1.0 / beta;
         rz = zdiv(mk(1.0, 0.0), mk(x1.re - beta, x1.im));
     }
     t1 = mk((beta - x1.re) * ib, -x1.im * ib);
-    v2 = zmul(x2, rz);
-    if (ident) { t1 = mk(0.0, 0.0); v2 = x2; beta = x1.re; }
+    v2 = mk(x2 * rz.re, x2 * rz.im);
+    t2 = -x2 * ib;
+    if (ident) { t1 = mk(0.0, 0.0); v2 = mk(x2, 0.0); t2 = 0.0; beta = x1.re; }
 }
 
 // Application of a 2-element reflector to one element pair (x, y):
@@ -292,6 +295,101 @@ __device__ void hessenberg(zd *A, const FullLayout lay, int n, int t, zd *taus =
     }
 }
 
+// ---- ZGEHD2 by a whole workgroup of NT threads on one LDS matrix ----
+// Same arithmetic order per element as hessenberg<>() except that each
+// row/column dot product is split over 4 consecutive threads (a quad) and
+// combined with two shuffles.  Workgroup barriers separate the phases.
+// `red` is LDS scratch of at least NT/64 doubles.
+template <int NT>
+__device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red)
+{
+    const int tid = threadIdx.x;
+    const int q = tid & 3, row = tid >> 2;        // quad lane, row/column slot
+    for (int i = 0; i + 2 <= n; ++i) {
+        const int m = n - 1 - i;
+        double ssq = 0.0, amx = 0.0;
+        for (int r = i + 2 + tid; r < n; r += NT) {
+            zd v = A[lay(r, i)];
+            ssq += v.re * v.re + v.im * v.im;
+            amx = fmax(amx, fmax(fabs(v.re), fabs(v.im)));
+        }
+        ssq = group_sum<64>(ssq);
+        amx = group_max<64>(amx);
+        if ((tid & 63) == 0) { red[tid >> 6] = ssq; red[(NT >> 6) + (tid >> 6)] = amx; }
+        __syncthreads();
+        ssq = 0.0;
+        amx = 0.0;
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) { ssq += red[w]; amx = fmax(amx, red[(NT >> 6) + w]); }
+        const zd alpha = A[lay(i + 1, i)];
+        double xnorm = 0.0;
+        if (amx > 0.0) {
+            if (ssq > 1e-290 && ssq < 1e290) {
+                xnorm = sqrt(ssq);
+            } else {
+                const double ia = 1.0 / amx;
+                double s2 = 0.0;
+                for (int r = i + 2 + tid; r < n; r += NT) {
+                    zd v = A[lay(r, i)];
+                    s2 += (v.re * ia) * (v.re * ia) + (v.im * ia) * (v.im * ia);
+                }
+                s2 = group_sum<64>(s2);
+                __syncthreads();
+                if ((tid & 63) == 0) red[tid >> 6] = s2;
+                __syncthreads();
+                s2 = 0.0;
+#pragma unroll
+                for (int w = 0; w < NT / 64; ++w) s2 += red[w];
+                xnorm = amx * sqrt(s2);
+            }
+        }
+        __syncthreads();   // everyone has read red[] and alpha
+        if (xnorm == 0.0 && alpha.im == 0.0) continue;   // tau = 0, H(i) = I
+        double w = fmax(fmax(fabs(alpha.re), fabs(alpha.im)), xnorm);
+        double iw = 1.0 / w;
+        double ar = alpha.re * iw, ai = alpha.im * iw, xn = xnorm * iw;
+        const double beta = -copysign(w * sqrt(ar * ar + ai * ai + xn * xn), alpha.re);
+        const zd tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
+        const zd scal = zdiv(mk(1.0, 0.0), mk(alpha.re - beta, alpha.im));
+        for (int r = i + 2 + tid; r < n; r += NT) A[lay(r, i)] = zmul(A[lay(r, i)], scal);
+        __syncthreads();
+        const zd *vcol = A + lay(i + 1, i);   // v(c) = vcol[c], c >= 1, v(0) = 1
+        const int ld = lay.ld;
+        // right: rows r = row (+ NT/4 ...), quad lanes split the columns
+        for (int r = row; r < n; r += NT / 4) {
+            zd *rp = A + r;
+            zd u = q == 0 ? rp[(i + 1) * ld] : mk(0.0, 0.0);
+            for (int c = (q == 0 ? 4 : q); c < m; c += 4) u = zadd(u, zmul(rp[(i + 1 + c) * ld], vcol[c]));
+            u.re += __shfl_xor(u.re, 1, 4);
+            u.im += __shfl_xor(u.im, 1, 4);
+            u.re += __shfl_xor(u.re, 2, 4);
+            u.im += __shfl_xor(u.im, 2, 4);
+            u = zmul(u, tau);
+            if (q == 0) rp[(i + 1) * ld] = zsub(rp[(i + 1) * ld], u);
+            for (int c = (q == 0 ? 4 : q); c < m; c += 4)
+                rp[(i + 1 + c) * ld] = zsub(rp[(i + 1 + c) * ld], zmul(u, zconj(vcol[c])));
+        }
+        __syncthreads();
+        // left: columns c = i+1+row (...), quad lanes split the rows
+        for (int c = i + 1 + row; c < n; c += NT / 4) {
+            zd *col = A + lay(i + 1, c);
+            zd sv = q == 0 ? col[0] : mk(0.0, 0.0);
+            for (int r = (q == 0 ? 4 : q); r < m; r += 4) sv = zadd(sv, zmulc(vcol[r], col[r]));
+            sv.re += __shfl_xor(sv.re, 1, 4);
+            sv.im += __shfl_xor(sv.im, 1, 4);
+            sv.re += __shfl_xor(sv.re, 2, 4);
+            sv.im += __shfl_xor(sv.im, 2, 4);
+            sv = zmulc(tau, sv);
+            if (q == 0) col[0] = zsub(col[0], sv);
+            for (int r = (q == 0 ? 4 : q); r < m; r += 4) col[r] = zsub(col[r], zmul(vcol[r], sv));
+        }
+        __syncthreads();
+        if (tid == 0) A[lay(i + 1, i)] = mk(beta, 0.0);
+        for (int r = i + 2 + tid; r < n; r += NT) A[lay(r, i)] = mk(0.0, 0.0);
+        __syncthreads();
+    }
+}
+
 // Q = H(0) H(1) ... H(n-2) from the reflectors kept by hessenberg<LPM, true>
 // (LAPACK ZUNGHR), accumulated backwards: Q <- H(i) Q acts on rows and
 // columns i+1..n-1 only.  Afterwards the reflector storage in A is zeroed.
@@ -398,14 +496,24 @@ __device__ __forceinline__ int group_last_true(int t, int lo, int hi, int none, 
 // non-null, Z <- Z * G for every reflector (WANTZ).  Returns 0 or 1 + the
 // 0-based index of the first eigenvalue that did not converge (LAPACK INFO).
 // On return the eigenvalues are H(i,i).
+// ZH ("Z on the upper half-wave", requires S == 1 and n <= LPM/2): lanes
+// t < LPM/2 are the H positions, lanes t >= LPM/2 own row t - LPM/2 of Z and
+// apply the reflectors to it in the same instructions that update H.
 struct NoZ {
+    __device__ __forceinline__ int cb(int) const { return 0; }
     __device__ __forceinline__ int operator()(int, int) const { return 0; }
 };
 
-template <int LPM, int S, bool WANTT, class Lay, class ZLay = NoZ>
+template <int LPM, int S, bool WANTT, bool ZH = false, class Lay, class ZLay = NoZ>
 __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_override, zd *Z = nullptr,
                            const ZLay zlay = ZLay())
 {
+    static_assert(!ZH || (S == 1 && WANTT), "half-wave Z needs one position per lane");
+    constexpr int HALF = LPM / 2;
+    const bool zlane = ZH && t >= HALF;            // this lane holds a row of Z
+    const int zrow = ZH ? (t >= HALF ? t - HALF : 0) : 0;
+    const bool zact = zlane && zrow < n;
+    const int tp = zlane ? (1 << 20) : t;          // H position base (none for Z lanes)
     const double ulp = DBL_EPSILON, safmin = DBL_MIN;
     const double smlnum = safmin * ((double)n / ulp);
     const int itmax = itmax_override >= 0 ? itmax_override : 30 * (n > 10 ? n : 10);
@@ -463,13 +571,14 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                 double h10 = H[lay(p, p - 1)].re;
                 return fabs(h10) * fabs(h21) <= ulp * (cabs1(h11s) * (cabs1(h11) + cabs1(h22)));
             });
-            zd x1, x2;
+            zd x1;
+            double x2;
             {
                 zd h11s = zsub(H[lay(m, m)], sh);
                 double h21 = H[lay(m + 1, m)].re;
                 double s = cabs1(h11s) + fabs(h21);
                 x1 = mk(h11s.re / s, h11s.im / s);
-                x2 = mk(h21 / s, 0.0);
+                x2 = h21 / s;
             }
             // ---- single-shift sweep k = m .. i-1; H rows I1.., columns ..I2
             // State between steps: reflector input (x1, x2), window
@@ -480,13 +589,16 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
             int cbp[S];   // column base of the position's own column
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                const int p = t + s * LPM;
+                const int p = tp + s * LPM;
                 cbp[s] = lay.cb(p < n ? p : n - 1);
                 carry[s] = mk(0.0, 0.0);
                 ypre[s] = mk(0.0, 0.0);
                 if (p >= m + 2 && p <= I2) { carry[s] = H[cbp[s] + m]; ypre[s] = H[cbp[s] + m + 1]; }
                 else if (p >= I1 && p <= m - 1) { carry[s] = H[lay(p, m)]; ypre[s] = H[lay(p, m + 1)]; }
-                if constexpr (WANTT) {
+                if constexpr (ZH) {
+                    if (zact) { carry[s] = Z[zlay(zrow, m)]; ypre[s] = Z[zlay(zrow, m + 1)]; }
+                }
+                if constexpr (WANTT && !ZH) {
                     zc[s] = mk(0.0, 0.0);
                     zy[s] = mk(0.0, 0.0);
                     if (wantz && p < n) { zc[s] = Z[zlay(p, m)]; zy[s] = Z[zlay(p, m + 1)]; }
@@ -496,19 +608,18 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
             zd e = (m + 2 <= i) ? H[lay(m + 2, m + 1)] : mk(0.0, 0.0);
 
             // one chase step kk; returns the new bulge H(kk+2, kk)
-            auto step = [&](const int kk, const bool first, const bool has_e) -> zd {
+            auto step = [&](const int kk, const bool first, const bool has_e) -> double {
                 zd t1, v2;
-                double beta;
+                double t2, beta;
                 bool ident;
-                zlarfg2(x1, x2, t1, v2, beta, ident);
-                const double t2 = t1.re * v2.re - t1.im * v2.im;   // DBLE(T1*V2)
+                zlarfg2(x1, x2, t1, v2, t2, beta, ident);
                 const int cbk = lay.cb(kk);
                 // bulk: left on columns kk+2 <= p <= I2, right on rows I1 <= p <= kk-1
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const int p = t + s * LPM;
+                    const int p = tp + s * LPM;
                     const bool rowop = (p >= kk + 2 && p <= I2);
-                    const bool colop = (p >= I1 && p <= kk - 1);
+                    const bool colop = (p >= I1 && p <= kk - 1) || zact;
                     if (rowop || colop) {
                         const zd c1 = mk(t1.re, rowop ? -t1.im : t1.im);
                         const zd vv = mk(v2.re, rowop ? v2.im : -v2.im);
@@ -517,12 +628,14 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                         if (rowop) {
                             H[cbp[s] + kk] = x;
                             H[cbp[s] + kk + 1] = y;
+                        } else if (ZH && zlane) {
+                            Z[zlay.cb(kk) + zrow] = x;
                         } else {
                             H[cbk + p] = x;
                         }
                         carry[s] = y;
                     }
-                    if constexpr (WANTT) {
+                    if constexpr (WANTT && !ZH) {
                         if (wantz && p < n) {
                             zd x = zc[s], y = zy[s];
                             pair_op(x, y, t1, t2, zconj(v2));
@@ -538,22 +651,22 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                 pair_op(b, d, tc, t2, v2);    // left: column kk+1
                 pair_op(a, b, t1, t2, vc);    // right: row kk
                 pair_op(c, d, t1, t2, vc);    // right: row kk+1
-                zd f = mk(0.0, 0.0);
-                if (has_e) {                  // right: row kk+2, H(kk+2,kk) = 0 before
-                    const double sr = t2 * e.re, si = t2 * e.im;
-                    f = mk(-sr, -si);
-                    e.re = fma(-sr, vc.re, fma(si, vc.im, e.re));
-                    e.im = fma(-sr, vc.im, fma(-si, vc.re, e.im));
+                double f = 0.0;
+                if (has_e) {                  // right: row kk+2, H(kk+2,kk) = 0 before, e real
+                    const double sr = t2 * e.re;
+                    f = -sr;
+                    e.re = fma(-sr, vc.re, e.re);
+                    e.im = -sr * vc.im;
                 }
                 if (t == 0) H[cbk + kk] = a;
 #pragma unroll
                 for (int s = 0; s < S; ++s)
-                    if (t + s * LPM == kk) carry[s] = b;
+                    if (tp + s * LPM == kk) carry[s] = b;
                 lds_order();
                 return f;
             };
             // move the state from step kk to step k1 = kk+1 (< i)
-            auto advance = [&](const int kk, const zd f) {
+            auto advance = [&](const int kk, const double f) {
                 const int k1 = kk + 1;
                 const int cb1 = lay.cb(k1 + 1);
                 x1 = c;
@@ -565,25 +678,34 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                 e = H[cb1 + (k1 + 2 <= i ? k1 + 2 : k1 + 1)];   // unused at the last step
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const int p = t + s * LPM;
+                    const int p = tp + s * LPM;
                     const bool rowop = (p >= k1 + 2 && p <= I2);
                     const int pc = p < k1 ? p : k1;
-                    ypre[s] = H[rowop ? cbp[s] + k1 + 1 : cb1 + pc];
-                    if constexpr (WANTT) {
+                    if constexpr (ZH) {
+                        const zd *src = zlane ? Z + zlay.cb(k1 + 1) + (zrow < n ? zrow : n - 1)
+                                              : H + (rowop ? cbp[s] + k1 + 1 : cb1 + pc);
+                        ypre[s] = *src;
+                    } else {
+                        ypre[s] = H[rowop ? cbp[s] + k1 + 1 : cb1 + pc];
+                    }
+                    if constexpr (WANTT && !ZH) {
                         if (wantz) zy[s] = Z[zlay(p < n ? p : n - 1, k1 + 1)];
                     }
                 }
             };
 
-            zd f = step(m, true, m + 2 <= i);
+            double f = step(m, true, m + 2 <= i);
             if (m > l) {
                 // ZLAHQR keeps H(m, m-1) real after a start at m > l:
                 // flush the step's state, scale in LDS, reload.
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const int p = t + s * LPM;
+                    const int p = tp + s * LPM;
                     if (p >= I1 && p <= m - 1) H[lay(p, m + 1)] = carry[s];
-                    if constexpr (WANTT) {
+                    if constexpr (ZH) {
+                        if (zact) Z[zlay(zrow, m + 1)] = carry[s];
+                    }
+                    if constexpr (WANTT && !ZH) {
                         if (wantz && p < n) Z[zlay(p, m + 1)] = zc[s];
                     }
                 }
@@ -595,9 +717,9 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                 }
                 lds_order();
                 zd t1, v2;
-                double beta;
+                double t2r, beta;
                 bool ident;
-                zlarfg2(x1, x2, t1, v2, beta, ident);   // the reflector of step m again
+                zlarfg2(x1, x2, t1, v2, t2r, beta, ident);   // the reflector of step m again
                 zd temp = zsub(mk(1.0, 0.0), t1);
                 temp = zscale(1.0 / zabs(temp), temp);
                 const zd tempc = zconj(temp);
@@ -631,10 +753,13 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                     e = (k1 + 2 <= i) ? H[lay(k1 + 2, k1 + 1)] : mk(0.0, 0.0);
 #pragma unroll
                     for (int s = 0; s < S; ++s) {
-                        const int p = t + s * LPM;
+                        const int p = tp + s * LPM;
                         if (p >= k1 + 2 && p <= I2) { carry[s] = H[lay(k1, p)]; ypre[s] = H[lay(k1 + 1, p)]; }
                         else if (p >= I1 && p <= k1 - 1) { carry[s] = H[lay(p, k1)]; ypre[s] = H[lay(p, k1 + 1)]; }
-                        if constexpr (WANTT) {
+                        if constexpr (ZH) {
+                            if (zact) { carry[s] = Z[zlay(zrow, k1)]; ypre[s] = Z[zlay(zrow, k1 + 1)]; }
+                        }
+                        if constexpr (WANTT && !ZH) {
                             if (wantz && p < n) { zc[s] = Z[zlay(p, k1)]; zy[s] = Z[zlay(p, k1 + 1)]; }
                         }
                     }
@@ -643,10 +768,13 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                     d = H[lay(i, i)];
 #pragma unroll
                     for (int s = 0; s < S; ++s) {
-                        const int p = t + s * LPM;
+                        const int p = tp + s * LPM;
                         if (p >= I1 && p <= i - 1) carry[s] = H[lay(p, i)];
                         else if (p >= i + 1 && p <= I2) carry[s] = H[lay(i, p)];
-                        if constexpr (WANTT) {
+                        if constexpr (ZH) {
+                            if (zact) carry[s] = Z[zlay(zrow, i)];
+                        }
+                        if constexpr (WANTT && !ZH) {
                             if (wantz && p < n) zc[s] = Z[zlay(p, i)];
                         }
                     }
@@ -671,10 +799,15 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
             }
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                const int p = t + s * LPM;
+                const int p = tp + s * LPM;
                 if (p >= I1 && p <= i - 1) H[lay(p, i)] = rot ? zmul(carry[s], temp) : carry[s];
                 if constexpr (WANTT) {
                     if (p >= i + 1 && p <= I2) H[lay(i, p)] = rot ? zmul(carry[s], zconj(temp)) : carry[s];
+                }
+                if constexpr (ZH) {
+                    if (zact) Z[zlay(zrow, i)] = rot ? zmul(carry[s], temp) : carry[s];
+                }
+                if constexpr (WANTT && !ZH) {
                     if (wantz && p < n) Z[zlay(p, i)] = rot ? zmul(zc[s], temp) : zc[s];
                 }
             }

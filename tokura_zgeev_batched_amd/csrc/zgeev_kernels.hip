@@ -77,6 +77,37 @@ __global__ __launch_bounds__(64) void k_hessenberg_packed(int n, int batch, cons
     if (t == 0) flag[mat] = ok ? 0 : 2;
 }
 
+// Stage 1 of the split path with a whole 256-thread workgroup per matrix
+// (quads of threads share each row/column): 8 waves per CU at NB = 64.
+template <int NB>
+__global__ __launch_bounds__(256) void k_hessenberg_packed_wg(int n, int batch, const zd *__restrict__ A,
+                                                              zd *__restrict__ work, int8_t *__restrict__ flag)
+{
+    constexpr int LDP = NB | 1;
+    __shared__ zd lds[NB * LDP];
+    __shared__ double red[8];
+    const long long mat = blockIdx.x;
+    const int tid = threadIdx.x;
+    const FullLayout lay{n | 1};
+    const long long nn = (long long)n * n;
+    const zd *src = A + mat * nn;
+    bool bad = false;
+    for (int e = tid; e < nn; e += 256) {
+        const zd v = src[e];
+        bad |= !zfinite(v);
+        lds[lay(e % n, e / n)] = v;
+    }
+    const int ok = __syncthreads_or(bad) == 0;
+    if (ok) hessenberg_wg<256>(lds, lay, n, red);
+    zd *dst = work + mat * (long long)packed_size(n);
+    const PackedLayout pk;
+    for (int c = 0; c < n; ++c) {
+        const int rows = (c + 2 < n ? c + 2 : n);
+        for (int r = tid; r < rows; r += 256) dst[pk(r, c)] = lds[lay(r, c)];
+    }
+    if (tid == 0) flag[mat] = ok ? 0 : 2;
+}
+
 // Stage 2 of the split path: QR on the packed Hessenberg matrix.
 template <int NB, int LPM>
 __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__restrict__ work,
@@ -155,7 +186,10 @@ static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *f
     constexpr int GPW = 64 / LPM;
     const int blocks = (batch + GPW - 1) / GPW;
     *stage = 1;
-    hipLaunchKernelGGL((k_hessenberg_packed<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, A, work, flag);
+    if (NB >= 48)   // large matrices: one 256-thread workgroup per matrix
+        hipLaunchKernelGGL((k_hessenberg_packed_wg<NB>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag);
+    else
+        hipLaunchKernelGGL((k_hessenberg_packed<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, A, work, flag);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (mid) hipEventRecord(mid, s);

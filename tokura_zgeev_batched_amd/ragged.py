@@ -84,10 +84,9 @@ class RaggedBatch:
         return self.config()
 
     def config(self):
-        c, a, b, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(lib().tokuraGetRaggedConfig(self.h.ptr, ctypes.byref(c), ctypes.byref(a), ctypes.byref(b),
-                                           ctypes.byref(d)), "tokuraGetRaggedConfig")
-        return c.value, (a.value, b.value, d.value)
+        v = [ctypes.c_int() for _ in range(5)]
+        _check(lib().tokuraGetRaggedConfig(self.h.ptr, *[ctypes.byref(x) for x in v]), "tokuraGetRaggedConfig")
+        return v[0].value, tuple(x.value for x in v[1:])
 
     def split(self, eig_flat: torch.Tensor):
         """List of per-matrix eigenvalue tensors."""
