@@ -146,6 +146,45 @@ def bench_ragged(dev, B=200_000, lo=4, hi=64, steps=3, seed=1):
             "max_rel_diff_ragged_vs_plain_n64_sample": agree}
 
 
+def bench_hard(dev, B=100_000, lo=8, hi=64, steps=3, seed=5, sample=1200):
+    """Hard spectra (half companions, half clustered with kappa(V) ~ 1e4),
+    ragged n ~ U{lo..hi}, through the persistent ragged path (hipGraph)."""
+    from tokura_zgeev_batched_amd.ragged import RaggedBatch, pack
+    from tokura_zgeev_batched_amd.workloads import hard_batch
+    import oracle
+    mats, kinds = hard_batch(B, lo, hi, seed)
+    flat, offs = pack(mats)
+    h = tk.Handle()
+    rb = RaggedBatch(offs, handle=h)
+    A = torch.tensor(flat, device=dev)
+    rb.autotune(A)
+    eig = torch.empty(int(rb.eig_offsets[-1]), dtype=torch.complex128, device=dev)
+    flag = torch.empty(B, dtype=torch.int8, device=dev)
+    t = timed(lambda: rb.solve(A, eig, flag, graph=True), steps, 1)
+    f = flag.cpu().numpy().astype(np.int64)
+    hist = {str(k): int(v) for k, v in enumerate(np.bincount(f, minlength=3))}
+    e = eig.cpu().numpy()
+    rng = np.random.default_rng(seed + 1)
+    pick = np.sort(rng.choice(B, size=min(sample, B), replace=False))
+    sizes = np.array([mats[b].shape[0] for b in pick])
+    worst = {"companion": 0.0, "clustered": 0.0}
+    for n in np.unique(sizes):
+        idx = pick[sizes == n]
+        M = np.stack([mats[b] for b in idx])
+        got = np.stack([e[rb.eig_offsets[b]:rb.eig_offsets[b + 1]] for b in idx])
+        err = oracle.condscaled_err(M, got).max(axis=1)
+        for b, x in zip(idx, err):
+            worst[kinds[b]] = max(worst[kinds[b]], float(x))
+    flops = float(sum(40.0 * m.shape[0] ** 3 for m in mats))
+    return {"workload": f"hard spectra, batch={B}, n~U{{{lo}..{hi}}} seed {seed}: half companions of monic "
+                        "polynomials with iid complex N(0,1) coefficients, half V diag(lambda) V^-1 with 3 "
+                        "clusters of spread 1e-3 and kappa(V)=1e4",
+            "matrices_per_s": B / t, "ms": t * 1e3, "model_tflops": flops / t / 1e12, "flag_histogram": hist,
+            "condition_scaled_err_max": worst,
+            "condition_scaled_err_def": "|lambda - lambda_zgeevx| * rcond / (eps ||A||_F), unbalanced zgeevx",
+            "accuracy_sample": int(len(pick))}
+
+
 def bench_diff(dev, B=50_000, n=32, steps=3, seed=3):
     """Forward (eigenvalues only) vs forward with Schur vectors + adjoint."""
     from tokura_zgeev_batched_amd.autograd import backward_batched, schur_batched
@@ -195,6 +234,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the ragged and differentiation legs")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch table")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the batched eigenvalue path shards by matrices "
@@ -232,6 +274,12 @@ def main():
     wall = time.perf_counter() - t0
     calls, hess_ms, qr_ms = h.stage_times()
     h.set_stage_timing(False)
+    if args.copy_probe_mib > 0:
+        src = torch.empty(args.copy_probe_mib * (1 << 20), dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     nflag = int((flag != 0).sum())
     value = B * args.steps / wall
     ms_step = wall * 1e3 / args.steps
@@ -282,7 +330,8 @@ def main():
     if not args.no_extra:
         del A, eig, flag, work
         torch.cuda.empty_cache()
-        result["extra"] = {"ragged": bench_ragged(dev), "differentiation": bench_diff(dev)}
+        result["extra"] = {"ragged": bench_ragged(dev), "differentiation": bench_diff(dev),
+                           "hard_spectra": bench_hard(dev)}
     print(json.dumps(result), flush=True)
 
 

@@ -213,3 +213,24 @@ def relerr_matched(ref: np.ndarray, got: np.ndarray, scale: np.ndarray | None = 
 def max_relerr(ref, got) -> float:
     e = relerr_matched(ref, got)
     return float(e.max()) if e.size else 0.0
+
+
+def condscaled_err(M, got):
+    """Condition-scaled eigenvalue errors of one batch of same-size matrices.
+
+    ref = LAPACK zgeevx eigenvalues of the unbalanced matrices (the GPU
+    method does not balance), matched optimally to `got`; error of
+    eigenvalue i = |got_i - ref_i| * rconde_i / (eps * ||A||_F), i.e. the
+    error in units of the first-order perturbation bound kappa_i eps ||A||.
+    Returns an array (batch, n)."""
+    M = np.asarray(M, np.complex128)
+    if M.ndim == 2:
+        M = M[None]
+    w, rc, info = lapack_rconde(M)
+    eps = np.finfo(np.float64).eps
+    out = np.zeros(w.shape)
+    for b in range(M.shape[0]):
+        perm = match(w[b], np.asarray(got[b]))
+        g = np.asarray(got[b])[perm]
+        out[b] = np.abs(g - w[b]) * rc[b] / (eps * np.linalg.norm(M[b]))
+    return out

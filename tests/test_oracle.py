@@ -208,3 +208,34 @@ def test_adjoint_vs_finite_differences():
             fd = (loss(A + h * E) - loss(A - h * E)) / (2 * h)
             an = np.real(np.conj(G[r, c]) * unit)   # dL = Re(conj(G) dA)
             assert abs(fd - an) < 1e-6 * max(1.0, abs(an))
+
+
+def test_hard_spectra_generator_and_condition_scaled_metric():
+    from tokura_zgeev_batched_amd.workloads import hard_batch, hard_spectrum, companion
+    rng = np.random.default_rng(13)
+    # companion: eigenvalues are the polynomial roots
+    c = rng.standard_normal(7) + 1j * rng.standard_normal(7)
+    e, _ = oracle.method_zgeev(companion(c))
+    roots = np.roots(np.concatenate([[1.0], c[::-1]]))
+    assert oracle.max_relerr(roots[None], e) < 1e-10
+    # clustered similarity: kappa(V) = 1e4 and three clusters of spread ~1e-3
+    A, lam = hard_spectrum(24, rng)
+    e, _ = oracle.method_zgeev(A)
+    assert np.max(np.abs(e[0][oracle.match(lam, e[0])] - lam)) < 1e-6
+    mats, kinds = hard_batch(60, 8, 30, 3)
+    assert kinds[0] == "companion" and kinds[1] == "clustered"
+    worst = 0.0
+    for M in mats:
+        e, f = oracle.method_zgeev(M)
+        assert f[0] == 0
+        worst = max(worst, oracle.condscaled_err(M, e).max())
+    assert worst < 100.0
+    # the metric detects an error of 1e3 condition units
+    M = mats[1]
+    e, _ = oracle.method_zgeev(M)
+    w, rc, _ = oracle.lapack_rconde(M[None])
+    bad = e.copy()
+    j = int(np.argmax(rc[0]))
+    i = int(np.argmin(np.abs(bad[0] - w[0][j])))
+    bad[0, i] += 1e4 * np.finfo(float).eps * np.linalg.norm(M) / rc[0][j]
+    assert oracle.condscaled_err(M, bad).max() > 1e3

@@ -296,15 +296,14 @@ __device__ void hessenberg(zd *A, const FullLayout lay, int n, int t, zd *taus =
 }
 
 // ---- ZGEHD2 by a whole workgroup of NT threads on one LDS matrix ----
-// Same arithmetic order per element as hessenberg<>() except that each
-// row/column dot product is split over 4 consecutive threads (a quad) and
-// combined with two shuffles.  Workgroup barriers separate the phases.
-// `red` is LDS scratch of at least NT/64 doubles.
-template <int NT>
+// Same algorithm as hessenberg<>(); each row/column dot product is split over
+// TPR consecutive threads and combined with shuffles.  Workgroup barriers
+// separate the phases.  `red` is LDS scratch of at least 2*NT/64 doubles.
+template <int NT, int TPR>
 __device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red)
 {
     const int tid = threadIdx.x;
-    const int q = tid & 3, row = tid >> 2;        // quad lane, row/column slot
+    const int q = tid % TPR, row = tid / TPR;     // lane within the row team, row/column slot
     for (int i = 0; i + 2 <= n; ++i) {
         const int m = n - 1 - i;
         double ssq = 0.0, amx = 0.0;
@@ -356,32 +355,34 @@ __device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red)
         const zd *vcol = A + lay(i + 1, i);   // v(c) = vcol[c], c >= 1, v(0) = 1
         const int ld = lay.ld;
         // right: rows r = row (+ NT/4 ...), quad lanes split the columns
-        for (int r = row; r < n; r += NT / 4) {
+        for (int r = row; r < n; r += NT / TPR) {
             zd *rp = A + r;
             zd u = q == 0 ? rp[(i + 1) * ld] : mk(0.0, 0.0);
-            for (int c = (q == 0 ? 4 : q); c < m; c += 4) u = zadd(u, zmul(rp[(i + 1 + c) * ld], vcol[c]));
-            u.re += __shfl_xor(u.re, 1, 4);
-            u.im += __shfl_xor(u.im, 1, 4);
-            u.re += __shfl_xor(u.re, 2, 4);
-            u.im += __shfl_xor(u.im, 2, 4);
+            for (int c = (q == 0 ? TPR : q); c < m; c += TPR) u = zadd(u, zmul(rp[(i + 1 + c) * ld], vcol[c]));
+#pragma unroll
+            for (int o = 1; o < TPR; o <<= 1) {
+                u.re += __shfl_xor(u.re, o, TPR);
+                u.im += __shfl_xor(u.im, o, TPR);
+            }
             u = zmul(u, tau);
             if (q == 0) rp[(i + 1) * ld] = zsub(rp[(i + 1) * ld], u);
-            for (int c = (q == 0 ? 4 : q); c < m; c += 4)
+            for (int c = (q == 0 ? TPR : q); c < m; c += TPR)
                 rp[(i + 1 + c) * ld] = zsub(rp[(i + 1 + c) * ld], zmul(u, zconj(vcol[c])));
         }
         __syncthreads();
         // left: columns c = i+1+row (...), quad lanes split the rows
-        for (int c = i + 1 + row; c < n; c += NT / 4) {
+        for (int c = i + 1 + row; c < n; c += NT / TPR) {
             zd *col = A + lay(i + 1, c);
             zd sv = q == 0 ? col[0] : mk(0.0, 0.0);
-            for (int r = (q == 0 ? 4 : q); r < m; r += 4) sv = zadd(sv, zmulc(vcol[r], col[r]));
-            sv.re += __shfl_xor(sv.re, 1, 4);
-            sv.im += __shfl_xor(sv.im, 1, 4);
-            sv.re += __shfl_xor(sv.re, 2, 4);
-            sv.im += __shfl_xor(sv.im, 2, 4);
+            for (int r = (q == 0 ? TPR : q); r < m; r += TPR) sv = zadd(sv, zmulc(vcol[r], col[r]));
+#pragma unroll
+            for (int o = 1; o < TPR; o <<= 1) {
+                sv.re += __shfl_xor(sv.re, o, TPR);
+                sv.im += __shfl_xor(sv.im, o, TPR);
+            }
             sv = zmulc(tau, sv);
             if (q == 0) col[0] = zsub(col[0], sv);
-            for (int r = (q == 0 ? 4 : q); r < m; r += 4) col[r] = zsub(col[r], zmul(vcol[r], sv));
+            for (int r = (q == 0 ? TPR : q); r < m; r += TPR) col[r] = zsub(col[r], zmul(vcol[r], sv));
         }
         __syncthreads();
         if (tid == 0) A[lay(i + 1, i)] = mk(beta, 0.0);

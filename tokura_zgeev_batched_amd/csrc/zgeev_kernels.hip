@@ -77,33 +77,34 @@ __global__ __launch_bounds__(64) void k_hessenberg_packed(int n, int batch, cons
     if (t == 0) flag[mat] = ok ? 0 : 2;
 }
 
-// Stage 1 of the split path with a whole 256-thread workgroup per matrix
-// (quads of threads share each row/column): 8 waves per CU at NB = 64.
-template <int NB>
-__global__ __launch_bounds__(256) void k_hessenberg_packed_wg(int n, int batch, const zd *__restrict__ A,
-                                                              zd *__restrict__ work, int8_t *__restrict__ flag)
+// Stage 1 of the split path with a whole workgroup of NT threads per matrix
+// (teams of NT/64 threads share each row/column): at NB = 64 two workgroups
+// of 512 threads fit per CU, four waves per SIMD.
+template <int NB, int NT>
+__global__ __launch_bounds__(NT) void k_hessenberg_packed_wg(int n, int batch, const zd *__restrict__ A,
+                                                             zd *__restrict__ work, int8_t *__restrict__ flag)
 {
     constexpr int LDP = NB | 1;
     __shared__ zd lds[NB * LDP];
-    __shared__ double red[8];
+    __shared__ double red[2 * NT / 64];
     const long long mat = blockIdx.x;
     const int tid = threadIdx.x;
     const FullLayout lay{n | 1};
     const long long nn = (long long)n * n;
     const zd *src = A + mat * nn;
     bool bad = false;
-    for (int e = tid; e < nn; e += 256) {
+    for (int e = tid; e < nn; e += NT) {
         const zd v = src[e];
         bad |= !zfinite(v);
         lds[lay(e % n, e / n)] = v;
     }
     const int ok = __syncthreads_or(bad) == 0;
-    if (ok) hessenberg_wg<256>(lds, lay, n, red);
+    if (ok) hessenberg_wg<NT, NT / 64>(lds, lay, n, red);
     zd *dst = work + mat * (long long)packed_size(n);
     const PackedLayout pk;
     for (int c = 0; c < n; ++c) {
         const int rows = (c + 2 < n ? c + 2 : n);
-        for (int r = tid; r < rows; r += 256) dst[pk(r, c)] = lds[lay(r, c)];
+        for (int r = tid; r < rows; r += NT) dst[pk(r, c)] = lds[lay(r, c)];
     }
     if (tid == 0) flag[mat] = ok ? 0 : 2;
 }
@@ -186,8 +187,8 @@ static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *f
     constexpr int GPW = 64 / LPM;
     const int blocks = (batch + GPW - 1) / GPW;
     *stage = 1;
-    if (NB >= 48)   // large matrices: one 256-thread workgroup per matrix
-        hipLaunchKernelGGL((k_hessenberg_packed_wg<NB>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag);
+    if (NB >= 48)   // large matrices: one 512-thread workgroup per matrix
+        hipLaunchKernelGGL((k_hessenberg_packed_wg<NB, 512>), dim3(batch), dim3(512), 0, s, n, batch, A, work, flag);
     else
         hipLaunchKernelGGL((k_hessenberg_packed<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, A, work, flag);
     hipError_t e = hipGetLastError();
