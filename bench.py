@@ -35,6 +35,11 @@ import tokura_zgeev_batched_amd as tk  # noqa: E402
 
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
+# HBM bytes per matrix moved by the QR kernel k_qr_packed<64,64>, from the
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes recorded in
+# profiles/qr_packed_64x64_n64_b20k_traffic.md (each counter scaled by its
+# reading for a 1 GiB device copy: FETCH x2.0, WRITE x1.0).
+QR64_TRAFFIC_BYTES_PER_MATRIX = 35_440.0
 
 
 def flops_per_solve(n: int) -> float:
@@ -306,7 +311,9 @@ def main():
                    "matrices_per_workgroup": mpb},
         "roofline": {"bound": "mfma", "achieved": qr_achieved / 1e12,
                      "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": qr_achieved / PEAK_FP64,
-                     "traffic": None,
+                     "traffic": (QR64_TRAFFIC_BYTES_PER_MATRIX * B if n == 64 else None),
+                     "traffic_unit": "bytes per launch (rocprof FETCH_SIZE+WRITE_SIZE, calibrated)",
+                     "algorithmic_bytes": (8.0 * (n - 1) * (n + 2) + 16.0 * n + 16.0 * n + 1.0) * B,
                      "kernel": "QR stage (k_qr_packed) charged 80/3 n^3 flop/matrix",
                      "compute_unit": "fp64 VALU; MI355X fp64 vector and matrix peaks coincide (78.6 TFLOP/s)",
                      "kernel_avg_ms": avg_qr_s * 1e3, "hessenberg_avg_ms": avg_hess_s * 1e3,

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(64) void k_hessenberg_packed(int n, int batch, cons
 
 // Stage 1 of the split path with a whole workgroup of NT threads per matrix
 // (teams of NT/64 threads share each row/column): at NB = 64 two workgroups
-// of 512 threads fit per CU, four waves per SIMD.
+// of 256 threads fit per CU, two waves per SIMD.
 template <int NB, int NT>
 __global__ __launch_bounds__(NT) void k_hessenberg_packed_wg(int n, int batch, const zd *__restrict__ A,
                                                              zd *__restrict__ work, int8_t *__restrict__ flag)
@@ -187,8 +187,8 @@ static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *f
     constexpr int GPW = 64 / LPM;
     const int blocks = (batch + GPW - 1) / GPW;
     *stage = 1;
-    if (NB >= 48)   // large matrices: one 512-thread workgroup per matrix
-        hipLaunchKernelGGL((k_hessenberg_packed_wg<NB, 512>), dim3(batch), dim3(512), 0, s, n, batch, A, work, flag);
+    if (NB >= 48)   // large matrices: one 256-thread workgroup per matrix (512 threads measured slower)
+        hipLaunchKernelGGL((k_hessenberg_packed_wg<NB, 256>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag);
     else
         hipLaunchKernelGGL((k_hessenberg_packed<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, A, work, flag);
     hipError_t e = hipGetLastError();
