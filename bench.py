@@ -190,6 +190,29 @@ def bench_hard(dev, B=100_000, lo=8, hi=64, steps=3, seed=5, sample=1200):
             "accuracy_sample": int(len(pick))}
 
 
+def bench_end_to_end(dev, sizes=(62, 64), B=16_000, chunk=8192, nstreams=8, seed=7):
+    """Host-resident batches through the streaming pipeline (H2D + solve + D2H
+    on 8 streams, 8192-matrix chunks), the reference harness's measurement
+    mode (stream_test_with_MKL/main.cu:245-296): PCIe-inclusive matrices/s."""
+    from tokura_zgeev_batched_amd.pipeline import pinned_colmajor, stream_eigvals
+    out = {}
+    g = torch.Generator().manual_seed(seed)
+    for n in sizes:
+        A = pinned_colmajor(B, n)
+        A.copy_(torch.complex(torch.randn(B, n * n, generator=g, dtype=torch.float64),
+                              torch.randn(B, n * n, generator=g, dtype=torch.float64)))
+        stream_eigvals(A, n, chunk, nstreams)   # warm-up
+        t0 = time.perf_counter()
+        eig, flag = stream_eigvals(A, n, chunk, nstreams)
+        dt = time.perf_counter() - t0
+        out[f"n{n}"] = {"matrices_per_s_incl_pcie": B / dt, "seconds_for_16000": dt,
+                        "flags_nonzero": int((flag != 0).sum())}
+    out["workload"] = (f"host-resident batch of {B} matrices per n, {nstreams} streams x {chunk}-matrix chunks, "
+                       "H2D + solve + D2H (PCIe-inclusive)")
+    out["reference_published_n62_matrices_per_s_incl_transfers"] = 3225.0
+    return out
+
+
 def bench_diff(dev, B=50_000, n=32, steps=3, seed=3):
     """Forward (eigenvalues only) vs forward with Schur vectors + adjoint."""
     from tokura_zgeev_batched_amd.autograd import backward_batched, schur_batched
@@ -239,6 +262,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the ragged and differentiation legs")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch table")
+    ap.add_argument("--only", choices=["ragged", "diff", "hard", "e2e"], default=None,
+                    help="run only this extra leg (for profiling); prints its JSON object")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
@@ -250,6 +275,10 @@ def main():
         sys.exit(2)
 
     dev = torch.device("cuda:0")
+    if args.only:
+        fn = {"ragged": bench_ragged, "diff": bench_diff, "hard": bench_hard, "e2e": bench_end_to_end}[args.only]
+        print(json.dumps({args.only: fn(dev)}), flush=True)
+        return
     n, B = args.n, args.batch
     A = make_batch(B, n, 0, dev)
     eig = torch.empty((B, n), dtype=torch.complex128, device=dev)
@@ -338,7 +367,7 @@ def main():
         del A, eig, flag, work
         torch.cuda.empty_cache()
         result["extra"] = {"ragged": bench_ragged(dev), "differentiation": bench_diff(dev),
-                           "hard_spectra": bench_hard(dev)}
+                           "hard_spectra": bench_hard(dev), "end_to_end": bench_end_to_end(dev)}
     print(json.dumps(result), flush=True)
 
 

@@ -3,8 +3,9 @@
 //
 // Pipeline (both launches are captured together by tokura_zgeev_ragged_graph):
 //  1. k_ragged_reset    zeroes the queue head (a fill kernel, graph-safe).
-//  2. k_ragged_hess     Householder Hessenberg of every matrix, by tiles,
-//                       full-storage LDS, packed result written to `work`.
+//  2. Householder Hessenberg of every matrix, packed result written to `work`:
+//     k_ragged_hess_wg  one 256-thread workgroup per matrix for n > 32,
+//     k_ragged_hess     one wavefront per tile of small matrices (n <= 32).
 //  3. k_ragged_qr       ONE persistent kernel: every wavefront pulls the next
 //                       tile from an atomic queue until the queue is empty, so
 //                       matrices that need more QR sweeps never hold up a
@@ -37,7 +38,9 @@ struct RaggedHeader {
     int cfg;
     long long off_tiles, off_order, off_n, off_moff, off_eoff, off_hoff, off_H;
     long long total_bytes;
-    long long pad[6];
+    int nbig;          // matrices of classes A1/A2 (n > 32): order[0 .. nbig)
+    int tile_small0;   // first tile of classes B/C
+    long long pad[5];
 };
 static_assert(sizeof(RaggedHeader) == 128, "header size");
 
@@ -124,21 +127,53 @@ __device__ __forceinline__ void qr_tile(zd *lds, const RaggedView &V, int4 td, z
 
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
-template <int LA1, int LA2, int LB, int LC>
+// Hessenberg of the small classes (B, C): one wavefront per tile.
+template <int LB, int LC>
 __global__ __launch_bounds__(64) void k_ragged_hess(const zd *__restrict__ A, char *work, int8_t *__restrict__ flag)
 {
-    constexpr int SZ = cmax(cmax((64 / LA1) * 64 * 65, (64 / LA2) * 48 * 49),
-                            cmax((64 / LB) * 32 * 33, (64 / LC) * 16 * 17));
+    constexpr int SZ = cmax((64 / LB) * 32 * 33, (64 / LC) * 16 * 17);
     __shared__ zd lds[SZ];
     const RaggedView V(work);
     const int nt = V.h->ntiles;
-    for (int tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    for (int tile = V.h->tile_small0 + blockIdx.x; tile < nt; tile += gridDim.x) {
         const int4 td = V.tiles[tile];
-        if (td.x == 0) hess_tile<64, LA1>(lds, V, td, A, flag);
-        else if (td.x == 1) hess_tile<48, LA2>(lds, V, td, A, flag);
-        else if (td.x == 2) hess_tile<32, LB>(lds, V, td, A, flag);
+        if (td.x == 2) hess_tile<32, LB>(lds, V, td, A, flag);
         else hess_tile<16, LC>(lds, V, td, A, flag);
         lds_order();
+    }
+}
+
+// Hessenberg of the large classes (A1, A2): one 256-thread workgroup per
+// matrix, quads of threads sharing each row/column.
+__global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A, char *work,
+                                                        int8_t *__restrict__ flag)
+{
+    __shared__ zd lds[64 * 65];
+    __shared__ double red[8];
+    const RaggedView V(work);
+    const int nbig = V.h->nbig;
+    const int tid = threadIdx.x;
+    for (int j = blockIdx.x; j < nbig; j += gridDim.x) {
+        const int b = V.order[j];
+        const int n = V.nsz[b];
+        const FullLayout lay{n | 1};
+        const zd *src = A + V.moff[b];
+        bool bad = false;
+        for (int e = tid; e < n * n; e += 256) {
+            const zd v = src[e];
+            bad |= !zfinite(v);
+            lds[lay(e % n, e / n)] = v;
+        }
+        const int ok = __syncthreads_or(bad) == 0;
+        if (ok) hessenberg_wg<256, 4>(lds, lay, n, red);
+        zd *dst = V.H + V.hoff[b];
+        const PackedLayout pk;
+        for (int c = 0; c < n; ++c) {
+            const int rows = (c + 2 < n ? c + 2 : n);
+            for (int r = tid; r < rows; r += 256) dst[pk(r, c)] = lds[lay(r, c)];
+        }
+        if (tid == 0) flag[b] = ok ? 0 : 2;
+        __syncthreads();
     }
 }
 
@@ -174,14 +209,19 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     // resident workgroups per CU, queried once (outside any graph capture the
     // first call is made from tokura_zgeev_ragged_graph's capture stream, where
     // occupancy queries are not stream operations and are allowed)
-    static int occ_h = 0, occ_q = 0;
+    static int occ_h = 0, occ_q = 0, occ_w = 0;
     if (occ_h == 0) {
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_h, k_ragged_hess<LA1, LA2, LB, LC>, 64, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_h, k_ragged_hess<LB, LC>, 64, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_w, k_ragged_hess_wg, 256, 0);
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_q, k_ragged_qr<LA1, LA2, LB, LC>, 64, 0);
         if (occ_h < 1) occ_h = 1;
+        if (occ_w < 1) occ_w = 1;
         if (occ_q < 1) occ_q = 1;
     }
-    hipLaunchKernelGGL((k_ragged_hess<LA1, LA2, LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag);
+    hipLaunchKernelGGL(k_ragged_hess_wg, dim3(cus * occ_w * 4), dim3(256), 0, s, A, work, flag);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_ragged_qr<LA1, LA2, LB, LC>), dim3(cus * occ_q), dim3(64), 0, s, work, eig, flag, itmax);
@@ -250,6 +290,10 @@ static int build_schedule(int batch, const int64_t *offsets, int cfg, Schedule &
     }
     const int ntiles = (int)(S.tiles.size() / 4);
     std::memset(&S.hdr, 0, sizeof S.hdr);
+    S.hdr.nbig = 0;
+    while (S.hdr.nbig < batch && S.nsz[S.order[S.hdr.nbig]] > 32) ++S.hdr.nbig;
+    S.hdr.tile_small0 = 0;
+    while (S.hdr.tile_small0 < ntiles && S.tiles[4 * S.hdr.tile_small0] < 2) ++S.hdr.tile_small0;
     S.hdr.ntiles = ntiles;
     S.hdr.batch = batch;
     S.hdr.cfg = cfg;
