@@ -251,6 +251,35 @@ def bench_diff(dev, B=50_000, n=32, steps=3, seed=3):
             "recompute_fwd_bwd_over_fwd_eigvals": (t_fwd + t_bwd_rc) / t_fwd}
 
 
+def bench_vectors(dev, cases=((32, 50_000), (64, 20_000)), steps=3, seed=11):
+    """Eigenvalues + right and left eigenvectors (ZGEEV 'V','V' analogue)."""
+    out = {"workload": "eigenvectors (left and right), same-size batches"}
+    h = tk.Handle()
+    for n, B in cases:
+        A = make_batch(B, n, seed, dev)
+        eig = torch.empty((B, n), dtype=torch.complex128, device=dev)
+        flag = torch.empty(B, dtype=torch.int8, device=dev)
+        VL = torch.empty((B, n, n), dtype=torch.complex128, device=dev)
+        VR = torch.empty((B, n, n), dtype=torch.complex128, device=dev)
+
+        def both():
+            rc = tk.zgeev_vectors(A, n, B, eig, VL, VR, flag, h)
+            if rc != 0:
+                raise RuntimeError(f"tokura_zgeev_batched_vectors_gpu returned {rc}")
+
+        def right():
+            tk.zgeev_vectors(A, n, B, eig, None, VR, flag, h)
+
+        t_both = timed(both, steps, 1)
+        t_right = timed(right, steps, 1)
+        out[f"n{n}"] = {"batch": B, "left_right_ms": t_both * 1e3, "right_only_ms": t_right * 1e3,
+                        "matrices_per_s_left_right": B / t_both, "matrices_per_s_right": B / t_right,
+                        "flags_nonzero": int((flag != 0).sum())}
+        del A, eig, flag, VL, VR
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,7 +291,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the ragged and differentiation legs")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch table")
-    ap.add_argument("--only", choices=["ragged", "diff", "hard", "e2e"], default=None,
+    ap.add_argument("--only", choices=["ragged", "diff", "hard", "e2e", "vectors"], default=None,
                     help="run only this extra leg (for profiling); prints its JSON object")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates the "
@@ -276,7 +305,8 @@ def main():
 
     dev = torch.device("cuda:0")
     if args.only:
-        fn = {"ragged": bench_ragged, "diff": bench_diff, "hard": bench_hard, "e2e": bench_end_to_end}[args.only]
+        fn = {"ragged": bench_ragged, "diff": bench_diff, "hard": bench_hard, "e2e": bench_end_to_end,
+              "vectors": bench_vectors}[args.only]
         print(json.dumps({args.only: fn(dev)}), flush=True)
         return
     n, B = args.n, args.batch
@@ -367,7 +397,8 @@ def main():
         del A, eig, flag, work
         torch.cuda.empty_cache()
         result["extra"] = {"ragged": bench_ragged(dev), "differentiation": bench_diff(dev),
-                           "hard_spectra": bench_hard(dev), "end_to_end": bench_end_to_end(dev)}
+                           "hard_spectra": bench_hard(dev), "end_to_end": bench_end_to_end(dev),
+                           "eigenvectors": bench_vectors(dev)}
     print(json.dumps(result), flush=True)
 
 

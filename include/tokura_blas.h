@@ -180,6 +180,22 @@ int tokura_zgeev_batched_backward_gpu(tokurablas_t *handle, int n, int batch_cou
                                       const hipDoubleComplex *gbar, hipDoubleComplex *dA,
                                       char *flag, hipStream_t stream);
 
+/* ---- eigenvectors -------------------------------------------------------
+ * No counterpart in the reference, which computes eigenvalues only
+ * (README.md:41-44); the analogue of LAPACK ZGEEV with JOBVL/JOBVR = 'V'
+ * (without balancing).  For every matrix j writes eigenvalues[i + j*n] and,
+ * where the pointer is non-NULL, right eigenvectors VR_j(:, i)
+ * (A_j v = lambda_i v) and left eigenvectors VL_j(:, i)
+ * (u^H A_j = lambda_i u^H), each n*n column-major per matrix, normalised as
+ * ZGEEV does: unit Euclidean norm, the component of largest modulus real and
+ * non-negative.  Eigenvalues are in the order of the Schur diagonal.  VL and
+ * VR may not both be NULL (-5).  flag as in tokura_zgeev_batched_gpu; a
+ * matrix with flag != 0 gets NaN vectors.  A is not modified; no workspace. */
+int tokura_zgeev_batched_vectors_gpu(tokurablas_t *handle, int n, int batch_count,
+                                     const hipDoubleComplex *A, hipDoubleComplex *eigenvalues,
+                                     hipDoubleComplex *VL, hipDoubleComplex *VR, char *flag,
+                                     hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,9 @@ Contents
 * ``lapack_rconde`` — LAPACK zgeevx eigenvalue condition numbers (balanc='N').
 * ``method_schur`` / ``method_adjoint`` — Schur form and dL/dA from the same
   method, used to check the adjoint kernel.
+* ``method_vectors`` — eigenvectors by the same method (ZTREVC on the Schur
+  form, back-transform, ZGEEV normalisation); ``lapack_zgeev_vectors`` —
+  host LAPACK zgeev('V', 'V') to pin it.
 * ``match_max_relerr`` — optimal-assignment eigenvalue matching; the
   reference's harness sorts lexicographically instead
   (stream_test_with_MKL/main.cu:16-181), which mis-pairs close eigenvalues.
@@ -72,6 +75,10 @@ def lib():
     L.oracle_zgehd2.restype = None
     L.oracle_ztrevc.argtypes = [I, P, P, P]
     L.oracle_ztrevc.restype = None
+    L.oracle_zgeev_vectors.argtypes = [I, P, P, P, P]
+    L.oracle_zgeev_vectors.restype = I
+    L.lapack_zgeev_vectors_batched.argtypes = [I, I, P, P, P, P, P, I]
+    L.lapack_zgeev_vectors_batched.restype = I
     L.lapack_open.argtypes = [ctypes.c_char_p]
     L.lapack_open.restype = I
     L.lapack_max_threads.restype = I
@@ -142,6 +149,44 @@ def method_adjoint(M, g):
     w = np.zeros(n, np.complex128)
     lib().oracle_zgeev_adjoint(n, _p(A), _p(g), _p(G), _p(w))
     return G.T.copy(), w
+
+
+def method_vectors(M, left: bool = True, right: bool = True):
+    """Eigenvalues and eigenvectors by the restated GPU method (Schur form,
+    ZTREVC, back-transform, ZGEEV normalisation) for a batch.
+
+    Returns (w (b, n), VL (b, n, n) or None, VR (b, n, n) or None, info (b,)),
+    with V[b, :, i] the eigenvector of w[b, i] (row/column sense)."""
+    A = _colmajor(M)
+    b, n, _ = A.shape
+    w = np.zeros((b, n), np.complex128)
+    VL = np.zeros((b, n, n), np.complex128) if left else None
+    VR = np.zeros((b, n, n), np.complex128) if right else None
+    info = np.zeros(b, np.int32)
+    for j in range(b):
+        wj = np.zeros(n, np.complex128)
+        vl = np.zeros((n, n), np.complex128) if left else None
+        vr = np.zeros((n, n), np.complex128) if right else None
+        info[j] = lib().oracle_zgeev_vectors(n, _p(np.ascontiguousarray(A[j])), _p(wj),
+                                             _p(vl) if left else None, _p(vr) if right else None)
+        w[j] = wj
+        if left:
+            VL[j] = vl.T     # C wrote column-major: vl[i] is column i
+        if right:
+            VR[j] = vr.T
+    return w, VL, VR, info
+
+
+def lapack_zgeev_vectors(M, nthreads: int = 0):
+    """Host LAPACK zgeev('V', 'V'): (w, VL, VR, info), V[b, :, i] pairs w[b, i]."""
+    A = _colmajor(M)
+    b, n, _ = A.shape
+    w = np.zeros((b, n), np.complex128)
+    VL = np.zeros((b, n, n), np.complex128)
+    VR = np.zeros((b, n, n), np.complex128)
+    info = np.zeros(b, np.int32)
+    lib().lapack_zgeev_vectors_batched(n, b, _p(A), _p(w), _p(VL), _p(VR), _p(info), nthreads)
+    return w, VL.transpose(0, 2, 1).copy(), VR.transpose(0, 2, 1).copy(), info
 
 
 def lapack_zgeev(M, nthreads: int = 0):

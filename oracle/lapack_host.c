@@ -11,6 +11,7 @@
  *
  *   lapack_open            dlopen + resolve symbols, force 1 BLAS thread
  *   lapack_zgeev_batched   zgeev (balanced, LAPACK default) per matrix
+ *   lapack_zgeev_vectors_batched  the same with jobvl = jobvr = 'V'
  *   lapack_zhrd_hseqr_batched  zgehrd + zhseqr('E','N') without balancing:
  *                          the exact method of the GPU path, run by LAPACK
  *   lapack_zgeevx_rcond    eigenvalue condition numbers (RCONDE) of the
@@ -67,6 +68,27 @@ int lapack_zgeev_batched(int n, int batch, const zc *A, zc *eig, int *info, int 
         for (int b = 0; b < batch; ++b) {
             memcpy(a, A + (size_t)b * n * n, sizeof(zc) * (size_t)n * n);
             int r = p_zgeev(COLMAJOR, 'N', 'N', n, a, n, eig + (size_t)b * n, NULL, 1, NULL, 1);
+            if (info) info[b] = r;
+        }
+        free(a);
+    }
+    return 0;
+}
+
+/* zgeev with eigenvectors (jobvl = jobvr = 'V', balanced as LAPACK does by
+ * default): w[b*n + i], VL/VR batch x (n x n) column-major. */
+int lapack_zgeev_vectors_batched(int n, int batch, const zc *A, zc *w, zc *VL, zc *VR, int *info, int nthreads)
+{
+    if (!p_zgeev) return -1;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    #pragma omp parallel num_threads(nthreads)
+    {
+        zc *a = (zc *)malloc(sizeof(zc) * (size_t)n * n);
+        #pragma omp for schedule(dynamic, 8)
+        for (int b = 0; b < batch; ++b) {
+            memcpy(a, A + (size_t)b * n * n, sizeof(zc) * (size_t)n * n);
+            size_t off = (size_t)b * n * n;
+            int r = p_zgeev(COLMAJOR, 'V', 'V', n, a, n, w + (size_t)b * n, VL + off, n, VR + off, n);
             if (info) info[b] = r;
         }
         free(a);

@@ -452,3 +452,53 @@ int oracle_zgeev_adjoint(int n, const zc *A, const zc *g, zc *G, zc *w)
     free(T); free(U); free(xr); free(yl); free(X); free(Y);
     return info;
 }
+
+/* ZGEEV's eigenvector normalisation (LAPACK zgeev.f after ZTREVC3 and the
+ * back-transform): unit Euclidean norm (DZNRM2, scaled), then the component
+ * of largest |re|^2 + |im|^2 (first one on ties, IDAMAX) is rotated to be
+ * real and non-negative, its imaginary part set to exactly zero. */
+static void zgeev_normalise(int n, zc *v)
+{
+    double scl = 1.0 / znrm2(n, v);
+    for (int r = 0; r < n; ++r) v[r] *= scl;
+    int k = 0;
+    double best = -1.0;
+    for (int r = 0; r < n; ++r) {
+        double m = creal(v[r]) * creal(v[r]) + cimag(v[r]) * cimag(v[r]);
+        if (m > best) { best = m; k = r; }
+    }
+    zc tmp = conj(v[k]) / sqrt(best);
+    for (int r = 0; r < n; ++r) v[r] *= tmp;
+    v[k] = creal(v[k]);
+}
+
+/* Eigenvalues and eigenvectors by the method of the GPU path: Schur form
+ * (oracle_zschur), eigenvectors of T (oracle_ztrevc), back-transform by U,
+ * ZGEEV normalisation.  VR(:,i) right (A v = w_i v), VL(:,i) left
+ * (u^H A = w_i u^H); either may be NULL.  Column-major n x n.  Returns the
+ * INFO of the Schur step (vectors are not formed when it is non-zero). */
+int oracle_zgeev_vectors(int n, const zc *A, zc *w, zc *VL, zc *VR)
+{
+    size_t nn = (size_t)n * n;
+    zc *T = (zc *)malloc(sizeof(zc) * nn), *U = (zc *)malloc(sizeof(zc) * nn);
+    zc *xr = (zc *)malloc(sizeof(zc) * nn), *yl = (zc *)malloc(sizeof(zc) * nn);
+    int info = oracle_zschur(n, A, T, U, w);
+    if (info == 0) {
+        oracle_ztrevc(n, T, xr, yl);
+        for (int side = 0; side < 2; ++side) {
+            zc *V = side == 0 ? VR : VL;
+            const zc *E = side == 0 ? xr : yl;
+            if (!V) continue;
+            for (int i = 0; i < n; ++i) {
+                for (int r = 0; r < n; ++r) {
+                    zc s = 0;
+                    for (int c = 0; c < n; ++c) s += U[IDX(r, c, n)] * E[IDX(c, i, n)];
+                    V[IDX(r, i, n)] = s;
+                }
+                zgeev_normalise(n, V + (size_t)i * n);
+            }
+        }
+    }
+    free(T); free(U); free(xr); free(yl);
+    return info;
+}

@@ -102,3 +102,21 @@ def test_python_layer_refuses_host_tensors():
     torch = pytest.importorskip("torch")
     with pytest.raises(tk.TokuraError):
         tk._dev_check(torch.zeros(2, 2, dtype=torch.complex128), "M", torch.complex128)
+
+
+def test_vectors_entry_validation():
+    L = tk.lib()
+    h = make_handle()
+    P = ctypes.c_void_p
+    a, e, vl, vr, f = P(0x1000), P(0x2000), P(0x3000), P(0x4000), P(0x5000)
+    s = P(0)
+    g = L.tokura_zgeev_batched_vectors_gpu
+    assert g(None, 4, 1, a, e, vl, vr, f, s) == -1
+    assert g(h, 0, 1, a, e, vl, vr, f, s) == -2
+    assert g(h, 65, 1, a, e, vl, vr, f, s) == -2
+    assert g(h, 4, 1, None, e, vl, vr, f, s) == -3
+    assert g(h, 4, 1, a, None, vl, vr, f, s) == -4
+    assert g(h, 4, 1, a, e, None, None, f, s) == -5     # neither left nor right vectors asked for
+    assert g(h, 4, 0, a, e, vl, vr, f, s) == -6
+    assert g(h, 4, 1, a, e, vl, vr, None, s) == -7
+    assert L.tokuraDestroy(h) == 0

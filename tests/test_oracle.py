@@ -239,3 +239,67 @@ def test_hard_spectra_generator_and_condition_scaled_metric():
     i = int(np.argmin(np.abs(bad[0] - w[0][j])))
     bad[0, i] += 1e4 * np.finfo(float).eps * np.linalg.norm(M) / rc[0][j]
     assert oracle.condscaled_err(M, bad).max() > 1e3
+
+
+def _check_normalised(V):
+    # ZGEEV: unit 2-norm, the component of largest modulus real, >= 0 and
+    # with an imaginary part of exactly zero
+    for v in V.transpose(1, 0):
+        assert abs(np.linalg.norm(v) - 1.0) < 1e-14
+        k = int(np.argmax(np.abs(v) ** 2))
+        assert v[k].imag == 0.0 and v[k].real > 0.0
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 16, 33])
+def test_vectors_vs_lapack_zgeev(n):
+    rng = np.random.default_rng(20 + n)
+    M = rand(rng, 6, n)
+    w, VL, VR, info = oracle.method_vectors(M)
+    wl, VLl, VRl, il = oracle.lapack_zgeev_vectors(M)
+    assert not info.any() and not il.any()
+    for b in range(M.shape[0]):
+        perm = oracle.match(wl[b], w[b])
+        assert np.abs(w[b][perm] - wl[b]).max() < 1e-12 * np.abs(wl[b]).max()
+        # same normalisation => the same vectors up to the eigenvector condition
+        assert np.abs(VR[b][:, perm] - VRl[b]).max() < 1e-10
+        assert np.abs(VL[b][:, perm] - VLl[b]).max() < 1e-10
+        nA = np.linalg.norm(M[b])
+        assert np.abs(M[b] @ VR[b] - VR[b] * w[b]).max() < 1e-13 * n * nA
+        assert np.abs(VL[b].conj().T @ M[b] - w[b][:, None] * VL[b].conj().T).max() < 1e-13 * n * nA
+        _check_normalised(VR[b])
+        _check_normalised(VL[b])
+
+
+def test_vectors_closed_forms():
+    # diagonal: unit vectors in diagonal order
+    d = np.array([3 + 1j, -2.0, 0.5j, 7 - 2j])
+    w, VL, VR, info = oracle.method_vectors(np.diag(d))
+    np.testing.assert_array_equal(w[0], d)
+    np.testing.assert_array_equal(VR[0], np.eye(4))
+    np.testing.assert_array_equal(VL[0], np.eye(4))
+    # 2x2 upper triangular [[a, b], [0, d]]: right vectors e1 and (b, d - a),
+    # left vectors (d - a, -b)^* ... written out: u1 = (1, conj(b)/conj(a - d)) scaled, u2 = e2
+    a, b, dd = 1 + 2j, 3 - 1j, -2 + 0.5j
+    T = np.array([[a, b], [0, dd]])
+    w, VL, VR, info = oracle.method_vectors(T)
+    assert np.allclose(w[0], [a, dd], rtol=0, atol=1e-15)
+
+    def zgeev_norm(v):
+        v = v / np.linalg.norm(v)
+        k = int(np.argmax(np.abs(v) ** 2))
+        return v * (np.conj(v[k]) / abs(v[k]))
+
+    np.testing.assert_allclose(VR[0][:, 0], [1, 0], atol=1e-15)
+    np.testing.assert_allclose(VR[0][:, 1], zgeev_norm(np.array([b, dd - a])), atol=1e-15)
+    np.testing.assert_allclose(VL[0][:, 1], [0, 1], atol=1e-15)
+    np.testing.assert_allclose(VL[0][:, 0], zgeev_norm(np.array([1, np.conj(b) / np.conj(a - dd)])), atol=1e-15)
+
+
+def test_vectors_hermitian_left_equals_right():
+    rng = np.random.default_rng(23)
+    X = rand(rng, 3, 12)
+    Hm = X + np.conj(X.transpose(0, 2, 1))
+    w, VL, VR, info = oracle.method_vectors(Hm)
+    for b in range(3):
+        np.testing.assert_allclose(VL[b], VR[b], atol=1e-12)
+        np.testing.assert_allclose(VR[b].conj().T @ VR[b], np.eye(12), atol=1e-12)

@@ -8,6 +8,8 @@ passes torch device pointers and the current HIP stream through ctypes:
 * ``zgeev_batched``    same-size batch, the reference call
                        (tokura_zgeev_batched_library/src/tokura_zgeev_batched.cu:212-285)
 * ``eigvals``          convenience: (B, n, n) tensor -> (B, n) eigenvalues + flags
+* ``zgeev_vectors`` / ``eig``  eigenvalues with right and/or left eigenvectors
+                       (ZGEEV with JOBVL/JOBVR = 'V', no balancing)
 * ``RaggedBatch``      mixed sizes through one persistent kernel (+ hipGraph)
 * ``eigvals_autograd`` differentiable eigenvalues (``torch.autograd.Function``
                        whose backward is the library's adjoint kernel)
@@ -25,7 +27,7 @@ import os
 
 import torch
 
-__all__ = ["lib", "Handle", "zgeev_batched", "eigvals", "workspace_size", "MAX_N",
+__all__ = ["lib", "Handle", "zgeev_batched", "eigvals", "workspace_size", "MAX_N", "zgeev_vectors", "eig",
            "RaggedBatch", "eigvals_autograd", "schur_batched", "backward_batched"]
 
 MAX_N = 64
@@ -59,6 +61,7 @@ _SIGNATURES = {
     "tokura_get_zgeev_schur_worksize": ([_I, _I], _SZ),
     "tokura_zgeev_batched_schur_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P], _I),
     "tokura_zgeev_batched_backward_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P, _P], _I),
+    "tokura_zgeev_batched_vectors_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P, _P], _I),
 }
 
 
@@ -206,6 +209,43 @@ def eigvals(M: torch.Tensor, handle: Handle | None = None, colmajor: bool = Fals
     work = torch.empty((workspace_size(n, B),), dtype=torch.uint8, device=M.device)
     _check(zgeev_batched(A, n, B, eig, work, flag, handle, stream), "tokura_zgeev_batched_gpu")
     return eig, flag
+
+
+def zgeev_vectors(A_colmajor: torch.Tensor, n: int, batch: int, eig: torch.Tensor, VL: torch.Tensor | None,
+                  VR: torch.Tensor | None, flag: torch.Tensor, handle: Handle | None = None, stream=None) -> int:
+    """Raw tokura_zgeev_batched_vectors_gpu: A column-major per matrix, eig
+    batch*n, VL/VR batch*n*n column-major (either may be None), flag batch
+    int8.  Returns the C code."""
+    h = handle or default_handle()
+    return int(lib().tokura_zgeev_batched_vectors_gpu(
+        h.ptr, n, batch, _P(A_colmajor.data_ptr()), _P(eig.data_ptr()),
+        _P(VL.data_ptr()) if VL is not None else None, _P(VR.data_ptr()) if VR is not None else None,
+        _P(flag.data_ptr()), _stream_ptr(stream)))
+
+
+def eig(M: torch.Tensor, left: bool = False, right: bool = True, handle: Handle | None = None, stream=None):
+    """Eigenvalues and eigenvectors of a (B, n, n) complex128 device tensor.
+
+    Returns (w (B, n), VL (B, n, n) or None, VR (B, n, n) or None, flag (B,)),
+    with VR[b, :, i] the right eigenvector of w[b, i] (M[b] v = w v) and
+    VL[b, :, i] the left one (u^H M[b] = w u^H), normalised as LAPACK zgeev
+    (unit 2-norm, largest component real).  At least one of left/right."""
+    _dev_check(M, "M", torch.complex128)
+    if M.dim() == 2:
+        M = M.unsqueeze(0)
+    B, n, n2 = M.shape
+    if n != n2 or not (1 <= n <= MAX_N):
+        raise TokuraError(f"matrices must be n x n with 1 <= n <= {MAX_N}, got {tuple(M.shape)}")
+    if not (left or right):
+        raise TokuraError("eig: ask for left and/or right eigenvectors (eigvals() gives eigenvalues only)")
+    A = M.transpose(-1, -2).contiguous()
+    w = torch.empty((B, n), dtype=torch.complex128, device=M.device)
+    flag = torch.empty((B,), dtype=torch.int8, device=M.device)
+    VL = torch.empty((B, n, n), dtype=torch.complex128, device=M.device) if left else None
+    VR = torch.empty((B, n, n), dtype=torch.complex128, device=M.device) if right else None
+    _check(zgeev_vectors(A, n, B, w, VL, VR, flag, handle, stream), "tokura_zgeev_batched_vectors_gpu")
+    # the library writes column-major: stored[b, i, r] = V(r, i)
+    return (w, VL.transpose(-1, -2) if left else None, VR.transpose(-1, -2) if right else None, flag)
 
 
 from .ragged import RaggedBatch  # noqa: E402
