@@ -86,7 +86,7 @@ def test_vectors_closed_forms(gpu):
     np.testing.assert_array_equal(w.cpu().numpy()[0], d)
     np.testing.assert_array_equal(VR.cpu().numpy()[0], np.eye(5))
     np.testing.assert_array_equal(VL.cpu().numpy()[0], np.eye(5))
-    w, VL, VR, flag = tk.eig(torch.tensor([[[2.0 - 1j]]], device=gpu), left=True)
+    w, VL, VR, flag = tk.eig(torch.tensor([[[2.0 - 1j]]], dtype=torch.complex128, device=gpu), left=True)
     torch.cuda.synchronize()
     assert int(flag[0]) == 0 and complex(w[0, 0]) == 2 - 1j
     assert complex(VR[0, 0, 0]) == 1 and complex(VL[0, 0, 0]) == 1
