@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <float.h>
+#include <type_traits>
 
 namespace tkz {
 
@@ -490,6 +491,41 @@ __device__ __forceinline__ int group_last_true(int t, int lo, int hi, int none, 
     return found;
 }
 
+// ZLAHQR's shift for the active block [l, i]: exceptional shifts on every
+// 10th (from the top) and 20th (from the bottom) sweep without a deflation,
+// else the Wilkinson shift (the eigenvalue of the trailing 2x2 closer to
+// H(i,i)).
+template <class Lay>
+__device__ __forceinline__ zd zlahqr_shift(const zd *H, const Lay &lay, int l, int i, int kdefl)
+{
+    zd sh;
+    if (kdefl % 20 == 0) {
+        double s = 0.75 * fabs(H[lay(i, i - 1)].re);
+        sh = zadd(H[lay(i, i)], mk(s, 0.0));
+    } else if (kdefl % 10 == 0) {
+        double s = 0.75 * fabs(H[lay(l + 1, l)].re);
+        sh = zadd(H[lay(l, l)], mk(s, 0.0));
+    } else {
+        sh = H[lay(i, i)];
+        zd u = zmul(zsqrt(H[lay(i - 1, i)]), zsqrt(H[lay(i, i - 1)]));
+        double s = cabs1(u);
+        if (s != 0.0) {
+            zd x = zscale(0.5, zsub(H[lay(i - 1, i - 1)], sh));
+            double sx = cabs1(x);
+            s = fmax(s, sx);
+            double is = 1.0 / s;
+            zd xs = zscale(is, x), us = zscale(is, u);
+            zd y = zscale(s, zsqrt(zadd(zmul(xs, xs), zmul(us, us))));
+            if (sx > 0.0) {
+                zd xn = zscale(1.0 / sx, x);
+                if (xn.re * y.re + xn.im * y.im < 0.0) y = mk(-y.re, -y.im);
+            }
+            sh = zsub(sh, zmul(u, zdiv(u, zadd(x, y))));
+        }
+    }
+    return sh;
+}
+
 // ---- ZLAHQR with ilo = 0, ihi = n-1 ----
 // WANTT = false: eigenvalues only (transformations restricted to the active
 // block [l, i], as ZLAHQR with WANTT = WANTZ = false).  WANTT = true: the full
@@ -536,31 +572,7 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
             ++kdefl;
             const int I1 = WANTT ? 0 : l, I2 = WANTT ? n - 1 : i;
             // ---- shift
-            zd sh;
-            if (kdefl % 20 == 0) {
-                double s = 0.75 * fabs(H[lay(i, i - 1)].re);
-                sh = zadd(H[lay(i, i)], mk(s, 0.0));
-            } else if (kdefl % 10 == 0) {
-                double s = 0.75 * fabs(H[lay(l + 1, l)].re);
-                sh = zadd(H[lay(l, l)], mk(s, 0.0));
-            } else {
-                sh = H[lay(i, i)];
-                zd u = zmul(zsqrt(H[lay(i - 1, i)]), zsqrt(H[lay(i, i - 1)]));
-                double s = cabs1(u);
-                if (s != 0.0) {
-                    zd x = zscale(0.5, zsub(H[lay(i - 1, i - 1)], sh));
-                    double sx = cabs1(x);
-                    s = fmax(s, sx);
-                    double is = 1.0 / s;
-                    zd xs = zscale(is, x), us = zscale(is, u);
-                    zd y = zscale(s, zsqrt(zadd(zmul(xs, xs), zmul(us, us))));
-                    if (sx > 0.0) {
-                        zd xn = zscale(1.0 / sx, x);
-                        if (xn.re * y.re + xn.im * y.im < 0.0) y = mk(-y.re, -y.im);
-                    }
-                    sh = zsub(sh, zmul(u, zdiv(u, zadd(x, y))));
-                }
-            }
+            const zd sh = zlahqr_shift(H, lay, l, i, kdefl);
             // ---- look for two consecutive small sub-diagonal elements
             const int m = group_last_true<LPM, S>(t, l + 1, i - 1, l, [&](int p) {
                 zd h11 = H[lay(p, p)], h22 = H[lay(p + 1, p + 1)];
@@ -817,6 +829,284 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                 H[lay(i, i)] = d;
             }
             lds_order();
+        }
+        if (!conv) return i + 1;
+        kdefl = 0;
+        i = l - 1;
+    }
+    return 0;
+}
+
+// View of the trailing part of a layout from (o, o): element (r, c) of the
+// view is element (o + r, o + c) of the base.
+template <class Lay>
+struct SubLayout {
+    Lay base;
+    int o;
+    __device__ __forceinline__ int cb(int c) const { return base.cb(c + o) + o; }
+    __device__ __forceinline__ int operator()(int r, int c) const { return cb(c) + r; }
+};
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    union { double d; int w[2]; } u;
+    u.d = v;
+    u.w[0] = __builtin_amdgcn_readlane(u.w[0], lane);
+    u.w[1] = __builtin_amdgcn_readlane(u.w[1], lane);
+    return u.d;
+}
+
+// Broadcast of the two half-waves: lo receives lanes 0..31's value, hi lanes
+// 32..63's, in every lane (v_permlane32_swap with both operands = v; the
+// halves are uniform where this is used).
+__device__ __forceinline__ void bcast_halves(double v, double &lo, double &hi)
+{
+    union { double d; unsigned w[2]; } u, a, b;
+    u.d = v;
+    const auto r0 = __builtin_amdgcn_permlane32_swap(u.w[0], u.w[0], false, false);
+    const auto r1 = __builtin_amdgcn_permlane32_swap(u.w[1], u.w[1], false, false);
+    a.w[0] = r0[0];
+    a.w[1] = r1[0];
+    b.w[0] = r0[1];
+    b.w[1] = r1[1];
+    lo = a.d;
+    hi = b.d;
+}
+
+// x with its sign bit xor-ed with m (0 or 0x80000000).
+__device__ __forceinline__ double flip_sign(double x, unsigned m)
+{
+    union { double d; unsigned w[2]; } u;
+    u.d = x;
+    u.w[1] ^= m;
+    return u.d;
+}
+
+
+// ---- eigenvalues-only QR with two bulges per sweep (one matrix per wave) ----
+// A "dual sweep" over an active block [l, i] is two consecutive single-shift
+// ZLAHQR sweeps with the shifts s1 (the Wilkinson shift) and s2 (the other
+// eigenvalue of the trailing 2x2), pipelined: bulge A (s1) starts at step 0,
+// bulge B (s2) three positions behind it.  Lanes 0..31 compute A's reflector
+// chain and window, lanes 32..63 B's, in the same instructions; every lane p
+// then applies both reflectors at its position (B's first, then A's).
+//  * The two bulges act on disjoint rows/columns except the 2x2 block rows
+//    {kB, kB+1} x columns {kA, kA+1}, which B changes from the left and A
+//    from the right; those products commute, and the phases run in order
+//    with every bulk update reading and writing LDS, so the result is that of
+//    the two sweeps run one after the other (implicit-Q argument), up to
+//    rounding.
+//  * Windows: A's window (rows/cols kA..kA+2) is untouched by B, B's by A;
+//    B reads H(kA, kA-1) as its next window element after A wrote it.
+//  * Each bulge ends with ZLAHQR's "make H(i,i-1) real" scaling of column i.
+// Blocks smaller than DUAL_MIN and the last rows go to the single-shift code
+// (zlahqr_core) on the sub-block, which is independent once H(l, l-1) = 0.
+// Returns 0 or 1 + the 0-based index of the first unconverged eigenvalue.
+//  * Packed layout only: column c starts c + 2 elements after column c - 1,
+//    so every address of a step is an increment of the previous step's.
+__device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n, int t, int itmax_override)
+{
+    constexpr int DUAL_MIN = 10;
+    const double ulp = DBL_EPSILON, safmin = DBL_MIN;
+    const double smlnum = safmin * ((double)n / ulp);
+    const int itmax = itmax_override >= 0 ? itmax_override : 30 * (n > 10 ? n : 10);
+    const int half = t >> 5;           // 0: bulge A's chain, 1: bulge B's
+    const bool leader = (t & 31) == 0; // lanes 0 and 32 store their half's window
+    const int p = t;                   // bulk position of this lane
+    const int cbp = lay.cb(p < n ? p : n - 1);
+    int kdefl = 0;
+    int i = n - 1;
+    while (i >= 1) {
+        int l = 0;
+        bool conv = false;
+        for (int its = 0; its <= itmax; ++its) {
+            const int kf = group_last_true<64, 1>(t, l + 1, i, l, [&](int q) {
+                return subdiag_negligible(H, lay, q, n, smlnum, ulp);
+            });
+            l = kf;
+            if (l > 0 && t == 0) H[lay(l, l - 1)] = mk(0.0, 0.0);
+            lds_order();
+            if (l >= i) { conv = true; break; }
+            if (i - l + 1 < DUAL_MIN) {
+                // the whole block [l, i] by single-shift sweeps
+                const int info = zlahqr_core<64, 1, false>(H, SubLayout<PackedLayout>{lay, l}, i - l + 1, t,
+                                                           itmax_override);
+                if (info != 0) return l + info;
+                conv = true;
+                break;
+            }
+            ++kdefl;
+            // ---- shifts: s1 as ZLAHQR (incl. exceptional), s2 the other
+            // eigenvalue of the trailing 2x2 (trace - Wilkinson shift), or the
+            // Wilkinson shift when s1 is exceptional
+            const zd s1 = zlahqr_shift(H, lay, l, i, kdefl);
+            const zd s2 = (kdefl % 10 == 0) ? zlahqr_shift(H, lay, l, i, 1)
+                                             : zsub(zadd(H[lay(i - 1, i - 1)], H[lay(i, i)]), s1);
+            const zd sig = half ? s2 : s1;
+            const int nsteps = i - l;          // steps of one bulge: k = l .. i-1 (>= DUAL_MIN - 1)
+            const bool pin = p >= l && p <= i;
+            // per-half chain state: window, reflector input, and the bulge's
+            // position kk with cbk = cb(kk).  A half without a bulge keeps
+            // x1 = 1 (inside the fast reflector's range); its results are
+            // never stored.
+            zd x1 = mk(1.0, 0.0), a = mk(0.0, 0.0), b = a, c = a, d = a, e = a;
+            double x2 = 0.0;
+            int kk = l, cbk = lay.cb(l);
+            auto start = [&]() {
+                kk = l;
+                cbk = lay.cb(l);
+                const int c1 = cbk + l + 2;   // cb(l + 1)
+                a = H[cbk + l];
+                c = H[cbk + l + 1];
+                b = H[c1 + l];
+                d = H[c1 + l + 1];
+                e = H[c1 + l + 2];            // l + 2 <= i: the block has >= DUAL_MIN rows
+                const zd h11s = zsub(a, sig);
+                const double h21 = c.re;
+                const double sc = cabs1(h11s) + fabs(h21);
+                x1 = mk(h11s.re / sc, h11s.im / sc);
+                x2 = h21 / sc;
+            };
+            // bulk operands of the bulge at k (cbks = cb(k)): from the left on
+            // rows k, k+1 of columns p >= k+2, from the right on columns k, k+1
+            // of rows l <= p <= k-1
+            auto phase_addr = [&](const int k, const int cbks, bool &row, bool &col, int &ax, int &ay) {
+                row = pin && p >= k + 2;
+                col = pin && p <= k - 1;
+                ax = row ? cbp + k : cbks + p;
+                ay = ax + (row ? 1 : k + 2);
+            };
+            auto phase_apply = [&](const bool row, const bool col, const int ax, const int ay, zd x, zd y,
+                                   const zd tr, const zd vr, const double t2r) {
+                if (row || col) {
+                    const unsigned m = row ? 0x80000000u : 0u;
+                    const zd c1 = mk(tr.re, flip_sign(tr.im, m));
+                    const zd vv = mk(vr.re, flip_sign(vr.im, m ^ 0x80000000u));
+                    pair_op(x, y, c1, t2r, vv);
+                    H[ax] = x;
+                    H[ay] = y;
+                }
+            };
+            // one time step: bulge A at kA = l + tau, bulge B at kA - 3.
+            // HA/HB: bulge A/B active; ADV bit 0: A advances, bit 1: B
+            // advances (compile-time: the steady state has no per-half branch)
+            auto dstep = [&](auto HA, auto HB, auto ADV, const int tau) {
+                constexpr bool hasA = decltype(HA)::value, hasB = decltype(HB)::value;
+                constexpr int adv = decltype(ADV)::value;
+                const int kA = l + tau, kB = kA - 3;
+                // phase-1 operands (bulge B): independent of this step's chain
+                bool rowB = false, colB = false;
+                int axB = 0, ayB = 0;
+                zd xB, yB;                    // read and used only where rowB || colB
+                if constexpr (hasB) {
+                    phase_addr(kB, lay.cb(kB), rowB, colB, axB, ayB);
+                    if (rowB || colB) {
+                        xB = H[axB];
+                        yB = H[ayB];
+                    }
+                }
+                // chain: reflector and window of this half's bulge; after a
+                // bulge's last step e = 0 and the row kk+2 update is a no-op
+                zd t1, v2;
+                double t2, beta;
+                bool ident;
+                zlarfg2(x1, x2, t1, v2, t2, beta, ident);
+                const zd tc = zconj(t1), vc = zconj(v2);
+                const zd xsub = ident ? x1 : mk(beta, 0.0);
+                pair_op(a, c, tc, t2, v2);    // left: column kk
+                pair_op(b, d, tc, t2, v2);    // left: column kk+1
+                pair_op(a, b, t1, t2, vc);    // right: row kk
+                pair_op(c, d, t1, t2, vc);    // right: row kk+1
+                const double sr = t2 * e.re;  // right: row kk+2 (H(kk+2,kk) = 0, e real)
+                const double f = -sr;
+                e.re = fma(-sr, vc.re, e.re);
+                e.im = -sr * vc.im;
+                // window stores: H(kk,kk-1) at cbk-1, H(kk,kk) at cbk+kk,
+                // H(kk,kk+1) at cbk+2kk+2
+                if (leader && (hasA && hasB ? true : (hasA ? half == 0 : half == 1))) {
+                    if (kk > l) H[cbk - 1] = xsub;
+                    H[cbk + kk] = a;
+                    H[cbk + 2 * kk + 2] = b;
+                }
+                // both reflectors to every lane as wave-uniform values (A from
+                // lane 0, B from lane 32)
+                const zd tA = mk(readlane_d(t1.re, 0), readlane_d(t1.im, 0));
+                const zd vA = mk(readlane_d(v2.re, 0), readlane_d(v2.im, 0));
+                const double t2A = readlane_d(t2, 0);
+                const zd tB = mk(readlane_d(t1.re, 32), readlane_d(t1.im, 32));
+                const zd vB = mk(readlane_d(v2.re, 32), readlane_d(v2.im, 32));
+                const double t2B = readlane_d(t2, 32);
+                if constexpr (hasB) phase_apply(rowB, colB, axB, ayB, xB, yB, tB, vB, t2B);
+                lds_order();
+                if constexpr (hasA) {         // phase 2 reads after phase 1's writes
+                    bool rowA, colA;
+                    int axA, ayA;
+                    phase_addr(kA, lay.cb(kA), rowA, colA, axA, ayA);
+                    zd xA, yA;                // read and used only where rowA || colA
+                    if (rowA || colA) {
+                        xA = H[axA];
+                        yA = H[ayA];
+                    }
+                    phase_apply(rowA, colA, axA, ayA, xA, yA, tA, vA, t2A);
+                }
+                lds_order();
+                // advance to kk+1: b, d, e are rows kk+1 .. kk+3 of column kk+2,
+                // which starts at cb(kk+2) = cbk + 2kk + 5
+                if constexpr (adv != 0) {
+                    if (adv == 3 || (adv == 1 ? half == 0 : half == 1)) {
+                        const int base = cbk + 3 * kk + 6;
+                        x1 = c;
+                        x2 = f;
+                        a = d;
+                        c = e;
+                        b = H[base];
+                        d = H[base + 1];
+                        e = (kk + 3 <= i) ? H[base + 2] : mk(0.0, 0.0);
+                        cbk += kk + 2;
+                        kk += 1;
+                    }
+                }
+            };
+            // end of a bulge's sweep (ZLAHQR): H(i,i-1) made real, column i
+            // rows l..i-1 scaled; c, d of half `hb` hold H(i,i-1), H(i,i)
+            auto finish = [&](const int hb) {
+                double cre, cim, dre, dim, o;
+                if (hb == 0) {
+                    bcast_halves(c.re, cre, o);
+                    bcast_halves(c.im, cim, o);
+                    bcast_halves(d.re, dre, o);
+                    bcast_halves(d.im, dim, o);
+                } else {
+                    bcast_halves(c.re, o, cre);
+                    bcast_halves(c.im, o, cim);
+                    bcast_halves(d.re, o, dre);
+                    bcast_halves(d.im, o, dim);
+                }
+                const zd ce = mk(cre, cim), de = mk(dre, dim);
+                if (ce.im != 0.0) {
+                    const double r = zabs(ce);
+                    const zd temp = zscale(1.0 / r, ce);
+                    if (p >= l && p <= i - 1) H[lay(p, i)] = zmul(H[lay(p, i)], temp);
+                    if (t == 0) H[lay(i, i - 1)] = mk(r, 0.0);
+                } else if (t == 0) {
+                    H[lay(i, i - 1)] = ce;
+                }
+                if (t == 0) H[lay(i, i)] = de;
+                lds_order();
+            };
+            using T_ = std::true_type;
+            using F_ = std::false_type;
+            // bulge A alone for three steps, then B starts three positions behind
+            if (half == 0) start();
+            for (int tau = 0; tau < 3; ++tau) dstep(T_(), F_(), std::integral_constant<int, 1>(), tau);
+            if (half == 1) start();
+            for (int tau = 3; tau < nsteps - 1; ++tau) dstep(T_(), T_(), std::integral_constant<int, 3>(), tau);
+            dstep(T_(), T_(), std::integral_constant<int, 2>(), nsteps - 1);   // A's last step
+            finish(0);
+            dstep(F_(), T_(), std::integral_constant<int, 2>(), nsteps);
+            dstep(F_(), T_(), std::integral_constant<int, 2>(), nsteps + 1);
+            dstep(F_(), T_(), std::integral_constant<int, 0>(), nsteps + 2);   // B's last step
+            finish(1);
         }
         if (!conv) return i + 1;
         kdefl = 0;

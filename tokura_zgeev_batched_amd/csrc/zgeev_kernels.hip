@@ -109,8 +109,9 @@ __global__ __launch_bounds__(NT) void k_hessenberg_packed_wg(int n, int batch, c
     if (tid == 0) flag[mat] = ok ? 0 : 2;
 }
 
-// Stage 2 of the split path: QR on the packed Hessenberg matrix.
-template <int NB, int LPM>
+// Stage 2 of the split path: QR on the packed Hessenberg matrix (DUAL: two
+// bulges per sweep, requires LPM == 64).
+template <int NB, int LPM, bool DUAL = false>
 __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__restrict__ work,
                                                   zd *__restrict__ eig, int8_t *__restrict__ flag,
                                                   int itmax)
@@ -133,7 +134,12 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
     lds_order();
     const PackedLayout pk;
     int info = 0;
-    if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
+    if constexpr (DUAL) {
+        static_assert(LPM == 64, "two-bulge QR: one matrix per wavefront");
+        if (n > 1) info = zlahqr_dual(H, pk, n, t, itmax);
+    } else {
+        if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
+    }
     for (int r = t; r < n; r += LPM) out[r] = H[pk(r, r)];
     if (t == 0) flag[mat] = info != 0 ? 1 : 0;
 }
@@ -160,7 +166,9 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
     X(15, SPLIT, 48, 32)                \
     X(16, SPLIT, 48, 64)                \
     X(17, SPLIT, 64, 32)                \
-    X(18, SPLIT, 64, 64)
+    X(18, SPLIT, 64, 64)                \
+    X(19, DUAL, 48, 64)                 \
+    X(20, DUAL, 64, 64)
 
 static const VariantInfo g_variants[] = {
 #define X(id, kind, nb, lpm) {id, VK_##kind, nb, lpm, 64 / lpm},
@@ -180,7 +188,7 @@ static hipError_t launch_fused(int n, int batch, const zd *A, zd *eig, int8_t *f
     return hipGetLastError();
 }
 
-template <int NB, int LPM>
+template <int NB, int LPM, bool DUAL = false>
 static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *flag, zd *work, int itmax,
                                hipStream_t s, int *stage, hipEvent_t mid)
 {
@@ -195,7 +203,8 @@ static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *f
     if (e != hipSuccess) return e;
     if (mid) hipEventRecord(mid, s);
     *stage = 2;
-    hipLaunchKernelGGL((k_qr_packed<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, work, eig, flag, itmax);
+    hipLaunchKernelGGL((k_qr_packed<NB, LPM, DUAL>), dim3(blocks), dim3(64), 0, s, n, batch, work, eig, flag,
+                       itmax);
     return hipGetLastError();
 }
 
@@ -211,7 +220,7 @@ hipError_t launch_plain(int variant, int n, int batch, const void *A, void *eig,
 #define X(id, kind, nb, lpm)                                                              \
     case id:                                                                              \
         if constexpr (VK_##kind == VK_FUSED) return launch_fused<nb, lpm>(n, batch, a, w, f, itmax, s); \
-        else return launch_split<nb, lpm>(n, batch, a, w, f, wk, itmax, s, stage, mid);
+        else return launch_split<nb, lpm, VK_##kind == VK_DUAL>(n, batch, a, w, f, wk, itmax, s, stage, mid);
         TKZ_VARIANTS(X)
 #undef X
     default:

@@ -4,7 +4,8 @@
 
 namespace tkz {
 
-enum VariantKind { VK_FUSED = 0, VK_SPLIT = 1 };
+// VK_DUAL: split path whose QR runs two bulges per sweep (one matrix per wave)
+enum VariantKind { VK_FUSED = 0, VK_SPLIT = 1, VK_DUAL = 2 };
 
 // One compiled launch configuration of the same-size path.
 //   nb   largest n the LDS allocation holds
