@@ -15,7 +15,9 @@
 // A1 (n 49..64), A2 (33..48), B (17..32) or C (1..16); the lanes-per-matrix of
 // each class form a "class configuration" that the autotuner picks.  Smaller
 // classes put more matrices on a wavefront, so the serial reflector chain of
-// every chase step serves several matrices at once.
+// every chase step serves several matrices at once; a class with one matrix
+// per wavefront runs the two-bulge QR (two shifts per sweep, one chain per
+// half-wave).
 //
 // `work` layout (built on the host by tokura_zgeev_ragged_prepare):
 //   RaggedHeader | tiles int4[ntiles] | order int[B] | n int[B] |
@@ -119,7 +121,11 @@ __device__ __forceinline__ void qr_tile(zd *lds, const RaggedView &V, int4 td, z
     lds_order();
     const PackedLayout pk;
     int info = 0;
-    if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
+    if constexpr (LPM == 64) {   // one matrix per wavefront: two bulges per sweep
+        if (n > 1) info = zlahqr_dual(H, pk, n, t, itmax);
+    } else {
+        if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
+    }
     for (int r = t; r < n; r += LPM) out[r] = H[pk(r, r)];
     if (t == 0) flag[b] = info != 0 ? 1 : 0;
     lds_order();

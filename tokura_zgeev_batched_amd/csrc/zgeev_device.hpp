@@ -906,7 +906,11 @@ __device__ __forceinline__ double flip_sign(double x, unsigned m)
 //    so every address of a step is an increment of the previous step's.
 __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n, int t, int itmax_override)
 {
-    constexpr int DUAL_MIN = 10;
+#ifndef TKZ_DUAL_MIN
+#define TKZ_DUAL_MIN 10
+#endif
+    constexpr int DUAL_MIN = TKZ_DUAL_MIN;
+    static_assert(DUAL_MIN >= 8, "the pipelined sweep needs at least 8 rows");
     const double ulp = DBL_EPSILON, safmin = DBL_MIN;
     const double smlnum = safmin * ((double)n / ulp);
     const int itmax = itmax_override >= 0 ? itmax_override : 30 * (n > 10 ? n : 10);
