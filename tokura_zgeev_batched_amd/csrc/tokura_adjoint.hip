@@ -55,7 +55,7 @@ __device__ __forceinline__ void fill_nan(zd *__restrict__ dst, int count, int t)
 // Schur form of the matrix at src: T into H (upper triangular), U into Z.
 // Returns -1 for a non-finite input, else the QR INFO.
 template <int NB, int LPM, bool ZH = false>
-__device__ int schur_in_lds(zd *H, zd *Z, zd *taus, const FullLayout lay, const zd *__restrict__ src, int n, int t,
+__device__ __forceinline__ int schur_in_lds(zd *H, zd *Z, zd *taus, const FullLayout lay, const zd *__restrict__ src, int n, int t,
                             int itmax)
 {
     constexpr int S = (NB + LPM - 1) / LPM;

@@ -35,8 +35,8 @@ int default_variant(int n)
     if (n <= 8) return 1;    // fused 8 x 4
     if (n <= 16) return 4;   // fused 16 x 8
     if (n <= 32) return 8;   // fused 32 x 32
-    if (n <= 48) return 16;  // split 48 x 64
-    return 18;               // split 64 x 64
+    if (n <= 48) return 19;  // split 48 x 64, two bulges per sweep
+    return 20;               // split 64 x 64, two bulges per sweep
 }
 
 bool variant_fits(int v, int n)
