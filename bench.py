@@ -36,10 +36,10 @@ import tokura_zgeev_batched_amd as tk  # noqa: E402
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
 # HBM bytes per matrix moved by the QR kernel k_qr_packed<64,64>, from the
-# rocprofv3 FETCH_SIZE / WRITE_SIZE passes recorded in
-# profiles/qr_packed_64x64_n64_b20k_traffic.md (each counter scaled by its
-# reading for a 1 GiB device copy: FETCH x2.0, WRITE x1.0).
-QR64_TRAFFIC_BYTES_PER_MATRIX = 35_440.0
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes over one n = 64 call of the
+# two-bulge QR kernel (each counter scaled by its reading for a 1 GiB device
+# copy: FETCH x2.0, WRITE x1.0).
+QR64_TRAFFIC_BYTES_PER_MATRIX = 35_480.0
 
 
 def flops_per_solve(n: int) -> float:
