@@ -158,6 +158,17 @@ int tokuraAutotuneRagged(tokurablas_t *handle, int batch_count, const int64_t *o
                          const hipDoubleComplex *A, hipDoubleComplex *eigenvalues, void *work,
                          size_t work_bytes, char *flag, hipStream_t stream);
 
+/* ---- balancing ------------------------------------------------------------
+ * No counterpart in the reference, which never balances (README.md:79).
+ * Sets the LAPACK ZGEBAL job applied before the Hessenberg reduction by the
+ * eigenvalue paths of this handle (same-size and ragged): 'N' none (the
+ * default, as the reference), 'P' permute to isolate eigenvalues, 'S' scale
+ * rows/columns by powers of two, 'B' both (LAPACK zgeev's choice).  The
+ * eigenvalues are those of the balanced matrix, a similarity transform of A;
+ * the Schur-form, backward and eigenvector entry points do not balance.
+ * Returns 0, -1 (NULL handle) or -2 (unknown job). */
+int tokuraSetBalancing(tokurablas_t *handle, char job);
+
 /* ---- differentiation ----------------------------------------------------
  * Forward with Schur vectors: besides eigenvalues[i + j*n] and flag, writes
  * the Schur factorisation A_j = U_j T_j U_j^H of every matrix into `schur`

@@ -12,6 +12,8 @@ Contents
   reference harness's oracle (stream_test_with_MKL/get_eig_MKL.c:26-51).
 * ``lapack_hrd_hseqr`` — LAPACK zgehrd + zhseqr('E','N') without balancing.
 * ``lapack_rconde`` — LAPACK zgeevx eigenvalue condition numbers (balanc='N').
+* ``zgebal``        — ZGEBAL restatement (permutation + power-of-two scaling),
+  or host LAPACK zgebal with ``use_lapack=True``.
 * ``method_schur`` / ``method_adjoint`` — Schur form and dL/dA from the same
   method, used to check the adjoint kernel.
 * ``method_vectors`` — eigenvectors by the same method (ZTREVC on the Schur
@@ -86,6 +88,12 @@ def lib():
         f = getattr(L, name)
         f.argtypes = [I, I, P, P, P, I]
         f.restype = I
+    L.oracle_zgeev_batched_bal.argtypes = [I, I, P, P, P, I, ctypes.c_char]
+    L.oracle_zgeev_batched_bal.restype = None
+    L.oracle_zgebal.argtypes = [ctypes.c_char, I, P, I, P, P, P]
+    L.oracle_zgebal.restype = I
+    L.lapack_zgebal.argtypes = [ctypes.c_char, I, P, P, P, P]
+    L.lapack_zgebal.restype = I
     L.lapack_zgeevx_rcond.argtypes = [I, I, P, P, P, P, I]
     L.lapack_zgeevx_rcond.restype = I
     rc = L.lapack_open(_lapack_path().encode())
@@ -106,12 +114,16 @@ def _p(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def method_zgeev(M, itmax: int = -1, with_sweeps: bool = False):
-    """Eigenvalues by the restated GPU method; returns (eig[b, i], flag[b])."""
+def method_zgeev(M, itmax: int = -1, with_sweeps: bool = False, balance: str = "N"):
+    """Eigenvalues by the restated GPU method; returns (eig[b, i], flag[b]).
+    balance: ZGEBAL job applied first ('N' none, 'P', 'S', 'B')."""
     A = _colmajor(M)
     b, n, _ = A.shape
     eig = np.zeros((b, n), np.complex128)
     flag = np.zeros(b, np.int8)
+    if balance != "N":
+        lib().oracle_zgeev_batched_bal(n, b, _p(A), _p(eig), _p(flag), itmax, balance.encode())
+        return eig, flag
     sweeps = np.zeros(b, np.int32)
     lib().oracle_zgeev_batched(n, b, _p(A), _p(eig), _p(flag), _p(sweeps), itmax)
     if with_sweeps:
@@ -187,6 +199,24 @@ def lapack_zgeev_vectors(M, nthreads: int = 0):
     info = np.zeros(b, np.int32)
     lib().lapack_zgeev_vectors_batched(n, b, _p(A), _p(w), _p(VL), _p(VR), _p(info), nthreads)
     return w, VL.transpose(0, 2, 1).copy(), VR.transpose(0, 2, 1).copy(), info
+
+
+def zgebal(M, job: str = "B", use_lapack: bool = False):
+    """Balance one matrix (M[r, c] sense): returns (B, ilo, ihi, scale, info)
+    with 0-based ilo/ihi; the oracle restatement, or host LAPACK zgebal when
+    use_lapack is set."""
+    A = _colmajor(M)[0].copy()
+    n = A.shape[0]
+    ilo, ihi = ctypes.c_int(), ctypes.c_int()
+    scale = np.zeros(n, np.float64)
+    if use_lapack:
+        info = lib().lapack_zgebal(job.encode(), n, _p(A), ctypes.byref(ilo), ctypes.byref(ihi), _p(scale))
+        lo, hi = ilo.value - 1, ihi.value - 1
+        scale = np.where((np.arange(n) < lo) | (np.arange(n) > hi), scale - 1.0, scale)  # 0-based indices
+    else:
+        info = lib().oracle_zgebal(job.encode(), n, _p(A), n, ctypes.byref(ilo), ctypes.byref(ihi), _p(scale))
+        lo, hi = ilo.value, ihi.value
+    return A.T.copy(), lo, hi, scale, info
 
 
 def lapack_zgeev(M, nthreads: int = 0):

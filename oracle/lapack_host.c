@@ -27,6 +27,7 @@ typedef double _Complex zc;
 typedef int (*zgeev_t)(int, char, char, int, zc *, int, zc *, zc *, int, zc *, int);
 typedef int (*zgehrd_t)(int, int, int, int, zc *, int, zc *);
 typedef int (*zhseqr_t)(int, char, char, int, int, int, zc *, int, zc *, zc *, int);
+typedef int (*zgebal_t)(int, char, int, zc *, int, int *, int *, double *);
 typedef int (*zgeevx_t)(int, char, char, char, char, int, zc *, int, zc *, zc *, int, zc *, int,
                         int *, int *, double *, double *, double *, double *);
 typedef void (*setthr_t)(int);
@@ -36,6 +37,7 @@ static zgeev_t p_zgeev;
 static zgehrd_t p_zgehrd;
 static zhseqr_t p_zhseqr;
 static zgeevx_t p_zgeevx;
+static zgebal_t p_zgebal;
 
 #define COLMAJOR 102
 
@@ -48,9 +50,10 @@ int lapack_open(const char *path)
     p_zgehrd = (zgehrd_t)dlsym(g_lib, "scipy_LAPACKE_zgehrd");
     p_zhseqr = (zhseqr_t)dlsym(g_lib, "scipy_LAPACKE_zhseqr");
     p_zgeevx = (zgeevx_t)dlsym(g_lib, "scipy_LAPACKE_zgeevx");
+    p_zgebal = (zgebal_t)dlsym(g_lib, "scipy_LAPACKE_zgebal");
     setthr_t st = (setthr_t)dlsym(g_lib, "scipy_openblas_set_num_threads");
     if (st) st(1);
-    if (!p_zgeev || !p_zgehrd || !p_zhseqr || !p_zgeevx) return -2;
+    if (!p_zgeev || !p_zgehrd || !p_zhseqr || !p_zgeevx || !p_zgebal) return -2;
     return 0;
 }
 
@@ -141,4 +144,12 @@ int lapack_zgeevx_rcond(int n, int batch, const zc *A, zc *w, double *rconde, in
         free(a); free(vl); free(vr); free(scale); free(rcondv);
     }
     return 0;
+}
+
+/* zgebal on one column-major matrix (in place); ilo/ihi 1-based as LAPACK
+ * returns them. */
+int lapack_zgebal(char job, int n, zc *a, int *ilo, int *ihi, double *scale)
+{
+    if (!p_zgebal) return -1;
+    return p_zgebal(COLMAJOR, job, n, a, n, ilo, ihi, scale);
 }

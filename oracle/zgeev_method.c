@@ -502,3 +502,133 @@ int oracle_zgeev_vectors(int n, const zc *A, zc *w, zc *VL, zc *VR)
     free(T); free(U); free(xr); free(yl);
     return info;
 }
+
+/* ZGEBAL (LAPACK 3.11+ form): balance the n x n column-major a.
+ * job: 'N' nothing, 'P' permute, 'S' scale, 'B' both.  On return rows/cols
+ * outside [ilo, ihi] (0-based) hold isolated eigenvalues; scale[j] is the
+ * permutation index (j outside [ilo, ihi]) or the scale factor (inside).
+ * Returns 0, or -3 for a NaN met while scaling. */
+int oracle_zgebal(char job, int n, zc *a, int lda, int *ilo_out, int *ihi_out, double *scale)
+{
+#define A(i, j) a[IDX(i, j, lda)]
+    const double sclfac = 2.0, factor = 0.95;
+    int k = 0, l = n - 1;
+    if (n == 0) { *ilo_out = 0; *ihi_out = -1; return 0; }
+    if (job == 'N') {
+        for (int i = 0; i < n; ++i) scale[i] = 1.0;
+        *ilo_out = 0; *ihi_out = n - 1;
+        return 0;
+    }
+    if (job != 'S') {
+        /* rows isolating an eigenvalue: push down */
+        int noconv = 1;
+        while (noconv) {
+            noconv = 0;
+            for (int i = l; i >= 0; --i) {
+                int canswap = 1;
+                for (int j = 0; j <= l; ++j)
+                    if (i != j && (creal(A(i, j)) != 0.0 || cimag(A(i, j)) != 0.0)) { canswap = 0; break; }
+                if (canswap) {
+                    scale[l] = i;
+                    if (i != l) {
+                        for (int r = 0; r <= l; ++r) { zc t = A(r, i); A(r, i) = A(r, l); A(r, l) = t; }
+                        for (int c = k; c < n; ++c) { zc t = A(i, c); A(i, c) = A(l, c); A(l, c) = t; }
+                    }
+                    noconv = 1;
+                    if (l == 0) { *ilo_out = 0; *ihi_out = 0; return 0; }
+                    --l;
+                }
+            }
+        }
+        /* columns isolating an eigenvalue: push left */
+        noconv = 1;
+        while (noconv) {
+            noconv = 0;
+            for (int j = k; j <= l; ++j) {
+                int canswap = 1;
+                for (int i = k; i <= l; ++i)
+                    if (i != j && (creal(A(i, j)) != 0.0 || cimag(A(i, j)) != 0.0)) { canswap = 0; break; }
+                if (canswap) {
+                    scale[k] = j;
+                    if (j != k) {
+                        for (int r = 0; r <= l; ++r) { zc t = A(r, j); A(r, j) = A(r, k); A(r, k) = t; }
+                        for (int c = k; c < n; ++c) { zc t = A(j, c); A(j, c) = A(k, c); A(k, c) = t; }
+                    }
+                    noconv = 1;
+                    ++k;
+                }
+            }
+        }
+    }
+    for (int i = k; i <= l; ++i) scale[i] = 1.0;
+    *ilo_out = k; *ihi_out = l;
+    if (job == 'P') return 0;
+    /* iterative norm reduction with powers of two */
+    const double sfmin1 = SAFMIN / ULP, sfmax1 = 1.0 / sfmin1;
+    const double sfmin2 = sfmin1 * sclfac, sfmax2 = 1.0 / sfmin2;
+    int noconv = 1;
+    while (noconv) {
+        noconv = 0;
+        for (int i = k; i <= l; ++i) {
+            double c = znrm2(l - k + 1, &A(k, i));
+            zc rowv[64];
+            for (int j = k; j <= l; ++j) rowv[j - k] = A(i, j);
+            double r = znrm2(l - k + 1, rowv);
+            /* IZAMAX picks the first entry of largest |re| + |im|; CA, RA are its modulus */
+            int ica = 0, ira = k;
+            for (int q = 1; q <= l; ++q) if (cabs1(A(q, i)) > cabs1(A(ica, i))) ica = q;
+            for (int q = k + 1; q < n; ++q) if (cabs1(A(i, q)) > cabs1(A(i, ira))) ira = q;
+            double ca = cabs(A(ica, i)), ra = cabs(A(i, ira));
+            if (c == 0.0 || r == 0.0) continue;
+            if (isnan(c + ca + r + ra)) return -3;
+            double g = r / sclfac, f = 1.0, s = c + r;
+            while (c < g && fmax(f, fmax(c, ca)) < sfmax2 && fmin(r, fmin(g, ra)) > sfmin2) {
+                f *= sclfac; c *= sclfac; ca *= sclfac; r /= sclfac; g /= sclfac; ra /= sclfac;
+            }
+            g = c / sclfac;
+            while (g >= r && fmax(r, ra) < sfmax2 && fmin(fmin(f, c), fmin(g, ca)) > sfmin2) {
+                f /= sclfac; c /= sclfac; g /= sclfac; ca /= sclfac; r *= sclfac; ra *= sclfac;
+            }
+            if (c + r >= factor * s) continue;
+            if (f < 1.0 && scale[i] < 1.0 && f * scale[i] <= sfmin1) continue;
+            if (f > 1.0 && scale[i] > 1.0 && scale[i] >= sfmax1 / f) continue;
+            g = 1.0 / f;
+            scale[i] *= f;
+            noconv = 1;
+            for (int q = k; q < n; ++q) A(i, q) *= g;
+            for (int q = 0; q <= l; ++q) A(q, i) *= f;
+        }
+    }
+    return 0;
+#undef A
+}
+
+/* The batched method with balancing first (ZGEBAL job 'N', 'P', 'S' or 'B'),
+ * then zgehd2 + eigenvalues-only zlahqr on the whole balanced matrix, as the
+ * GPU path does. */
+void oracle_zgeev_batched_bal(int n, int batch, const zc *A, zc *eig, signed char *flag, int itmax_override,
+                              char job)
+{
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (int b = 0; b < batch; ++b) {
+        zc *h = (zc *)malloc(sizeof(zc) * (size_t)n * n);
+        double *scale = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+        memcpy(h, A + (size_t)b * n * n, sizeof(zc) * (size_t)n * n);
+        zc *w = eig + (size_t)b * n;
+        if (!all_finite(n, h)) {
+            for (int i = 0; i < n; ++i) w[i] = CMPLX(NAN, NAN);
+            flag[b] = 2;
+        } else {
+            int ilo, ihi;
+            oracle_zgebal(job, n, h, n, &ilo, &ihi, scale);
+            oracle_zgehd2(n, h, n, NULL, NULL);
+            for (int i = 0; i < n; ++i) w[i] = h[IDX(i, i, n)];
+            int info = oracle_zlahqr(0, 0, n, h, n, w, NULL, n, itmax_override, NULL);
+            if (info != 0)
+                for (int i = 0; i < info; ++i) w[i] = h[IDX(i, i, n)];
+            flag[b] = info != 0 ? 1 : 0;
+        }
+        free(h);
+        free(scale);
+    }
+}

@@ -120,3 +120,13 @@ def test_vectors_entry_validation():
     assert g(h, 4, 0, a, e, vl, vr, f, s) == -6
     assert g(h, 4, 1, a, e, vl, vr, None, s) == -7
     assert L.tokuraDestroy(h) == 0
+
+
+def test_balancing_switch():
+    L = tk.lib()
+    h = make_handle()
+    for job in (b"N", b"P", b"S", b"B", b"b"):
+        assert L.tokuraSetBalancing(h, job) == 0
+    assert L.tokuraSetBalancing(h, b"X") == -2
+    assert L.tokuraSetBalancing(None, b"B") == -1
+    assert L.tokuraDestroy(h) == 0

@@ -303,3 +303,61 @@ def test_vectors_hermitian_left_equals_right():
     for b in range(3):
         np.testing.assert_allclose(VL[b], VR[b], atol=1e-12)
         np.testing.assert_allclose(VR[b].conj().T @ VR[b], np.eye(12), atol=1e-12)
+
+
+def _graded(rng, b, n, span):
+    M = rand(rng, b, n)
+    d = 2.0 ** np.linspace(0, span, n)
+    return (d[None, :, None] * M) / d[None, None, :], M
+
+
+def _balancing_cases(rng):
+    cases = []
+    for n in (2, 5, 16, 40, 64):
+        cases.append(rand(rng, 1, n)[0])
+        cases.append(_graded(rng, 1, n, 40)[0][0])
+    n = 12
+    T = np.triu(rand(rng, 1, n)[0])
+    T[4:9, 4:9] = rand(rng, 1, 5)[0]
+    P = rng.permutation(n)
+    cases.append(T[np.ix_(P, P)])                    # isolated eigenvalues after permutation
+    Z = rand(rng, 1, 10)[0]
+    Z[6, :] = 0
+    Z[6, 6] = 2
+    Z[:, 1] = 0
+    Z[1, 1] = -1
+    cases.append(Z)                                  # a zero row and a zero column
+    return cases
+
+
+def test_zgebal_matches_lapack_bitwise():
+    # the restatement of ZGEBAL reproduces host LAPACK zgebal exactly
+    # (permutations and power-of-two scalings are exact operations)
+    rng = np.random.default_rng(30)
+    for M in _balancing_cases(rng):
+        for job in "NPSB":
+            B1, lo1, hi1, s1, i1 = oracle.zgebal(M, job)
+            B2, lo2, hi2, s2, i2 = oracle.zgebal(M, job, use_lapack=True)
+            assert (lo1, hi1) == (lo2, hi2), job
+            np.testing.assert_array_equal(B1, B2)
+            np.testing.assert_array_equal(s1, s2)
+
+
+def test_balanced_method_on_graded_matrices():
+    # A = D M D^-1 with D spanning 2^40: the unbalanced method loses the
+    # eigenvalues (error ~ eps ||A||), the balanced one matches LAPACK zgeev
+    rng = np.random.default_rng(31)
+    G, M = _graded(rng, 6, 24, 40)
+    ref = np.stack([np.linalg.eigvals(m) for m in M])
+    e, f = oracle.method_zgeev(G, balance="B")
+    assert not f.any()
+    assert oracle.max_relerr(ref, e) < 1e-12
+    el, _ = oracle.lapack_zgeev(G)
+    assert oracle.max_relerr(el, e) < 1e-12
+    e0, _ = oracle.method_zgeev(G)
+    assert oracle.max_relerr(ref, e0) > 1e-6          # balancing is what makes the difference
+    # balancing never changes the spectrum of a well-scaled matrix beyond rounding
+    R = rand(rng, 4, 30)
+    a, _ = oracle.method_zgeev(R)
+    b, _ = oracle.method_zgeev(R, balance="B")
+    assert oracle.max_relerr(a, b) < 1e-12

@@ -45,6 +45,7 @@ _SIGNATURES = {
     "tokura_get_zgeeveigenvaluesgetworspacesize": ([_I, _I], _SZ),
     "tokura_zgeev_batched_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P], _I),
     "tokuraSetMaxSweeps": ([_P, _I], _I),
+    "tokuraSetBalancing": ([_P, ctypes.c_char], _I),
     "tokuraGetLaunchConfig": ([_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "tokuraAutotune": ([_P, _I, _I, _I, ctypes.c_char_p, _P], _I),
     "tokuraLoadTuning": ([_P, ctypes.c_char_p], _I),
@@ -133,6 +134,11 @@ class Handle:
 
     def set_max_sweeps(self, k: int):
         _check(lib().tokuraSetMaxSweeps(self._h, k), "tokuraSetMaxSweeps")
+
+    def set_balancing(self, job: str):
+        """LAPACK ZGEBAL job for this handle's eigenvalue paths: 'N' (default),
+        'P', 'S' or 'B'."""
+        _check(lib().tokuraSetBalancing(self._h, job.encode()[:1]), "tokuraSetBalancing")
 
     def launch_config(self, n: int):
         v, l, m = _I(), _I(), _I()

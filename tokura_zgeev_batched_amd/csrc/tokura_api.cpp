@@ -107,8 +107,8 @@ extern "C" int tokura_zgeev_batched_gpu(tokurablas_t *handle, int n, int batch_c
     if (!variant_fits(v, n)) v = default_variant(n);
     int stage = 1;
     std::array<hipEvent_t, 3> *ev = handle->stage_timing ? handle->timing_begin(stream) : nullptr;
-    hipError_t e = launch_plain(v, n, batch_count, A, eigenvalues, flag, work, handle->max_sweeps, stream, &stage,
-                                ev ? (*ev)[1] : nullptr);
+    hipError_t e = launch_plain(v, n, batch_count, A, eigenvalues, flag, work, handle->max_sweeps, handle->balance,
+                                stream, &stage, ev ? (*ev)[1] : nullptr);
     if (ev) {
         if (variant_info(v)->kind == VK_FUSED) hipEventRecord((*ev)[1], stream);
         handle->timing_end(ev, stream);
@@ -166,6 +166,18 @@ extern "C" int tokuraSetMaxSweeps(tokurablas_t *handle, int max_sweeps)
     if (handle == nullptr) return -1;
     handle->max_sweeps = max_sweeps < 0 ? -1 : max_sweeps;
     return 0;
+}
+
+extern "C" int tokuraSetBalancing(tokurablas_t *handle, char job)
+{
+    if (handle == nullptr) return -1;
+    switch (job) {
+    case 'N': case 'n': handle->balance = 0; return 0;
+    case 'P': case 'p': handle->balance = BAL_PERMUTE; return 0;
+    case 'S': case 's': handle->balance = BAL_SCALE; return 0;
+    case 'B': case 'b': handle->balance = BAL_PERMUTE | BAL_SCALE; return 0;
+    default: return -2;
+    }
 }
 
 extern "C" int tokuraGetLaunchConfig(tokurablas_t *handle, int n, int *variant, int *lanes_per_matrix,
@@ -253,7 +265,10 @@ extern "C" int tokuraAutotune(tokurablas_t *handle, int n_lo, int n_hi, int batc
             float tms = 0.f;
             for (int rep = 0; rep < 3; ++rep) {
                 hipEventRecord(e0, stream);
-                if (launch_plain(v, n, batch_count, dA, dE, dF, dW, -1, stream, &stage) != hipSuccess) { rc = 4; break; }
+                if (launch_plain(v, n, batch_count, dA, dE, dF, dW, -1, handle->balance, stream, &stage) != hipSuccess) {
+                    rc = 4;
+                    break;
+                }
                 hipEventRecord(e1, stream);
                 hipEventSynchronize(e1);
                 float ms;

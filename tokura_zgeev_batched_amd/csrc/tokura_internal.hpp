@@ -17,6 +17,7 @@ struct RaggedGraphCache {
     int batch = -1;
     int cfg = -1;
     int max_sweeps = -1;
+    int balance = 0;
 };
 
 struct RaggedPrepared {
@@ -31,6 +32,7 @@ struct tokurablas_t {
     int cu_count;
     char arch[64];
     int max_sweeps;                                           // < 0: LAPACK cap
+    int balance = 0;                                          // ZGEBAL job bits (1 permute, 2 scale)
     int variant_for_n[TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE + 1];  // same-size path
     int ragged_cfg = 0;                                       // ragged class configuration
     RaggedGraphCache graph;

@@ -84,7 +84,7 @@ __global__ void k_ragged_reset(char *work) { ((RaggedHeader *)work)->counter = 0
 
 template <int NB, int LPM>
 __device__ __forceinline__ void hess_tile(zd *lds, const RaggedView &V, int4 td, const zd *__restrict__ A,
-                                          int8_t *__restrict__ flag)
+                                          int8_t *__restrict__ flag, int bal)
 {
     constexpr int LDP = NB | 1;
     const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
@@ -95,6 +95,7 @@ __device__ __forceinline__ void hess_tile(zd *lds, const RaggedView &V, int4 td,
     const FullLayout lay{n | 1};
     const bool ok = load_full<LPM>(H, lay, A + V.moff[b], n, t);
     lds_order();
+    if (ok && bal) zgebal_group<LPM, (NB + LPM - 1) / LPM>(H, lay, n, t, bal);
     if (ok) hessenberg<LPM>(H, lay, n, t);
     store_packed<LPM>(H, lay, V.H + V.hoff[b], n, t);
     if (t == 0) flag[b] = ok ? 0 : 2;
@@ -135,7 +136,8 @@ __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 // Hessenberg of the small classes (B, C): one wavefront per tile.
 template <int LB, int LC>
-__global__ __launch_bounds__(64) void k_ragged_hess(const zd *__restrict__ A, char *work, int8_t *__restrict__ flag)
+__global__ __launch_bounds__(64) void k_ragged_hess(const zd *__restrict__ A, char *work, int8_t *__restrict__ flag,
+                                                    int bal)
 {
     constexpr int SZ = cmax((64 / LB) * 32 * 33, (64 / LC) * 16 * 17);
     __shared__ zd lds[SZ];
@@ -143,8 +145,8 @@ __global__ __launch_bounds__(64) void k_ragged_hess(const zd *__restrict__ A, ch
     const int nt = V.h->ntiles;
     for (int tile = V.h->tile_small0 + blockIdx.x; tile < nt; tile += gridDim.x) {
         const int4 td = V.tiles[tile];
-        if (td.x == 2) hess_tile<32, LB>(lds, V, td, A, flag);
-        else hess_tile<16, LC>(lds, V, td, A, flag);
+        if (td.x == 2) hess_tile<32, LB>(lds, V, td, A, flag, bal);
+        else hess_tile<16, LC>(lds, V, td, A, flag, bal);
         lds_order();
     }
 }
@@ -152,7 +154,7 @@ __global__ __launch_bounds__(64) void k_ragged_hess(const zd *__restrict__ A, ch
 // Hessenberg of the large classes (A1, A2): one 256-thread workgroup per
 // matrix, quads of threads sharing each row/column.
 __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A, char *work,
-                                                        int8_t *__restrict__ flag)
+                                                        int8_t *__restrict__ flag, int bal)
 {
     __shared__ zd lds[64 * 65];
     __shared__ double red[8];
@@ -171,6 +173,10 @@ __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A
             lds[lay(e % n, e / n)] = v;
         }
         const int ok = __syncthreads_or(bad) == 0;
+        if (ok && bal) {             // balancing by the first wavefront
+            if (tid < 64) zgebal_group<64, 1>(lds, lay, n, tid, bal);
+            __syncthreads();
+        }
         if (ok) hessenberg_wg<256, 4>(lds, lay, n, red);
         zd *dst = V.H + V.hoff[b];
         const PackedLayout pk;
@@ -206,7 +212,7 @@ __global__ __launch_bounds__(64) void k_ragged_qr(char *work, zd *__restrict__ e
 }
 
 template <int LA1, int LA2, int LB, int LC>
-static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int cus,
+static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int bal, int cus,
                                     hipStream_t s)
 {
     hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(1), 0, s, work);
@@ -224,22 +230,22 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
         if (occ_w < 1) occ_w = 1;
         if (occ_q < 1) occ_q = 1;
     }
-    hipLaunchKernelGGL(k_ragged_hess_wg, dim3(cus * occ_w * 4), dim3(256), 0, s, A, work, flag);
+    hipLaunchKernelGGL(k_ragged_hess_wg, dim3(cus * occ_w * 4), dim3(256), 0, s, A, work, flag, bal);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag);
+    hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag, bal);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_ragged_qr<LA1, LA2, LB, LC>), dim3(cus * occ_q), dim3(64), 0, s, work, eig, flag, itmax);
     return hipGetLastError();
 }
 
-static hipError_t launch_ragged(int cfg, const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int cus,
-                                hipStream_t s)
+static hipError_t launch_ragged(int cfg, const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int bal,
+                                int cus, hipStream_t s)
 {
     switch (cfg) {
 #define X(id, a1, a2, b, c) \
-    case id: return launch_ragged_cfg<a1, a2, b, c>(A, work, eig, flag, itmax, cus, s);
+    case id: return launch_ragged_cfg<a1, a2, b, c>(A, work, eig, flag, itmax, bal, cus, s);
         TKZ_RAGGED_CFGS(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -406,7 +412,8 @@ extern "C" int tokura_zgeev_ragged_gpu(tokurablas_t *handle, int batch_count, co
     const int cfg = prepared_cfg(handle, work, batch_count);
     if (cfg < 0) return -8;
     hipError_t e = launch_ragged(cfg, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
-                                 handle->max_sweeps, handle->cu_count > 0 ? handle->cu_count : 1, stream);
+                                 handle->max_sweeps, handle->balance, handle->cu_count > 0 ? handle->cu_count : 1,
+                                 stream);
     return e == hipSuccess ? 0 : 2;
 }
 
@@ -423,7 +430,8 @@ extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, 
     if (cfg < 0) return -8;
     RaggedGraphCache &G = handle->graph;
     const bool hit = G.exec && G.A == (const void *)A && G.eig == (void *)eigenvalues && G.work == work &&
-                     G.flag == flag && G.batch == batch_count && G.cfg == cfg && G.max_sweeps == handle->max_sweeps;
+                     G.flag == flag && G.batch == batch_count && G.cfg == cfg && G.max_sweeps == handle->max_sweeps &&
+                     G.balance == handle->balance;
     if (!hit) {
         if (G.exec) hipGraphExecDestroy(G.exec);
         if (G.graph) hipGraphDestroy(G.graph);
@@ -433,7 +441,8 @@ extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, 
         hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
             hipError_t el = launch_ragged(cfg, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
-                                          handle->max_sweeps, handle->cu_count > 0 ? handle->cu_count : 1, cs);
+                                          handle->max_sweeps, handle->balance,
+                                          handle->cu_count > 0 ? handle->cu_count : 1, cs);
             e = hipStreamEndCapture(cs, &G.graph);
             if (el != hipSuccess) e = el;
         }
@@ -446,6 +455,7 @@ extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, 
         }
         G.A = A; G.eig = eigenvalues; G.work = work; G.flag = flag; G.batch = batch_count; G.cfg = cfg;
         G.max_sweeps = handle->max_sweeps;
+        G.balance = handle->balance;
     }
     return hipGraphLaunch(G.exec, stream) == hipSuccess ? 0 : 2;
 }

@@ -32,6 +32,8 @@
 #include <float.h>
 #include <type_traits>
 
+#include "zgeev_launch.hpp"
+
 namespace tkz {
 
 struct __align__(16) zd {
@@ -389,6 +391,169 @@ __device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red)
         if (tid == 0) A[lay(i + 1, i)] = mk(beta, 0.0);
         for (int r = i + 2 + tid; r < n; r += NT) A[lay(r, i)] = mk(0.0, 0.0);
         __syncthreads();
+    }
+}
+
+// ---- balancing (LAPACK ZGEBAL, the 3.11+ algorithm) ----
+// job bit 0: permute rows/columns to isolate eigenvalues; bit 1: scale rows
+// and columns by powers of two until their norms are comparable.  The matrix
+// is held by a group of LPM lanes (positions p = t + s*LPM).  Only the
+// balanced matrix is produced: the Hessenberg reduction and QR that follow
+// run on the whole matrix, and the block triangular structure the
+// permutations create survives both (reflectors of columns whose tail is
+// already zero are the identity; the isolated eigenvalues deflate at once).
+template <int LPM, int S>
+__device__ void zgebal_group(zd *A, const FullLayout lay, int n, int t, int job)
+{
+    int k = 0, l = n - 1;
+    // exchange column i with j over rows 0..l, then row i with j over columns k..n-1
+    auto swap_rc = [&](int i, int j) {
+        for (int r = t; r <= l; r += LPM) {
+            const zd x = A[lay(r, i)];
+            A[lay(r, i)] = A[lay(r, j)];
+            A[lay(r, j)] = x;
+        }
+        lds_order();
+        for (int c = k + t; c < n; c += LPM) {
+            const zd x = A[lay(i, c)];
+            A[lay(i, c)] = A[lay(j, c)];
+            A[lay(j, c)] = x;
+        }
+        lds_order();
+    };
+    auto nonzero = [](const zd z) { return z.re != 0.0 || z.im != 0.0; };
+    if (job & BAL_PERMUTE) {
+        // rows with no off-diagonal nonzero in columns 0..l: push down
+        bool noconv = true;
+        while (noconv) {
+            noconv = false;
+            for (int i = l; i >= 0; --i) {
+                bool nz = false;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int j = t + s * LPM;
+                    if (j <= l && j != i) nz |= nonzero(A[lay(i, j)]);
+                }
+                if (!group_any<LPM>(nz)) {
+                    if (i != l) swap_rc(i, l);
+                    noconv = true;
+                    if (l == 0) return;
+                    --l;
+                }
+            }
+        }
+        // columns with no off-diagonal nonzero in rows k..l: push left
+        noconv = true;
+        while (noconv) {
+            noconv = false;
+            for (int j = k; j <= l; ++j) {
+                bool nz = false;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int i = t + s * LPM;
+                    if (i >= k && i <= l && i != j) nz |= nonzero(A[lay(i, j)]);
+                }
+                if (!group_any<LPM>(nz)) {
+                    if (j != k) swap_rc(j, k);
+                    noconv = true;
+                    ++k;
+                }
+            }
+        }
+    }
+    if (!(job & BAL_SCALE)) return;
+    const double sclfac = 2.0, factor = 0.95;
+    const double sfmin1 = DBL_MIN / DBL_EPSILON, sfmax1 = 1.0 / sfmin1;
+    const double sfmin2 = sfmin1 * sclfac, sfmax2 = 1.0 / sfmin2;
+    double sc[S];                                  // scale factor of position p
+#pragma unroll
+    for (int s = 0; s < S; ++s) sc[s] = 1.0;
+    // first index of the largest |re| + |im| over the group; returns its modulus
+    auto izamax_abs = [&](double v1, int idx, zd z) {
+        double best = v1;
+        int bi = idx;
+#pragma unroll
+        for (int o = LPM / 2; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o, LPM);
+            const int oi = __shfl_xor(bi, o, LPM);
+            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        // the owner lane of bi holds the element in z when bi is its position
+        double m = (idx == bi) ? zabs(z) : 0.0;
+        return group_max<LPM>(m);
+    };
+    bool noconv = true;
+    for (int sweep = 0; noconv && sweep < 100; ++sweep) {   // LAPACK converges in a few sweeps
+        noconv = false;
+        for (int i = k; i <= l; ++i) {
+            // column i: rows k..l for the norm, rows 0..l for IZAMAX;
+            // row i: columns k..l for the norm, columns k..n-1 for IZAMAX
+            double cm = 0.0, rm = 0.0;
+            double ckey = -1.0, rkey = -1.0;
+            int cidx = 0x7fffffff, ridx = 0x7fffffff;
+            zd cz = mk(0.0, 0.0), rz = cz;
+            zd colv[S], rowv[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int q = t + s * LPM;
+                colv[s] = (q <= l) ? A[lay(q, i)] : mk(0.0, 0.0);
+                rowv[s] = (q >= k && q < n) ? A[lay(i, q)] : mk(0.0, 0.0);
+                if (q >= k && q <= l) {
+                    cm = fmax(cm, fmax(fabs(colv[s].re), fabs(colv[s].im)));
+                    rm = fmax(rm, fmax(fabs(rowv[s].re), fabs(rowv[s].im)));
+                }
+                if (q <= l && cabs1(colv[s]) > ckey) { ckey = cabs1(colv[s]); cidx = q; cz = colv[s]; }
+                if (q >= k && q < n && cabs1(rowv[s]) > rkey) { rkey = cabs1(rowv[s]); ridx = q; rz = rowv[s]; }
+            }
+            cm = group_max<LPM>(cm);
+            rm = group_max<LPM>(rm);
+            double cs = 0.0, rs = 0.0;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int q = t + s * LPM;
+                if (q >= k && q <= l) {
+                    if (cm > 0.0) {
+                        const double a = colv[s].re / cm, b = colv[s].im / cm;
+                        cs += a * a + b * b;
+                    }
+                    if (rm > 0.0) {
+                        const double a = rowv[s].re / rm, b = rowv[s].im / rm;
+                        rs += a * a + b * b;
+                    }
+                }
+            }
+            double c = cm * sqrt(group_sum<LPM>(cs));
+            double r = rm * sqrt(group_sum<LPM>(rs));
+            double ca = izamax_abs(ckey, cidx, cz);
+            double ra = izamax_abs(rkey, ridx, rz);
+            if (c == 0.0 || r == 0.0) continue;
+            double g = r / sclfac, f = 1.0;
+            const double ssum = c + r;
+            while (c < g && fmax(f, fmax(c, ca)) < sfmax2 && fmin(r, fmin(g, ra)) > sfmin2) {
+                f *= sclfac; c *= sclfac; ca *= sclfac; r /= sclfac; g /= sclfac; ra /= sclfac;
+            }
+            g = c / sclfac;
+            while (g >= r && fmax(r, ra) < sfmax2 && fmin(fmin(f, c), fmin(g, ca)) > sfmin2) {
+                f /= sclfac; c /= sclfac; g /= sclfac; ca /= sclfac; r *= sclfac; ra *= sclfac;
+            }
+            if (c + r >= factor * ssum) continue;
+            double sci = 0.0;                       // scale factor of position i
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                if (t + s * LPM == i) sci = sc[s];
+            sci = group_max<LPM>(sci);
+            if (f < 1.0 && sci < 1.0 && f * sci <= sfmin1) continue;
+            if (f > 1.0 && sci > 1.0 && sci >= sfmax1 / f) continue;
+            g = 1.0 / f;
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                if (t + s * LPM == i) sc[s] = sci * f;
+            noconv = true;
+            for (int q = k + t; q < n; q += LPM) A[lay(i, q)] = zscale(g, A[lay(i, q)]);
+            lds_order();
+            for (int q = t; q <= l; q += LPM) A[lay(q, i)] = zscale(f, A[lay(q, i)]);
+            lds_order();
+        }
     }
 }
 

@@ -32,7 +32,7 @@ struct Geometry {
 template <int NB, int LPM>
 __global__ __launch_bounds__(64) void k_zgeev_fused(int n, int batch, const zd *__restrict__ A,
                                                     zd *__restrict__ eig, int8_t *__restrict__ flag,
-                                                    int itmax)
+                                                    int itmax, int bal)
 {
     using G = Geometry<NB, LPM>;
     __shared__ zd lds[G::GPW * NB * G::LDP];
@@ -50,6 +50,7 @@ __global__ __launch_bounds__(64) void k_zgeev_fused(int n, int batch, const zd *
         if (t == 0) flag[mat] = 2;
         return;
     }
+    if (bal) zgebal_group<LPM, G::S>(H, lay, n, t, bal);
     hessenberg<LPM>(H, lay, n, t);
     int info = 0;
     if (n > 1) info = zlahqr_eigs<LPM, G::S>(H, lay, n, t, itmax);
@@ -60,7 +61,8 @@ __global__ __launch_bounds__(64) void k_zgeev_fused(int n, int batch, const zd *
 // Stage 1 of the split path: Hessenberg in LDS, packed store to `work`.
 template <int NB, int LPM>
 __global__ __launch_bounds__(64) void k_hessenberg_packed(int n, int batch, const zd *__restrict__ A,
-                                                          zd *__restrict__ work, int8_t *__restrict__ flag)
+                                                          zd *__restrict__ work, int8_t *__restrict__ flag,
+                                                          int bal)
 {
     using G = Geometry<NB, LPM>;
     __shared__ zd lds[G::GPW * NB * G::LDP];
@@ -72,6 +74,7 @@ __global__ __launch_bounds__(64) void k_hessenberg_packed(int n, int batch, cons
     const FullLayout lay{n | 1};
     const bool ok = load_full<LPM>(H, lay, A + mat * (long long)n * n, n, t);
     lds_order();
+    if (ok && bal) zgebal_group<LPM, G::S>(H, lay, n, t, bal);
     if (ok) hessenberg<LPM>(H, lay, n, t);
     store_packed<LPM>(H, lay, work + mat * (long long)packed_size(n), n, t);
     if (t == 0) flag[mat] = ok ? 0 : 2;
@@ -82,7 +85,8 @@ __global__ __launch_bounds__(64) void k_hessenberg_packed(int n, int batch, cons
 // of 256 threads fit per CU, two waves per SIMD.
 template <int NB, int NT>
 __global__ __launch_bounds__(NT) void k_hessenberg_packed_wg(int n, int batch, const zd *__restrict__ A,
-                                                             zd *__restrict__ work, int8_t *__restrict__ flag)
+                                                             zd *__restrict__ work, int8_t *__restrict__ flag,
+                                                             int bal)
 {
     constexpr int LDP = NB | 1;
     __shared__ zd lds[NB * LDP];
@@ -99,6 +103,10 @@ __global__ __launch_bounds__(NT) void k_hessenberg_packed_wg(int n, int batch, c
         lds[lay(e % n, e / n)] = v;
     }
     const int ok = __syncthreads_or(bad) == 0;
+    if (ok && bal) {                 // balancing by the first wavefront
+        if (tid < 64) zgebal_group<64, (NB + 63) / 64>(lds, lay, n, tid, bal);
+        __syncthreads();
+    }
     if (ok) hessenberg_wg<NT, NT / 64>(lds, lay, n, red);
     zd *dst = work + mat * (long long)packed_size(n);
     const PackedLayout pk;
@@ -180,25 +188,27 @@ int num_variants() { return (int)(sizeof(g_variants) / sizeof(g_variants[0])); }
 const VariantInfo *variant_info(int v) { return (v >= 0 && v < num_variants()) ? &g_variants[v] : nullptr; }
 
 template <int NB, int LPM>
-static hipError_t launch_fused(int n, int batch, const zd *A, zd *eig, int8_t *flag, int itmax, hipStream_t s)
+static hipError_t launch_fused(int n, int batch, const zd *A, zd *eig, int8_t *flag, int itmax, int bal,
+                               hipStream_t s)
 {
     constexpr int GPW = 64 / LPM;
     const int blocks = (batch + GPW - 1) / GPW;
-    hipLaunchKernelGGL((k_zgeev_fused<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, A, eig, flag, itmax);
+    hipLaunchKernelGGL((k_zgeev_fused<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, A, eig, flag, itmax, bal);
     return hipGetLastError();
 }
 
 template <int NB, int LPM, bool DUAL = false>
 static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *flag, zd *work, int itmax,
-                               hipStream_t s, int *stage, hipEvent_t mid)
+                               int bal, hipStream_t s, int *stage, hipEvent_t mid)
 {
     constexpr int GPW = 64 / LPM;
     const int blocks = (batch + GPW - 1) / GPW;
     *stage = 1;
     if (NB >= 48)   // large matrices: one 256-thread workgroup per matrix (512 threads measured slower)
-        hipLaunchKernelGGL((k_hessenberg_packed_wg<NB, 256>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag);
+        hipLaunchKernelGGL((k_hessenberg_packed_wg<NB, 256>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag,
+                           bal);
     else
-        hipLaunchKernelGGL((k_hessenberg_packed<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, A, work, flag);
+        hipLaunchKernelGGL((k_hessenberg_packed<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, A, work, flag, bal);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (mid) hipEventRecord(mid, s);
@@ -209,7 +219,7 @@ static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *f
 }
 
 hipError_t launch_plain(int variant, int n, int batch, const void *A, void *eig, char *flag, void *work,
-                        int itmax, hipStream_t s, int *stage, hipEvent_t mid)
+                        int itmax, int bal, hipStream_t s, int *stage, hipEvent_t mid)
 {
     const zd *a = (const zd *)A;
     zd *w = (zd *)eig;
@@ -219,8 +229,8 @@ hipError_t launch_plain(int variant, int n, int batch, const void *A, void *eig,
     switch (variant) {
 #define X(id, kind, nb, lpm)                                                              \
     case id:                                                                              \
-        if constexpr (VK_##kind == VK_FUSED) return launch_fused<nb, lpm>(n, batch, a, w, f, itmax, s); \
-        else return launch_split<nb, lpm, VK_##kind == VK_DUAL>(n, batch, a, w, f, wk, itmax, s, stage, mid);
+        if constexpr (VK_##kind == VK_FUSED) return launch_fused<nb, lpm>(n, batch, a, w, f, itmax, bal, s); \
+        else return launch_split<nb, lpm, VK_##kind == VK_DUAL>(n, batch, a, w, f, wk, itmax, bal, s, stage, mid);
         TKZ_VARIANTS(X)
 #undef X
     default:

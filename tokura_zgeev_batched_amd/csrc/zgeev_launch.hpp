@@ -7,6 +7,9 @@ namespace tkz {
 // VK_DUAL: split path whose QR runs two bulges per sweep (one matrix per wave)
 enum VariantKind { VK_FUSED = 0, VK_SPLIT = 1, VK_DUAL = 2 };
 
+// ZGEBAL job bits of tokuraSetBalancing ('P' = permute, 'S' = scale, 'B' = both)
+enum BalanceBits { BAL_PERMUTE = 1, BAL_SCALE = 2 };
+
 // One compiled launch configuration of the same-size path.
 //   nb   largest n the LDS allocation holds
 //   lpm  lanes per matrix (a wavefront holds 64/lpm matrices)
@@ -19,10 +22,11 @@ int num_variants();
 const VariantInfo *variant_info(int v);
 
 // Launches variant `variant` on `s`; *stage reports 1 (Hessenberg) or 2 (QR)
-// for the launch that failed.
+// for the launch that failed.  bal: ZGEBAL job bits (1 permute, 2 scale),
+// 0 for none.
 // When `mid` is non-null it is recorded on `s` between the Hessenberg and the
 // QR kernel of a split variant (fused variants record nothing).
 hipError_t launch_plain(int variant, int n, int batch, const void *A, void *eig, char *flag, void *work,
-                        int itmax, hipStream_t s, int *stage, hipEvent_t mid = nullptr);
+                        int itmax, int bal, hipStream_t s, int *stage, hipEvent_t mid = nullptr);
 
 }  // namespace tkz
