@@ -300,53 +300,78 @@ __device__ void hessenberg(zd *A, const FullLayout lay, int n, int t, zd *taus =
 
 // ---- ZGEHD2 by a whole workgroup of NT threads on one LDS matrix ----
 // Same algorithm as hessenberg<>(); each row/column dot product is split over
-// TPR consecutive threads and combined with shuffles.  Workgroup barriers
-// separate the phases.  `red` is LDS scratch of at least 2*NT/64 doubles.
+// TPR consecutive threads and combined with shuffles.  Three workgroup
+// barriers per column: after the reflector vector is scaled, after the
+// right update, and after the left update.  The threads that update column
+// i+1 in the left update also form the sum of squares and max of its tail
+// (the next reflector's x), so no separate norm pass is needed; entries
+// below the sub-diagonal are zeroed once at the end.  `red` is LDS scratch
+// of at least 2 doubles.
 template <int NT, int TPR>
 __device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red)
 {
     const int tid = threadIdx.x;
     const int q = tid % TPR, row = tid / TPR;     // lane within the row team, row/column slot
-    for (int i = 0; i + 2 <= n; ++i) {
-        const int m = n - 1 - i;
+    const int ld = lay.ld;
+    // tail norm of column 0 (rows 2..n-1) by the first team
+    if (row == 0) {
         double ssq = 0.0, amx = 0.0;
-        for (int r = i + 2 + tid; r < n; r += NT) {
-            zd v = A[lay(r, i)];
+        for (int r = 2 + q; r < n; r += TPR) {
+            const zd v = A[lay(r, 0)];
             ssq += v.re * v.re + v.im * v.im;
             amx = fmax(amx, fmax(fabs(v.re), fabs(v.im)));
         }
-        ssq = group_sum<64>(ssq);
-        amx = group_max<64>(amx);
-        if ((tid & 63) == 0) { red[tid >> 6] = ssq; red[(NT >> 6) + (tid >> 6)] = amx; }
-        __syncthreads();
-        ssq = 0.0;
-        amx = 0.0;
 #pragma unroll
-        for (int w = 0; w < NT / 64; ++w) { ssq += red[w]; amx = fmax(amx, red[(NT >> 6) + w]); }
+        for (int o = 1; o < TPR; o <<= 1) {
+            ssq += __shfl_xor(ssq, o, TPR);
+            amx = fmax(amx, __shfl_xor(amx, o, TPR));
+        }
+        if (q == 0) { red[0] = ssq; red[1] = amx; }
+    }
+    __syncthreads();
+    for (int i = 0; i + 2 <= n; ++i) {
+        const int m = n - 1 - i;
+        const double ssq = red[0], amx = red[1];
         const zd alpha = A[lay(i + 1, i)];
         double xnorm = 0.0;
         if (amx > 0.0) {
             if (ssq > 1e-290 && ssq < 1e290) {
                 xnorm = sqrt(ssq);
-            } else {
+            } else {                      // rare: rescaled second pass
                 const double ia = 1.0 / amx;
                 double s2 = 0.0;
-                for (int r = i + 2 + tid; r < n; r += NT) {
-                    zd v = A[lay(r, i)];
-                    s2 += (v.re * ia) * (v.re * ia) + (v.im * ia) * (v.im * ia);
-                }
-                s2 = group_sum<64>(s2);
-                __syncthreads();
-                if ((tid & 63) == 0) red[tid >> 6] = s2;
-                __syncthreads();
-                s2 = 0.0;
+                if (row == 0)
+                    for (int r = i + 2 + q; r < n; r += TPR) {
+                        const zd v = A[lay(r, i)];
+                        s2 += (v.re * ia) * (v.re * ia) + (v.im * ia) * (v.im * ia);
+                    }
 #pragma unroll
-                for (int w = 0; w < NT / 64; ++w) s2 += red[w];
-                xnorm = amx * sqrt(s2);
+                for (int o = 1; o < TPR; o <<= 1) s2 += __shfl_xor(s2, o, TPR);
+                __syncthreads();          // everyone has read red[]
+                if (tid == 0) red[0] = s2;
+                __syncthreads();
+                xnorm = amx * sqrt(red[0]);
             }
         }
-        __syncthreads();   // everyone has read red[] and alpha
-        if (xnorm == 0.0 && alpha.im == 0.0) continue;   // tau = 0, H(i) = I
+        if (xnorm == 0.0 && alpha.im == 0.0) {   // tau = 0, H(i) = I; next tail norm
+            __syncthreads();                      // everyone has read red[] and alpha
+            if (row == 0) {
+                double s = 0.0, mx = 0.0;
+                for (int r = i + 3 + q; r < n; r += TPR) {
+                    const zd v = A[lay(r, i + 1)];
+                    s += v.re * v.re + v.im * v.im;
+                    mx = fmax(mx, fmax(fabs(v.re), fabs(v.im)));
+                }
+#pragma unroll
+                for (int o = 1; o < TPR; o <<= 1) {
+                    s += __shfl_xor(s, o, TPR);
+                    mx = fmax(mx, __shfl_xor(mx, o, TPR));
+                }
+                if (q == 0) { red[0] = s; red[1] = mx; }
+            }
+            __syncthreads();
+            continue;
+        }
         double w = fmax(fmax(fabs(alpha.re), fabs(alpha.im)), xnorm);
         double iw = 1.0 / w;
         double ar = alpha.re * iw, ai = alpha.im * iw, xn = xnorm * iw;
@@ -354,10 +379,9 @@ __device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red)
         const zd tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
         const zd scal = zdiv(mk(1.0, 0.0), mk(alpha.re - beta, alpha.im));
         for (int r = i + 2 + tid; r < n; r += NT) A[lay(r, i)] = zmul(A[lay(r, i)], scal);
-        __syncthreads();
+        __syncthreads();                 // v is scaled; red[] and alpha have been read
         const zd *vcol = A + lay(i + 1, i);   // v(c) = vcol[c], c >= 1, v(0) = 1
-        const int ld = lay.ld;
-        // right: rows r = row (+ NT/4 ...), quad lanes split the columns
+        // right: rows r = row (+ NT/TPR ...), team lanes split the columns
         for (int r = row; r < n; r += NT / TPR) {
             zd *rp = A + r;
             zd u = q == 0 ? rp[(i + 1) * ld] : mk(0.0, 0.0);
@@ -373,7 +397,8 @@ __device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red)
                 rp[(i + 1 + c) * ld] = zsub(rp[(i + 1 + c) * ld], zmul(u, zconj(vcol[c])));
         }
         __syncthreads();
-        // left: columns c = i+1+row (...), quad lanes split the rows
+        // left: columns c = i+1+row (...), team lanes split the rows; the team
+        // of column i+1 also accumulates its tail (rows i+3..) for the next step
         for (int c = i + 1 + row; c < n; c += NT / TPR) {
             zd *col = A + lay(i + 1, c);
             zd sv = q == 0 ? col[0] : mk(0.0, 0.0);
@@ -385,13 +410,31 @@ __device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red)
             }
             sv = zmulc(tau, sv);
             if (q == 0) col[0] = zsub(col[0], sv);
-            for (int r = (q == 0 ? TPR : q); r < m; r += TPR) col[r] = zsub(col[r], zmul(vcol[r], sv));
+            double s = 0.0, mx = 0.0;
+            for (int r = (q == 0 ? TPR : q); r < m; r += TPR) {
+                const zd x = zsub(col[r], zmul(vcol[r], sv));
+                col[r] = x;
+                if (r >= 2) {
+                    s += x.re * x.re + x.im * x.im;
+                    mx = fmax(mx, fmax(fabs(x.re), fabs(x.im)));
+                }
+            }
+            if (c == i + 1) {             // the whole team takes this branch together
+#pragma unroll
+                for (int o = 1; o < TPR; o <<= 1) {
+                    s += __shfl_xor(s, o, TPR);
+                    mx = fmax(mx, __shfl_xor(mx, o, TPR));
+                }
+                if (q == 0) { red[0] = s; red[1] = mx; }
+            }
         }
-        __syncthreads();
         if (tid == 0) A[lay(i + 1, i)] = mk(beta, 0.0);
-        for (int r = i + 2 + tid; r < n; r += NT) A[lay(r, i)] = mk(0.0, 0.0);
         __syncthreads();
     }
+    // the reflector vectors below the sub-diagonal are not kept
+    for (int c = 0; c + 2 < n; ++c)
+        for (int r = c + 2 + tid; r < n; r += NT) A[lay(r, c)] = mk(0.0, 0.0);
+    __syncthreads();
 }
 
 // ---- balancing (LAPACK ZGEBAL, the 3.11+ algorithm) ----
