@@ -181,13 +181,29 @@ def bench_hard(dev, B=100_000, lo=8, hi=64, steps=3, seed=5, sample=1200):
         for b, x in zip(idx, err):
             worst[kinds[b]] = max(worst[kinds[b]], float(x))
     flops = float(sum(40.0 * m.shape[0] ** 3 for m in mats))
+    # ragged fwd (Schur vectors) + bwd (adjoint) on the same batch, cotangents iid complex N(0,1)
+    g = torch.Generator(device=dev).manual_seed(seed + 2)
+    ne = int(rb.eig_offsets[-1])
+    gbar = torch.complex(torch.randn(ne, generator=g, device=dev, dtype=torch.float64),
+                         torch.randn(ne, generator=g, device=dev, dtype=torch.float64))
+    st = {}
+
+    def fwd_schur():
+        st["r"] = rb.schur(A)
+
+    t_schur = timed(fwd_schur, steps, 1)
+    _, schur, fl = st["r"]
+    t_bwd = timed(lambda: rb.backward(schur, gbar, fl), steps, 1)
+    diff = {"fwd_schur_ms": t_schur * 1e3, "bwd_ms": t_bwd * 1e3, "fwd_bwd_solves_per_s": B / (t_schur + t_bwd),
+            "fwd_bwd_over_fwd_eigvals": (t_schur + t_bwd) / t, "flags_nonzero": int((fl != 0).sum())}
+    del schur, st
     return {"workload": f"hard spectra, batch={B}, n~U{{{lo}..{hi}}} seed {seed}: half companions of monic "
                         "polynomials with iid complex N(0,1) coefficients, half V diag(lambda) V^-1 with 3 "
                         "clusters of spread 1e-3 and kappa(V)=1e4",
             "matrices_per_s": B / t, "ms": t * 1e3, "model_tflops": flops / t / 1e12, "flag_histogram": hist,
             "condition_scaled_err_max": worst,
             "condition_scaled_err_def": "|lambda - lambda_zgeevx| * rcond / (eps ||A||_F), unbalanced zgeevx",
-            "accuracy_sample": int(len(pick))}
+            "accuracy_sample": int(len(pick)), "ragged_fwd_bwd": diff}
 
 
 def bench_end_to_end(dev, sizes=(62, 64), B=16_000, chunk=8192, nstreams=8, seed=7):

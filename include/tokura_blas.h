@@ -191,6 +191,22 @@ int tokura_zgeev_batched_backward_gpu(tokurablas_t *handle, int n, int batch_cou
                                       const hipDoubleComplex *gbar, hipDoubleComplex *dA,
                                       char *flag, hipStream_t stream);
 
+/* Ragged differentiation (no counterpart in the reference): the forward
+ * with Schur vectors and the backward for a ragged batch prepared with
+ * tokura_zgeev_ragged_prepare (same `work`, same offsets).  The Schur form of
+ * matrix b is stored at schur + 2*offsets[b] (T then U, n_b^2 each), so
+ * `schur` holds tokura_get_zgeev_ragged_schur_worksize bytes; eigenvalues and
+ * cotangents are at the ragged eigenvalue offsets (prefix sums of n_b), dA at
+ * dA + offsets[b].  Every matrix gets the arithmetic of the same-size entries.
+ * Returns as tokura_zgeev_ragged_gpu (-8: work not prepared). */
+size_t tokura_get_zgeev_ragged_schur_worksize(int batch_count, const int64_t *offsets_host);
+int tokura_zgeev_ragged_schur_gpu(tokurablas_t *handle, int batch_count, const hipDoubleComplex *A,
+                                  hipDoubleComplex *eigenvalues, hipDoubleComplex *schur, void *work,
+                                  char *flag, hipStream_t stream);
+int tokura_zgeev_ragged_backward_gpu(tokurablas_t *handle, int batch_count, const hipDoubleComplex *schur,
+                                     const hipDoubleComplex *gbar, hipDoubleComplex *dA, void *work,
+                                     char *flag, hipStream_t stream);
+
 /* ---- eigenvectors -------------------------------------------------------
  * No counterpart in the reference, which computes eigenvalues only
  * (README.md:41-44); the analogue of LAPACK ZGEEV with JOBVL/JOBVR = 'V'

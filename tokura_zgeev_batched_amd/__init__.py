@@ -63,6 +63,9 @@ _SIGNATURES = {
     "tokura_zgeev_batched_schur_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P], _I),
     "tokura_zgeev_batched_backward_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P, _P], _I),
     "tokura_zgeev_batched_vectors_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P, _P], _I),
+    "tokura_get_zgeev_ragged_schur_worksize": ([_I, _P], _SZ),
+    "tokura_zgeev_ragged_schur_gpu": ([_P, _I, _P, _P, _P, _P, _P, _P], _I),
+    "tokura_zgeev_ragged_backward_gpu": ([_P, _I, _P, _P, _P, _P, _P, _P], _I),
 }
 
 

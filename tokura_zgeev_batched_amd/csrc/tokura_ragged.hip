@@ -29,7 +29,7 @@
 #include <vector>
 
 #include "tokura_internal.hpp"
-#include "zgeev_device.hpp"
+#include "zgeev_schur.hpp"
 
 namespace tkz {
 
@@ -42,7 +42,9 @@ struct RaggedHeader {
     long long total_bytes;
     int nbig;          // matrices of classes A1/A2 (n > 32): order[0 .. nbig)
     int tile_small0;   // first tile of classes B/C
-    long long pad[5];
+    int n16;           // matrices with n > 16: order[0 .. n16)
+    int pad_i;
+    long long pad[4];
 };
 static_assert(sizeof(RaggedHeader) == 128, "header size");
 
@@ -251,6 +253,102 @@ static hipError_t launch_ragged(int cfg, const zd *A, char *work, zd *eig, int8_
     default: return hipErrorInvalidValue;
     }
 }
+// ---- differentiation of a ragged batch ------------------------------------
+// One launch per size class (n > 32, 17..32, <= 16), each with the LDS
+// footprint and lanes-per-matrix of the same-size Schur / adjoint kernels of
+// that class, so every matrix gets the same arithmetic as in a same-size
+// call.  order[] lists the class's matrices contiguously (n descending); the
+// Schur form of matrix b is at 2*moff[b] (T then U), dA at moff[b], the
+// eigenvalues and cotangents at eoff[b].
+__device__ __forceinline__ void class_range(const RaggedView &V, int cls, int &lo, int &hi)
+{
+    lo = cls == 0 ? 0 : (cls == 1 ? V.h->nbig : V.h->n16);
+    hi = cls == 0 ? V.h->nbig : (cls == 1 ? V.h->n16 : V.h->batch);
+}
+
+template <int NB, int LPM, bool ZH, int CLS>
+__global__ __launch_bounds__(64) void k_ragged_schur(char *work, const zd *__restrict__ A, zd *__restrict__ eig,
+                                                     zd *__restrict__ schur, int8_t *__restrict__ flag, int itmax)
+{
+    using G = AdjGeom<NB>;
+    constexpr int GPW = 64 / LPM;
+    __shared__ zd lds[GPW * G::PER];
+    const RaggedView V(work);
+    int lo, hi;
+    class_range(V, CLS, lo, hi);
+    const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
+    zd *H = lds + g * G::PER, *Z = H + G::MAT, *taus = Z + G::MAT;
+    for (int base = lo + (int)blockIdx.x * GPW; base < hi; base += (int)gridDim.x * GPW) {
+        const int j = base + g;
+        if (j < hi) {
+            const int b = V.order[j];
+            schur_one<NB, LPM, ZH>(H, Z, taus, V.nsz[b], t, A + V.moff[b], eig + V.eoff[b], schur + 2 * V.moff[b],
+                                   flag + b, itmax);
+        }
+        lds_order();
+    }
+}
+
+template <int NB, int LPM, int CLS>
+__global__ __launch_bounds__(64) void k_ragged_adjoint(char *work, const zd *__restrict__ schur,
+                                                       const zd *__restrict__ gbar, zd *dA,
+                                                       int8_t *__restrict__ flag, int itmax)
+{
+    using G = AdjGeom<NB>;
+    constexpr int GPW = 64 / LPM;
+    __shared__ zd lds[GPW * G::PER];
+    const RaggedView V(work);
+    int lo, hi;
+    class_range(V, CLS, lo, hi);
+    const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
+    zd *B1 = lds + g * G::PER, *B2 = B1 + G::MAT, *taus = B2 + G::MAT, *gv = taus + NB;
+    for (int base = lo + (int)blockIdx.x * GPW; base < hi; base += (int)gridDim.x * GPW) {
+        const int j = base + g;
+        if (j < hi) {
+            const int b = V.order[j];
+            adjoint_one<NB, LPM, false>(B1, B2, taus, gv, V.nsz[b], t, nullptr, schur + 2 * V.moff[b],
+                                        gbar + V.eoff[b], dA + V.moff[b], flag + b, itmax);
+        }
+        lds_order();
+    }
+}
+
+// resident one-wave workgroups of `k` per CU, queried once per kernel
+template <class K>
+static int resident_blocks(K k)
+{
+    int occ = 0;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, 64, 0);
+    return occ < 1 ? 1 : occ;
+}
+
+static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schur, int8_t *flag, int itmax, int cus,
+                                      hipStream_t s)
+{
+    static const int o0 = resident_blocks(k_ragged_schur<64, 64, false, 0>);
+    static const int o1 = resident_blocks(k_ragged_schur<32, 64, true, 1>);
+    static const int o2 = resident_blocks(k_ragged_schur<16, 32, true, 2>);
+    hipLaunchKernelGGL((k_ragged_schur<64, 64, false, 0>), dim3(cus * o0), dim3(64), 0, s, work, A, eig, schur, flag,
+                       itmax);
+    hipLaunchKernelGGL((k_ragged_schur<32, 64, true, 1>), dim3(cus * o1), dim3(64), 0, s, work, A, eig, schur, flag,
+                       itmax);
+    hipLaunchKernelGGL((k_ragged_schur<16, 32, true, 2>), dim3(cus * o2), dim3(64), 0, s, work, A, eig, schur, flag,
+                       itmax);
+    return hipGetLastError();
+}
+
+static hipError_t launch_ragged_adjoint(const zd *schur, const zd *gbar, zd *dA, char *work, int8_t *flag, int itmax,
+                                        int cus, hipStream_t s)
+{
+    static const int o0 = resident_blocks(k_ragged_adjoint<64, 64, 0>);
+    static const int o1 = resident_blocks(k_ragged_adjoint<32, 32, 1>);
+    static const int o2 = resident_blocks(k_ragged_adjoint<16, 16, 2>);
+    hipLaunchKernelGGL((k_ragged_adjoint<64, 64, 0>), dim3(cus * o0), dim3(64), 0, s, work, schur, gbar, dA, flag, itmax);
+    hipLaunchKernelGGL((k_ragged_adjoint<32, 32, 1>), dim3(cus * o1), dim3(64), 0, s, work, schur, gbar, dA, flag, itmax);
+    hipLaunchKernelGGL((k_ragged_adjoint<16, 16, 2>), dim3(cus * o2), dim3(64), 0, s, work, schur, gbar, dA, flag, itmax);
+    return hipGetLastError();
+}
+
 
 // ---- host-side schedule ---------------------------------------------------
 struct Schedule {
@@ -304,6 +402,8 @@ static int build_schedule(int batch, const int64_t *offsets, int cfg, Schedule &
     std::memset(&S.hdr, 0, sizeof S.hdr);
     S.hdr.nbig = 0;
     while (S.hdr.nbig < batch && S.nsz[S.order[S.hdr.nbig]] > 32) ++S.hdr.nbig;
+    S.hdr.n16 = S.hdr.nbig;
+    while (S.hdr.n16 < batch && S.nsz[S.order[S.hdr.n16]] > 16) ++S.hdr.n16;
     S.hdr.tile_small0 = 0;
     while (S.hdr.tile_small0 < ntiles && S.tiles[4 * S.hdr.tile_small0] < 2) ++S.hdr.tile_small0;
     S.hdr.ntiles = ntiles;
@@ -512,4 +612,51 @@ extern "C" int tokuraAutotuneRagged(tokurablas_t *handle, int batch_count, const
     int rc2 = tokura_zgeev_ragged_prepare(handle, batch_count, offsets_host, work, work_bytes, stream);
     hipStreamSynchronize(stream);
     return rc ? rc : rc2;
+}
+
+extern "C" size_t tokura_get_zgeev_ragged_schur_worksize(int batch_count, const int64_t *offsets_host)
+{
+    if (batch_count <= 0) return (size_t)-2;
+    if (offsets_host == nullptr) return (size_t)-1;
+    long long top = 0;
+    for (int b = 0; b < batch_count; ++b) {
+        const long long cnt = (long long)offsets_host[b + 1] - (long long)offsets_host[b];
+        const long long n = (long long)std::llround(std::sqrt((double)cnt));
+        if (offsets_host[b] < 0 || cnt < 1 || n * n != cnt || n > TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE)
+            return (size_t)-1;
+        top = std::max(top, (long long)offsets_host[b] + cnt);
+    }
+    return sizeof(hipDoubleComplex) * 2 * (size_t)top;
+}
+
+extern "C" int tokura_zgeev_ragged_schur_gpu(tokurablas_t *handle, int batch_count, const hipDoubleComplex *A,
+                                             hipDoubleComplex *eigenvalues, hipDoubleComplex *schur, void *work,
+                                             char *flag, hipStream_t stream)
+{
+    if (handle == nullptr) return -1;
+    if (A == nullptr) return -3;
+    if (eigenvalues == nullptr) return -4;
+    if (work == nullptr || schur == nullptr) return -5;
+    if (batch_count <= 0) return -6;
+    if (flag == nullptr) return -7;
+    if (prepared_cfg(handle, work, batch_count) < 0) return -8;
+    hipError_t e = launch_ragged_schur((const zd *)A, (char *)work, (zd *)eigenvalues, (zd *)schur, (int8_t *)flag,
+                                       handle->max_sweeps, handle->cu_count > 0 ? handle->cu_count : 1, stream);
+    return e == hipSuccess ? 0 : 2;
+}
+
+extern "C" int tokura_zgeev_ragged_backward_gpu(tokurablas_t *handle, int batch_count,
+                                                const hipDoubleComplex *schur, const hipDoubleComplex *gbar,
+                                                hipDoubleComplex *dA, void *work, char *flag, hipStream_t stream)
+{
+    if (handle == nullptr) return -1;
+    if (schur == nullptr) return -3;
+    if (gbar == nullptr) return -4;
+    if (dA == nullptr || work == nullptr) return -5;
+    if (batch_count <= 0) return -6;
+    if (flag == nullptr) return -7;
+    if (prepared_cfg(handle, work, batch_count) < 0) return -8;
+    hipError_t e = launch_ragged_adjoint((const zd *)schur, (const zd *)gbar, (zd *)dA, (char *)work, (int8_t *)flag,
+                                         handle->max_sweeps, handle->cu_count > 0 ? handle->cu_count : 1, stream);
+    return e == hipSuccess ? 0 : 2;
 }

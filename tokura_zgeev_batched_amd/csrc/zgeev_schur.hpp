@@ -1,0 +1,283 @@
+// Per-matrix building blocks of the Schur-form paths (forward with Schur
+// vectors, the adjoint, eigenvectors), shared by the same-size kernels and
+// the ragged ones.
+#pragma once
+#include "zgeev_device.hpp"
+
+namespace tkz {
+
+template <int NB>
+struct AdjGeom {
+    static constexpr int LDP = NB | 1;
+    static constexpr int MAT = NB * LDP;
+    static constexpr int PER = 2 * MAT + 2 * NB;   // buf1, buf2, taus, g
+};
+
+// Coalesced store of an n x n LDS matrix to column-major global memory.
+template <int LPM>
+__device__ __forceinline__ void store_full(const zd *H, const FullLayout lay, zd *__restrict__ dst, int n, int t)
+{
+    const int nn = n * n;
+    int r = t % n, c = t / n;
+    const int dr = LPM % n, dc = LPM / n;
+    for (int e = t; e < nn; e += LPM) {
+        dst[e] = H[lay(r, c)];
+        r += dr;
+        c += dc;
+        if (r >= n) { r -= n; ++c; }
+    }
+}
+
+template <int LPM>
+__device__ __forceinline__ void fill_nan(zd *__restrict__ dst, int count, int t)
+{
+    for (int e = t; e < count; e += LPM) dst[e] = mk(__builtin_nan(""), __builtin_nan(""));
+}
+
+// Schur form of the matrix at src: T into H (upper triangular), U into Z.
+// Returns -1 for a non-finite input, else the QR INFO.
+template <int NB, int LPM, bool ZH = false>
+__device__ __forceinline__ int schur_in_lds(zd *H, zd *Z, zd *taus, const FullLayout lay, const zd *__restrict__ src, int n, int t,
+                            int itmax)
+{
+    constexpr int S = (NB + LPM - 1) / LPM;
+    const bool ok = load_full<LPM>(H, lay, src, n, t);
+    lds_order();
+    if (!ok) return -1;
+    hessenberg<LPM, true>(H, lay, n, t, taus);
+    form_q<LPM>(Z, lay, H, lay, taus, n, t);
+    int info = 0;
+    if (n > 1) info = zlahqr_core<LPM, S, true, ZH>(H, lay, n, t, itmax, Z, lay);
+    for (int c = 0; c < n; ++c)
+        for (int r = c + 1 + t; r < n; r += LPM) H[lay(r, c)] = mk(0.0, 0.0);
+    lds_order();
+    return info;
+}
+
+// Schur form of one matrix (src, n x n column-major) with the buffers H, Z,
+// taus of AdjGeom<NB>: eigenvalues to out[0..n), T then U (n*n each) to T,
+// the flag to *flagp (2: non-finite input, 1: QR did not converge).
+template <int NB, int LPM, bool ZH>
+__device__ __forceinline__ void schur_one(zd *H, zd *Z, zd *taus, int n, int t, const zd *__restrict__ src,
+                                          zd *__restrict__ out, zd *__restrict__ T, int8_t *flagp, int itmax)
+{
+    const FullLayout lay{n | 1};
+    const long long nn = (long long)n * n;
+    zd *U = T + nn;
+    const int info = schur_in_lds<NB, LPM, ZH>(H, Z, taus, lay, src, n, t, itmax);
+    if (info < 0) {
+        fill_nan<LPM>(out, n, t);
+        fill_nan<LPM>(T, (int)(2 * nn), t);
+        if (t == 0) *flagp = 2;
+        return;
+    }
+    for (int r = t; r < n; r += LPM) out[r] = H[lay(r, r)];
+    store_full<LPM>(H, lay, T, n, t);
+    store_full<LPM>(Z, lay, U, n, t);
+    if (t == 0) *flagp = info != 0 ? 1 : 0;
+}
+
+// Left eigenvectors of the upper triangular T held in the upper part of B
+// (ZTREVC, SIDE = 'L', without rescaling): y_i^H (T - t_ii) = 0 with
+// y_i(i) = 1 and y_i(r) = 0 for r < i; y_i(r), r > i, is written to the
+// strictly lower part of B at (r, i).  Near-singular pivots become
+// smin = max(ulp |t_ii|, smlnum) as in ZTREVC.
+template <int LPM, int S>
+__device__ void trevc_left_lower(zd *B, const FullLayout lay, int n, int t)
+{
+    const double ulp = DBL_EPSILON, smlnum = DBL_MIN * ((double)n / ulp);
+    for (int r = 1; r < n; ++r) {
+        const zd trr = B[lay(r, r)];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int i = t + s * LPM;
+            if (i < r) {
+                zd acc = zconj(B[lay(i, r)]);   // c = i, y_i(i) = 1
+                for (int c = i + 1; c < r; ++c) acc = zadd(acc, zmulc(B[lay(c, r)], B[lay(c, i)]));
+                const zd lam = B[lay(i, i)];
+                const double smin = fmax(ulp * cabs1(lam), smlnum);
+                zd dd = zconj(zsub(trr, lam));
+                if (cabs1(dd) < smin) dd = mk(smin, 0.0);
+                zd q = zdiv(acc, dd);
+                B[lay(r, i)] = mk(-q.re, -q.im);
+            }
+        }
+        lds_order();
+    }
+}
+
+// Right eigenvectors of the upper triangular T in the upper part of B
+// (ZTREVC, SIDE = 'R', without rescaling): (T - t_ii) x_i = 0 with
+// x_i(i) = 1 and x_i(r) = 0 for r > i; x_i(r), r < i, is written TRANSPOSED
+// to the strictly lower part of B at (i, r), so T stays intact.
+template <int LPM, int S>
+__device__ void trevc_right_lower_t(zd *B, const FullLayout lay, int n, int t)
+{
+    const double ulp = DBL_EPSILON, smlnum = DBL_MIN * ((double)n / ulp);
+    for (int r = n - 2; r >= 0; --r) {
+        const zd trr = B[lay(r, r)];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int i = t + s * LPM;
+            if (i > r && i < n) {
+                zd acc = mk(0.0, 0.0);
+                for (int c = r + 1; c < i; ++c) acc = zadd(acc, zmul(B[lay(r, c)], B[lay(i, c)]));
+                acc = zadd(acc, B[lay(r, i)]);   // c = i, x_i(i) = 1
+                const zd lam = B[lay(i, i)];
+                const double smin = fmax(ulp * cabs1(lam), smlnum);
+                zd dd = zsub(trr, lam);
+                if (cabs1(dd) < smin) dd = mk(smin, 0.0);
+                zd q = zdiv(acc, dd);
+                B[lay(i, r)] = mk(-q.re, -q.im);
+            }
+        }
+        lds_order();
+    }
+}
+
+// ZGEEV's normalisation of one eigenvector held one component per lane
+// (S == 1): unit Euclidean norm with DZNRM2-style scaling, then the component
+// of largest |re|^2 + |im|^2 (the first on ties, as IDAMAX) is rotated to be
+// real and non-negative with an imaginary part of exactly zero.  Lanes with
+// `live` false hold no component and return their input.
+template <int LPM>
+__device__ __forceinline__ zd zgeev_normalise(zd v, bool live, int r)
+{
+    const double amx = group_max<LPM>(live ? fmax(fabs(v.re), fabs(v.im)) : 0.0);
+    if (amx == 0.0) return v;
+    const double ia = 1.0 / amx;
+    const double sr = v.re * ia, si = v.im * ia;
+    const double nrm = amx * sqrt(group_sum<LPM>(live ? sr * sr + si * si : 0.0));
+    const double scl = 1.0 / nrm;
+    v = zscale(scl, v);
+    double best = live ? v.re * v.re + v.im * v.im : -1.0;
+    int bi = live ? r : 0x7fffffff;
+#pragma unroll
+    for (int o = LPM / 2; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, LPM);
+        const int oi = __shfl_xor(bi, o, LPM);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    const zd vk = mk(__shfl(v.re, bi % LPM, LPM), __shfl(v.im, bi % LPM, LPM));
+    const double inv = 1.0 / sqrt(best);
+    v = zmul(v, mk(vk.re * inv, -vk.im * inv));
+    if (r == bi) v.im = 0.0;
+    return v;
+}
+
+// dL/dA of one matrix into dst (n x n column-major) for cotangents gbar_m,
+// with the buffers B1, B2, taus, gv of AdjGeom<NB>.  RECOMPUTE: the Schur
+// form is rebuilt from Amat and the flag written to *flagp; otherwise it is
+// read from schur_m (T then U) and *flagp is the forward's flag.  A matrix
+// with a non-zero flag gets dA = NaN.
+template <int NB, int LPM, bool RECOMPUTE>
+__device__ __forceinline__ void adjoint_one(zd *B1, zd *B2, zd *taus, zd *gv, int n, int t,
+                                            const zd *__restrict__ Amat, const zd *__restrict__ schur_m,
+                                            const zd *__restrict__ gbar_m, zd *dst, int8_t *flagp, int itmax)
+{
+    constexpr int S = (NB + LPM - 1) / LPM;
+    const FullLayout lay{n | 1};
+    const long long nn = (long long)n * n;
+    const zd *Ug;   // global copy of U
+    int f;
+    if constexpr (RECOMPUTE) {
+        const int info = schur_in_lds<NB, LPM>(B1, B2, taus, lay, Amat, n, t, itmax);
+        f = info < 0 ? 2 : (info != 0 ? 1 : 0);
+        if (t == 0) *flagp = (int8_t)f;
+        if (f == 0) {
+            store_full<LPM>(B2, lay, dst, n, t);   // park U in dA
+            __threadfence();
+        }
+        Ug = dst;
+    } else {
+        f = *flagp;
+        if (f == 0) load_full<LPM>(B1, lay, schur_m, n, t);
+        Ug = schur_m + nn;
+    }
+    if (f != 0) {
+        fill_nan<LPM>(dst, (int)nn, t);
+        return;
+    }
+    for (int i = t; i < n; i += LPM) gv[i] = gbar_m[i];
+    const double ulp = DBL_EPSILON, smlnum = DBL_MIN * ((double)n / ulp);
+    // ---- X (buf2, upper, unit diagonal): (T - t_ii) x_i = 0, rows r < i
+    for (int c = t; c < n; c += LPM)
+        for (int r = 0; r < n; ++r) B2[lay(r, c)] = mk(r == c ? 1.0 : 0.0, 0.0);
+    lds_order();
+    for (int r = n - 2; r >= 0; --r) {
+        const zd trr = B1[lay(r, r)];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int i = t + s * LPM;
+            if (i > r && i < n) {
+                zd acc = mk(0.0, 0.0);
+                for (int c = r + 1; c <= i; ++c) acc = zadd(acc, zmul(B1[lay(r, c)], B2[lay(c, i)]));
+                const zd lam = B1[lay(i, i)];
+                const double smin = fmax(ulp * cabs1(lam), smlnum);
+                zd dd = zsub(trr, lam);
+                if (cabs1(dd) < smin) dd = mk(smin, 0.0);
+                zd q = zdiv(acc, dd);
+                B2[lay(r, i)] = mk(-q.re, -q.im);
+            }
+        }
+        lds_order();
+    }
+    // ---- Y (buf1 strictly lower, y_i(i) = 1): y_i^H (T - t_ii) = 0, rows r > i
+    trevc_left_lower<LPM, S>(B1, lay, n, t);
+    // ---- W = Y diag(g) X^H (lower triangular) into buf1, one row at a time
+    for (int a = 0; a < n; ++a) {
+        zd w[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int b = t + s * LPM;
+            w[s] = mk(0.0, 0.0);
+            if (b <= a) {
+                zd acc = mk(0.0, 0.0);
+                for (int i = b; i <= a; ++i) {
+                    const zd y = (i == a) ? mk(1.0, 0.0) : B1[lay(a, i)];
+                    const zd x = (i == b) ? mk(1.0, 0.0) : B2[lay(b, i)];
+                    acc = zadd(acc, zmul(zmul(y, gv[i]), zconj(x)));
+                }
+                w[s] = acc;
+            }
+        }
+        lds_order();
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int b = t + s * LPM;
+            if (b <= a) B1[lay(a, b)] = w[s];
+        }
+        lds_order();
+    }
+    // ---- P = W U^H into buf2 (X is consumed); U read from global once
+    for (int c = t; c < n; c += LPM)
+        for (int a = 0; a < n; ++a) B2[lay(a, c)] = mk(0.0, 0.0);
+    lds_order();
+    for (int b = 0; b < n; ++b) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int c = t + s * LPM;
+            if (c < n) {
+                const zd u = zconj(Ug[c + (long long)b * n]);
+                for (int a = b; a < n; ++a) B2[lay(a, c)] = zadd(B2[lay(a, c)], zmul(B1[lay(a, b)], u));
+            }
+        }
+    }
+    lds_order();
+    // ---- dA = U P: U into buf1, coalesced, then rows of dA per lane
+    load_full<LPM>(B1, lay, Ug, n, t);
+    lds_order();
+    for (int c = 0; c < n; ++c) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int r = t + s * LPM;
+            if (r < n) {
+                zd acc = mk(0.0, 0.0);
+                for (int a = 0; a < n; ++a) acc = zadd(acc, zmul(B1[lay(r, a)], B2[lay(a, c)]));
+                dst[r + (long long)c * n] = acc;
+            }
+        }
+    }
+}
+
+}  // namespace tkz

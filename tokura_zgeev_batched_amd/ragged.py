@@ -75,6 +75,32 @@ class RaggedBatch:
         _check(rc, "tokura_zgeev_ragged_graph" if graph else "tokura_zgeev_ragged_gpu")
         return eig, flag
 
+    def schur(self, A: torch.Tensor, stream=None):
+        """Forward with Schur vectors: (eig flat (sum n), schur flat (2 * len A),
+        flag (batch,)); matrix b's T then U start at 2 * offsets[b]."""
+        eig, flag = self._bufs(A, None, None)
+        ws = int(lib().tokura_get_zgeev_ragged_schur_worksize(self.batch, self.offsets.ctypes.data_as(_P)))
+        if ws >= (1 << 63):
+            raise TokuraError("tokura_get_zgeev_ragged_schur_worksize rejected the offsets")
+        schur = torch.empty(ws // 16, dtype=torch.complex128, device=A.device)
+        rc = lib().tokura_zgeev_ragged_schur_gpu(self.h.ptr, self.batch, _P(A.data_ptr()), _P(eig.data_ptr()),
+                                                 _P(schur.data_ptr()), _P(self.work.data_ptr()),
+                                                 _P(flag.data_ptr()), _stream_ptr(stream))
+        _check(rc, "tokura_zgeev_ragged_schur_gpu")
+        return eig, schur, flag
+
+    def backward(self, schur: torch.Tensor, gbar: torch.Tensor, flag: torch.Tensor, stream=None):
+        """dL/dA (flat, laid out like A) for flat cotangents gbar (sum n) of
+        L = Re sum conj(gbar) lambda; flag from schur()."""
+        _dev_check(schur, "schur", torch.complex128)
+        gbar = gbar.to(torch.complex128).contiguous()
+        dA = torch.empty(int(self.offsets.max()), dtype=torch.complex128, device=schur.device)
+        rc = lib().tokura_zgeev_ragged_backward_gpu(self.h.ptr, self.batch, _P(schur.data_ptr()), _P(gbar.data_ptr()),
+                                                    _P(dA.data_ptr()), _P(self.work.data_ptr()), _P(flag.data_ptr()),
+                                                    _stream_ptr(stream))
+        _check(rc, "tokura_zgeev_ragged_backward_gpu")
+        return dA
+
     def autotune(self, A: torch.Tensor, stream=None):
         eig, flag = self._bufs(A, None, None)
         rc = lib().tokuraAutotuneRagged(self.h.ptr, self.batch, self.offsets.ctypes.data_as(_P), _P(A.data_ptr()),
