@@ -296,6 +296,28 @@ def bench_vectors(dev, cases=((32, 50_000), (64, 20_000)), steps=3, seed=11):
     return out
 
 
+def bench_small(dev, cases=((16, 10_000), (16, 200_000), (32, 100_000)), steps=5, seed=13):
+    """Plain same-size path at small n (the reference's CPU-runnable case is
+    n = 16 x 10,000), autotuned per n, accuracy vs host LAPACK zgeev."""
+    import oracle
+    out = {"workload": "plain same-size zgeev at small n, eigenvalues only"}
+    h = tk.Handle()
+    for n, B in cases:
+        A = make_batch(B, n, seed, dev)
+        eig = torch.empty((B, n), dtype=torch.complex128, device=dev)
+        flag = torch.empty(B, dtype=torch.int8, device=dev)
+        work = torch.empty(tk.workspace_size(n, B), dtype=torch.uint8, device=dev)
+        h.autotune(n, n, min(B, 8192))
+        t = timed(lambda: tk.zgeev_batched(A, n, B, eig, work, flag, h), steps, 1)
+        k = min(B, 4000)
+        err = oracle.max_relerr(oracle.lapack_zgeev(A[:k].reshape(k, n, n).transpose(-1, -2).cpu().numpy())[0],
+                                eig[:k].cpu().numpy())
+        out[f"n{n}_b{B}"] = {"matrices_per_s": B / t, "ms": t * 1e3, "variant": h.launch_config(n)[0],
+                             "max_rel_err_vs_lapack": err, "flags_nonzero": int((flag != 0).sum())}
+        del A, eig, flag, work
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,7 +329,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the ragged and differentiation legs")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch table")
-    ap.add_argument("--only", choices=["ragged", "diff", "hard", "e2e", "vectors"], default=None,
+    ap.add_argument("--only", choices=["ragged", "diff", "hard", "e2e", "vectors", "small"], default=None,
                     help="run only this extra leg (for profiling); prints its JSON object")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates the "
@@ -322,7 +344,7 @@ def main():
     dev = torch.device("cuda:0")
     if args.only:
         fn = {"ragged": bench_ragged, "diff": bench_diff, "hard": bench_hard, "e2e": bench_end_to_end,
-              "vectors": bench_vectors}[args.only]
+              "vectors": bench_vectors, "small": bench_small}[args.only]
         print(json.dumps({args.only: fn(dev)}), flush=True)
         return
     n, B = args.n, args.batch
@@ -414,7 +436,7 @@ def main():
         torch.cuda.empty_cache()
         result["extra"] = {"ragged": bench_ragged(dev), "differentiation": bench_diff(dev),
                            "hard_spectra": bench_hard(dev), "end_to_end": bench_end_to_end(dev),
-                           "eigenvectors": bench_vectors(dev)}
+                           "eigenvectors": bench_vectors(dev), "small_n": bench_small(dev)}
     print(json.dumps(result), flush=True)
 
 
