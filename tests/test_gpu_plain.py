@@ -173,3 +173,26 @@ def test_stream_and_reuse(handle):
     assert oracle.max_relerr(ref, eig.cpu().numpy()) <= TOL_LAPACK
     # A is not clobbered
     np.testing.assert_array_equal(A.transpose(-1, -2).cpu().numpy(), M)
+
+
+def test_deterministic_across_calls_batches_and_graph():
+    # The same matrix gives the same bits however it is batched: alone, inside
+    # a larger batch, in a later call, and through the ragged path (same
+    # kernel code for n = 64) -- no atomics or reductions depend on scheduling.
+    from tokura_zgeev_batched_amd.ragged import RaggedBatch, pack
+    rng = np.random.default_rng(12)
+    n = 64
+    M = rand(rng, 500, n)
+    h = tk.Handle()
+    e1, _ = gpu_eigs(M, h)
+    e2, _ = gpu_eigs(M, h)
+    np.testing.assert_array_equal(e1, e2)
+    e3, _ = gpu_eigs(M[123:124], h)
+    np.testing.assert_array_equal(e1[123:124], e3)
+    flat, offs = pack(list(M[:50]))
+    rb = RaggedBatch(offs, handle=h)
+    for graph in (False, True, True):
+        eg, fg = rb.solve(torch.tensor(flat, device="cuda"), graph=graph)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(eg.cpu().numpy().reshape(50, n), e1[:50])
+    h.close()
