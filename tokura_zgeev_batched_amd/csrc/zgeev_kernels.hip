@@ -80,6 +80,15 @@ __global__ __launch_bounds__(64) void k_hessenberg_packed(int n, int batch, cons
     if (t == 0) flag[mat] = ok ? 0 : 2;
 }
 
+// Workgroup size and team size (threads sharing a row or column) of the
+// large-matrix Hessenberg kernel.
+#ifndef TKZ_HESS_NT
+#define TKZ_HESS_NT 256
+#endif
+#ifndef TKZ_HESS_TPR
+#define TKZ_HESS_TPR (TKZ_HESS_NT / 64)
+#endif
+
 // Stage 1 of the split path with a whole workgroup of NT threads per matrix
 // (teams of NT/64 threads share each row/column): at NB = 64 two workgroups
 // of 256 threads fit per CU, two waves per SIMD.
@@ -107,7 +116,7 @@ __global__ __launch_bounds__(NT) void k_hessenberg_packed_wg(int n, int batch, c
         if (tid < 64) zgebal_group<64, (NB + 63) / 64>(lds, lay, n, tid, bal);
         __syncthreads();
     }
-    if (ok) hessenberg_wg<NT, NT / 64>(lds, lay, n, red);
+    if (ok) hessenberg_wg<NT, TKZ_HESS_TPR>(lds, lay, n, red);
     zd *dst = work + mat * (long long)packed_size(n);
     const PackedLayout pk;
     for (int c = 0; c < n; ++c) {
@@ -204,9 +213,9 @@ static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *f
     constexpr int GPW = 64 / LPM;
     const int blocks = (batch + GPW - 1) / GPW;
     *stage = 1;
-    if (NB >= 48)   // large matrices: one 256-thread workgroup per matrix (512 threads measured slower)
-        hipLaunchKernelGGL((k_hessenberg_packed_wg<NB, 256>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag,
-                           bal);
+    if (NB >= 48)   // large matrices: one workgroup per matrix
+        hipLaunchKernelGGL((k_hessenberg_packed_wg<NB, TKZ_HESS_NT>), dim3(batch), dim3(TKZ_HESS_NT), 0, s, n, batch,
+                           A, work, flag, bal);
     else
         hipLaunchKernelGGL((k_hessenberg_packed<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, A, work, flag, bal);
     hipError_t e = hipGetLastError();
