@@ -7,7 +7,7 @@
  * implements on the GPU (tokura_zgeev_batched_library/src/tokura_zgeev_batched.cu:212-285:
  * Hessenberg reduction, then shifted QR, eigenvalues only, no balancing,
  * README.md:79).  The arithmetic follows LAPACK's textbook routines, because
- * parity is judged against host LAPACK zgeev:
+ * parity is measured against host LAPACK zgeev:
  *
  *   oracle_zlarfg   <- LAPACK ZLARFG   (reflector with real beta); the
  *                      reference's version is tokura_zgehrd_MWB_kernel.cuh:28-133

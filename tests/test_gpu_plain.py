@@ -1,7 +1,7 @@
 """GPU parity of the same-size path (tokura_zgeev_batched_gpu) through the C ABI.
 
 Bar: max relative error <= 1e-10 against host LAPACK zgeev after optimal
-eigenvalue matching (BASELINE target), and <= 1e-11 against the oracle's
+eigenvalue matching (the accuracy target), and <= 1e-11 against the oracle's
 restatement of the same method (zgehd2 + zlahqr without balancing).
 """
 import numpy as np
