@@ -20,13 +20,22 @@
  *                      deflation, top-left shift and n^4 cap are not reproduced)
  *   oracle_ztrevc   <- LAPACK ZTREVC   (right/left eigenvectors of the Schur form
  *                      by back-substitution), used by the adjoint checks
+ *   oracle_zgeev_vectors  eigenvectors (Schur form, ZTREVC, back-transform,
+ *                      ZGEEV's normalisation); no reference counterpart
+ *                      (README.md:41-44: eigenvalues only)
+ *   oracle_zgebal   <- LAPACK ZGEBAL   (3.11+: permutations, power-of-two
+ *                      scaling), and oracle_zgeev_batched_bal = balancing then
+ *                      the method; optional, the reference never balances
+ *                      (README.md:79)
  *
  * Flag codes per matrix (reference README.md:72 defines only 0 == success):
  *   0 converged, 1 iteration cap reached, 2 non-finite input.
  *
  * Pinned by tests/test_oracle.py against LAPACK zgehrd+zhseqr (the same
- * method, run by OpenBLAS), LAPACK zgeev (balanced), closed forms (triangular,
- * 2x2, companion of z^n - 1, Hermitian), and by Hessenberg invariants.
+ * method, run by OpenBLAS), LAPACK zgeev (balanced, with and without
+ * eigenvectors), LAPACK zgebal (bitwise), closed forms (triangular, 2x2,
+ * companion of z^n - 1, Hermitian, diagonal eigenvectors), and by Hessenberg
+ * invariants.
  */
 #include <complex.h>
 #include <math.h>
