@@ -101,3 +101,46 @@ def test_dual_flags():
     e, f = eigs(rand(rng, 3, 50), h)
     assert (f == 1).all()
     h.close()
+
+
+def test_dual_structured_matrices():
+    rng = np.random.default_rng(42)
+    n = 60
+    mats = []
+    # block diagonal: the active block splits into independent blocks
+    M = np.zeros((n, n), complex)
+    M[:30, :30] = rand(rng, 1, 30)[0]
+    M[30:, 30:] = rand(rng, 1, 30)[0]
+    mats.append(M)
+    # upper triangular: no sweep at all, eigenvalues = diagonal
+    mats.append(np.triu(rand(rng, 1, n)[0]))
+    # zero matrix
+    mats.append(np.zeros((n, n), complex))
+    # companion of z^n - 1: roots of unity (needs the exceptional shifts)
+    C = np.zeros((n, n), complex)
+    C[1:, :-1] = np.eye(n - 1)
+    C[0, -1] = 1.0
+    mats.append(C)
+    # a Jordan-like block (eigenvalue 2 with multiplicity 4) inside a random matrix
+    J = rand(rng, 1, n)[0]
+    J[:4, :] = 0
+    J[:, :4] = 0
+    J[:4, :4] = 2 * np.eye(4) + np.diag(np.ones(3), 1)
+    mats.append(J)
+    M = np.stack(mats)
+    h = dual_handle(n)
+    e, f = eigs(M, h)
+    h.close()
+    assert not f.any()
+    ref, _ = oracle.lapack_zgeev(M[:2])
+    assert oracle.max_relerr(ref, e[:2]) <= TOL_LAPACK
+    np.testing.assert_allclose(np.sort_complex(e[1]), np.sort_complex(np.diag(mats[1])), rtol=1e-13, atol=1e-13)
+    assert np.abs(e[2]).max() == 0.0
+    roots = np.exp(2j * np.pi * np.arange(n) / n)[None]
+    assert oracle.max_relerr(roots, e[3:4]) <= 1e-11
+    # the defective eigenvalue 2 (multiplicity 4) is perturbed by ~eps^(1/4); the rest are regular
+    refJ, _ = oracle.lapack_zgeev(M[4:5])
+    far = np.abs(refJ[0] - 2.0) > 1e-2
+    perm = oracle.match(refJ[0], e[4])
+    assert np.abs(e[4][perm][far] - refJ[0][far]).max() <= 1e-10 * np.abs(refJ[0]).max()
+    assert np.abs(e[4][perm][~far] - 2.0).max() <= 1e-3
