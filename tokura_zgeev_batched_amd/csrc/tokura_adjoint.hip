@@ -26,43 +26,49 @@ namespace tkz {
 
 // ZH: the upper half of each lane group applies the reflectors to Z in the
 // same instructions that update H (n <= LPM/2), doubling the matrices per CU
-// compared with one group of n lanes doing both.
-template <int NB, int LPM, bool ZH>
+// compared with one group of n lanes doing both.  GZ: U is accumulated in
+// the output (the Z buffer lives in global memory), so the LDS holds only H.
+template <int NB, int LPM, bool ZH, bool GZ>
 __global__ __launch_bounds__(64) void k_zgeev_schur(int n, int batch, const zd *__restrict__ A, zd *__restrict__ eig,
                                                     zd *__restrict__ schur, int8_t *__restrict__ flag, int itmax)
 {
     using G = AdjGeom<NB>;
-    constexpr int GPW = 64 / LPM;
-    __shared__ zd lds[GPW * G::PER];
+    constexpr int GPW = 64 / LPM, PER = GZ ? G::PER_GZ : G::PER;
+    __shared__ zd lds[GPW * PER];
     const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
     const long long mat = (long long)blockIdx.x * GPW + g;
     if (mat >= batch) return;
-    zd *H = lds + g * G::PER, *Z = H + G::MAT, *taus = Z + G::MAT;
+    zd *H = lds + g * PER, *Z = GZ ? nullptr : H + G::MAT, *taus = H + (GZ ? 1 : 2) * G::MAT;
     const long long nn = (long long)n * n;
-    schur_one<NB, LPM, ZH>(H, Z, taus, n, t, A + mat * nn, eig + mat * n, schur + mat * 2 * nn, flag + mat, itmax);
+    schur_one<NB, LPM, ZH, GZ>(H, Z, taus, n, t, A + mat * nn, eig + mat * n, schur + mat * 2 * nn, flag + mat, itmax);
 }
 
 // Eigenvalues and right and/or left eigenvectors (LAPACK ZGEEV with
 // JOBVR/JOBVL = 'V' where VR/VL is non-null, no balancing): the Schur form
 // of the forward kernel, ZTREVC on T, back-transform by U, ZGEEV
 // normalisation.  LDS per matrix: T (+ the triangular eigenvector factor in
-// its strictly lower part) and U.
-template <int NB, int LPM, bool ZH>
+// its strictly lower part) and U.  GZ: U is accumulated in the home of VR
+// (VL when only left vectors are asked for) and the LDS holds only T.  The
+// vectors are then formed in place: VL ascending (column i of VL needs U's
+// columns >= i), VR descending (columns <= i), VL first when both are asked
+// for.
+template <int NB, int LPM, bool ZH, bool GZ>
 __global__ __launch_bounds__(64) void k_zgeev_vectors(int n, int batch, const zd *__restrict__ A, zd *__restrict__ eig,
-                                                      zd *__restrict__ VL, zd *__restrict__ VR,
-                                                      int8_t *__restrict__ flag, int itmax)
+                                                      zd *VL, zd *VR, int8_t *__restrict__ flag, int itmax)
 {
     using G = AdjGeom<NB>;
     static_assert(NB <= LPM, "one eigenvector component per lane");
-    constexpr int GPW = 64 / LPM;
-    __shared__ zd lds[GPW * G::PER];
+    constexpr int GPW = 64 / LPM, PER = GZ ? G::PER_GZ : G::PER;
+    __shared__ zd lds[GPW * PER];
     const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
     const long long mat = (long long)blockIdx.x * GPW + g;
     if (mat >= batch) return;
-    zd *B1 = lds + g * G::PER, *U = B1 + G::MAT, *taus = U + G::MAT;
-    const FullLayout lay{n | 1};
     const long long nn = (long long)n * n;
-    const int info = schur_in_lds<NB, LPM, ZH>(B1, U, taus, lay, A + mat * nn, n, t, itmax);
+    zd *B1 = lds + g * PER, *taus = B1 + (GZ ? 1 : 2) * G::MAT;
+    zd *U = GZ ? (VR ? VR : VL) + mat * nn : B1 + G::MAT;
+    const FullLayout lay{n | 1};
+    const FullLayout ul{GZ ? n : lay.ld};
+    const int info = schur_in_lds<NB, LPM, ZH, GZ>(B1, U, taus, lay, A + mat * nn, n, t, itmax);
     zd *out = eig + mat * n;
     if (info < 0) fill_nan<LPM>(out, n, t);
     else
@@ -75,33 +81,45 @@ __global__ __launch_bounds__(64) void k_zgeev_vectors(int n, int batch, const zd
     }
     const bool live = t < n;
     const int r = t;
-    if (VR) {
-        // v_i = U x_i = U(:, i) + sum_{c < i} U(:, c) x_i(c); x_i(c) at B1(i, c)
-        trevc_right_lower_t<LPM, 1>(B1, lay, n, t);
-        zd *dst = VR + mat * nn;
-        for (int i = 0; i < n; ++i) {
-            zd v = mk(0.0, 0.0);
-            if (live) {
-                for (int c = 0; c < i; ++c) v = zadd(v, zmul(U[lay(r, c)], B1[lay(i, c)]));
-                v = zadd(v, U[lay(r, i)]);
-            }
-            v = zgeev_normalise<LPM>(v, live, r);
-            if (live) dst[r + (long long)i * n] = v;
+    // v_i = U x_i = U(:, i) + sum_{c < i} U(:, c) x_i(c); x_i(c) at B1(i, c)
+    auto right = [&](int i) {
+        zd v = mk(0.0, 0.0);
+        if (live) {
+            for (int c = 0; c < i; ++c) v = zadd(v, zmul(U[ul(r, c)], B1[lay(i, c)]));
+            v = zadd(v, U[ul(r, i)]);
         }
-        lds_order();
-    }
-    if (VL) {
-        // u_i = U y_i = U(:, i) + sum_{c > i} U(:, c) y_i(c); y_i(c) at B1(c, i)
-        trevc_left_lower<LPM, 1>(B1, lay, n, t);
-        zd *dst = VL + mat * nn;
-        for (int i = 0; i < n; ++i) {
-            zd v = mk(0.0, 0.0);
-            if (live) {
-                v = U[lay(r, i)];
-                for (int c = i + 1; c < n; ++c) v = zadd(v, zmul(U[lay(r, c)], B1[lay(c, i)]));
-            }
-            v = zgeev_normalise<LPM>(v, live, r);
-            if (live) dst[r + (long long)i * n] = v;
+        v = zgeev_normalise<LPM>(v, live, r);
+        if (live) VR[mat * nn + r + (long long)i * n] = v;
+    };
+    // u_i = U y_i = U(:, i) + sum_{c > i} U(:, c) y_i(c); y_i(c) at B1(c, i)
+    auto left = [&](int i) {
+        zd v = mk(0.0, 0.0);
+        if (live) {
+            v = U[ul(r, i)];
+            for (int c = i + 1; c < n; ++c) v = zadd(v, zmul(U[ul(r, c)], B1[lay(c, i)]));
+        }
+        v = zgeev_normalise<LPM>(v, live, r);
+        if (live) VL[mat * nn + r + (long long)i * n] = v;
+    };
+    if constexpr (GZ) {
+        if (VL) {
+            trevc_left_lower<LPM, 1>(B1, lay, n, t);
+            for (int i = 0; i < n; ++i) left(i);
+            lds_order();
+        }
+        if (VR) {
+            trevc_right_lower_t<LPM, 1>(B1, lay, n, t);
+            for (int i = n - 1; i >= 0; --i) right(i);
+        }
+    } else {
+        if (VR) {
+            trevc_right_lower_t<LPM, 1>(B1, lay, n, t);
+            for (int i = 0; i < n; ++i) right(i);
+            lds_order();
+        }
+        if (VL) {
+            trevc_left_lower<LPM, 1>(B1, lay, n, t);
+            for (int i = 0; i < n; ++i) left(i);
         }
     }
 }
@@ -128,17 +146,18 @@ __global__ __launch_bounds__(64) void k_zgeev_adjoint(int n, int batch, const zd
 
 static int adj_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 
-// Schur forward configurations (NB, lanes per matrix, Z on the upper half-wave)
-#define TKZ_SCHUR_CFG(X) X(16, 32, true) X(32, 64, true) X(64, 64, false)
+// Schur forward configurations (NB, lanes per matrix, Z on the upper
+// half-wave, Z in global memory)
+#define TKZ_SCHUR_CFG(X) X(16, 32, true, false) X(32, 64, true, false) X(64, 64, false, true)
 
 static hipError_t launch_schur(int n, int batch, const zd *A, zd *eig, zd *schur, int8_t *flag, int itmax,
                                hipStream_t s)
 {
     switch (adj_nb(n)) {
-#define X(nb, lpm, zh)                                                                                       \
-    case nb:                                                                                                 \
-        hipLaunchKernelGGL((k_zgeev_schur<nb, lpm, zh>), dim3((batch + 64 / lpm - 1) / (64 / lpm)), dim3(64), 0, \
-                           s, n, batch, A, eig, schur, flag, itmax);                                         \
+#define X(nb, lpm, zh, gz)                                                                                       \
+    case nb:                                                                                                     \
+        hipLaunchKernelGGL((k_zgeev_schur<nb, lpm, zh, gz>), dim3((batch + 64 / lpm - 1) / (64 / lpm)), dim3(64), 0, \
+                           s, n, batch, A, eig, schur, flag, itmax);                                             \
         return hipGetLastError();
         TKZ_SCHUR_CFG(X)
 #undef X
@@ -150,10 +169,10 @@ static hipError_t launch_vectors(int n, int batch, const zd *A, zd *eig, zd *VL,
                                  hipStream_t s)
 {
     switch (adj_nb(n)) {
-#define X(nb, lpm, zh)                                                                                          \
-    case nb:                                                                                                    \
-        hipLaunchKernelGGL((k_zgeev_vectors<nb, lpm, zh>), dim3((batch + 64 / lpm - 1) / (64 / lpm)), dim3(64), 0, \
-                           s, n, batch, A, eig, VL, VR, flag, itmax);                                           \
+#define X(nb, lpm, zh, gz)                                                                                        \
+    case nb:                                                                                                      \
+        hipLaunchKernelGGL((k_zgeev_vectors<nb, lpm, zh, gz>), dim3((batch + 64 / lpm - 1) / (64 / lpm)), dim3(64), 0, \
+                           s, n, batch, A, eig, VL, VR, flag, itmax);                                             \
         return hipGetLastError();
         TKZ_SCHUR_CFG(X)
 #undef X

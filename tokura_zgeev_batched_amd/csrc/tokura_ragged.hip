@@ -266,24 +266,24 @@ __device__ __forceinline__ void class_range(const RaggedView &V, int cls, int &l
     hi = cls == 0 ? V.h->nbig : (cls == 1 ? V.h->n16 : V.h->batch);
 }
 
-template <int NB, int LPM, bool ZH, int CLS>
+template <int NB, int LPM, bool ZH, bool GZ, int CLS>
 __global__ __launch_bounds__(64) void k_ragged_schur(char *work, const zd *__restrict__ A, zd *__restrict__ eig,
                                                      zd *__restrict__ schur, int8_t *__restrict__ flag, int itmax)
 {
     using G = AdjGeom<NB>;
-    constexpr int GPW = 64 / LPM;
-    __shared__ zd lds[GPW * G::PER];
+    constexpr int GPW = 64 / LPM, PER = GZ ? G::PER_GZ : G::PER;
+    __shared__ zd lds[GPW * PER];
     const RaggedView V(work);
     int lo, hi;
     class_range(V, CLS, lo, hi);
     const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
-    zd *H = lds + g * G::PER, *Z = H + G::MAT, *taus = Z + G::MAT;
+    zd *H = lds + g * PER, *Z = GZ ? nullptr : H + G::MAT, *taus = H + (GZ ? 1 : 2) * G::MAT;
     for (int base = lo + (int)blockIdx.x * GPW; base < hi; base += (int)gridDim.x * GPW) {
         const int j = base + g;
         if (j < hi) {
             const int b = V.order[j];
-            schur_one<NB, LPM, ZH>(H, Z, taus, V.nsz[b], t, A + V.moff[b], eig + V.eoff[b], schur + 2 * V.moff[b],
-                                   flag + b, itmax);
+            schur_one<NB, LPM, ZH, GZ>(H, Z, taus, V.nsz[b], t, A + V.moff[b], eig + V.eoff[b],
+                                       schur + 2 * V.moff[b], flag + b, itmax);
         }
         lds_order();
     }
@@ -325,14 +325,14 @@ static int resident_blocks(K k)
 static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schur, int8_t *flag, int itmax, int cus,
                                       hipStream_t s)
 {
-    static const int o0 = resident_blocks(k_ragged_schur<64, 64, false, 0>);
-    static const int o1 = resident_blocks(k_ragged_schur<32, 64, true, 1>);
-    static const int o2 = resident_blocks(k_ragged_schur<16, 32, true, 2>);
-    hipLaunchKernelGGL((k_ragged_schur<64, 64, false, 0>), dim3(cus * o0), dim3(64), 0, s, work, A, eig, schur, flag,
+    static const int o0 = resident_blocks(k_ragged_schur<64, 64, false, true, 0>);
+    static const int o1 = resident_blocks(k_ragged_schur<32, 64, true, false, 1>);
+    static const int o2 = resident_blocks(k_ragged_schur<16, 32, true, false, 2>);
+    hipLaunchKernelGGL((k_ragged_schur<64, 64, false, true, 0>), dim3(cus * o0), dim3(64), 0, s, work, A, eig, schur, flag,
                        itmax);
-    hipLaunchKernelGGL((k_ragged_schur<32, 64, true, 1>), dim3(cus * o1), dim3(64), 0, s, work, A, eig, schur, flag,
+    hipLaunchKernelGGL((k_ragged_schur<32, 64, true, false, 1>), dim3(cus * o1), dim3(64), 0, s, work, A, eig, schur, flag,
                        itmax);
-    hipLaunchKernelGGL((k_ragged_schur<16, 32, true, 2>), dim3(cus * o2), dim3(64), 0, s, work, A, eig, schur, flag,
+    hipLaunchKernelGGL((k_ragged_schur<16, 32, true, false, 2>), dim3(cus * o2), dim3(64), 0, s, work, A, eig, schur, flag,
                        itmax);
     return hipGetLastError();
 }

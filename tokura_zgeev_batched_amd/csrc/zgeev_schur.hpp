@@ -11,6 +11,7 @@ struct AdjGeom {
     static constexpr int LDP = NB | 1;
     static constexpr int MAT = NB * LDP;
     static constexpr int PER = 2 * MAT + 2 * NB;   // buf1, buf2, taus, g
+    static constexpr int PER_GZ = MAT + 2 * NB;    // buf1, taus, g (Z in global memory)
 };
 
 // Coalesced store of an n x n LDS matrix to column-major global memory.
@@ -34,20 +35,74 @@ __device__ __forceinline__ void fill_nan(zd *__restrict__ dst, int count, int t)
     for (int e = t; e < count; e += LPM) dst[e] = mk(__builtin_nan(""), __builtin_nan(""));
 }
 
+// Q = H(0) H(1) ... H(n-2) formed in place of the reflectors in A (LAPACK
+// ZUNG2R's backward accumulation, the arithmetic of form_q): before H(i) is
+// applied, row and column i+1 of Q are set to e_{i+1}.  Q(i+1:n, i+1:n)
+// then covers only columns whose reflectors were applied already, while
+// H(i)'s vector still sits in column i below row i+1.  The Hessenberg part
+// of A is overwritten; the caller saves it first.
+template <int LPM>
+__device__ __forceinline__ void form_q_inplace(zd *A, const FullLayout lay, const zd *taus, int n, int t)
+{
+    for (int i = n - 2; i >= -1; --i) {
+        const int j = i + 1;
+        for (int r = j + t; r < n; r += LPM) {
+            A[lay(r, j)] = mk(r == j ? 1.0 : 0.0, 0.0);
+            if (r != j) A[lay(j, r)] = mk(0.0, 0.0);
+        }
+        lds_order();
+        if (i < 0) break;
+        const zd tau = taus[i];
+        if (tau.re == 0.0 && tau.im == 0.0) continue;
+        const int m = n - 1 - i;
+        for (int c = i + 1 + t; c < n; c += LPM) {
+            zd s = A[lay(i + 1, c)];
+            for (int r = 1; r < m; ++r) s = zadd(s, zmulc(A[lay(i + 1 + r, i)], A[lay(i + 1 + r, c)]));
+            s = zmul(tau, s);
+            A[lay(i + 1, c)] = zsub(A[lay(i + 1, c)], s);
+            for (int r = 1; r < m; ++r) A[lay(i + 1 + r, c)] = zsub(A[lay(i + 1 + r, c)], zmul(A[lay(i + 1 + r, i)], s));
+        }
+        lds_order();
+    }
+}
+
 // Schur form of the matrix at src: T into H (upper triangular), U into Z.
 // Returns -1 for a non-finite input, else the QR INFO.
-template <int NB, int LPM, bool ZH = false>
+// GZ: Z is an n x n column-major global buffer (ld n), so that the LDS holds
+// only H (twice the matrices per CU at n = 64).  The Hessenberg matrix is
+// parked in Z's home while Q is formed in its LDS buffer (form_q_inplace),
+// then the two are swapped.  Every global access here and in the QR's Z
+// updates is by the lane that owns the row (lane p: rows p, p+LPM, ...),
+// coalesced along columns, so no fence is needed; the results are bitwise
+// those of the all-LDS path.
+template <int NB, int LPM, bool ZH = false, bool GZ = false>
 __device__ __forceinline__ int schur_in_lds(zd *H, zd *Z, zd *taus, const FullLayout lay, const zd *__restrict__ src, int n, int t,
-                            int itmax)
+                                            int itmax)
 {
+    static_assert(!(GZ && ZH), "global Z keeps one row per lane");
     constexpr int S = (NB + LPM - 1) / LPM;
+    const FullLayout zl{GZ ? n : lay.ld};
     const bool ok = load_full<LPM>(H, lay, src, n, t);
     lds_order();
     if (!ok) return -1;
     hessenberg<LPM, true>(H, lay, n, t, taus);
-    form_q<LPM>(Z, lay, H, lay, taus, n, t);
+    if constexpr (GZ) {
+        for (int r = t; r < n; r += LPM)
+            for (int c = 0; c < n; ++c) Z[r + c * n] = r > c + 1 ? mk(0.0, 0.0) : H[lay(r, c)];
+        lds_order();
+        form_q_inplace<LPM>(H, lay, taus, n, t);
+        for (int r = t; r < n; r += LPM)
+            for (int c = 0; c < n; ++c) {
+                const zd h = Z[r + c * n];
+                Z[r + c * n] = H[lay(r, c)];
+                H[lay(r, c)] = h;
+            }
+        lds_order();
+    } else {
+        form_q<LPM>(Z, zl, H, lay, taus, n, t);
+    }
     int info = 0;
-    if (n > 1) info = zlahqr_core<LPM, S, true, ZH>(H, lay, n, t, itmax, Z, lay);
+    if (n > 1) info = zlahqr_core<LPM, S, true, ZH>(H, lay, n, t, itmax, Z, zl);
     for (int c = 0; c < n; ++c)
         for (int r = c + 1 + t; r < n; r += LPM) H[lay(r, c)] = mk(0.0, 0.0);
     lds_order();
@@ -56,15 +111,16 @@ __device__ __forceinline__ int schur_in_lds(zd *H, zd *Z, zd *taus, const FullLa
 
 // Schur form of one matrix (src, n x n column-major) with the buffers H, Z,
 // taus of AdjGeom<NB>: eigenvalues to out[0..n), T then U (n*n each) to T,
-// the flag to *flagp (2: non-finite input, 1: QR did not converge).
-template <int NB, int LPM, bool ZH>
+// the flag to *flagp (2: non-finite input, 1: QR did not converge).  GZ: U is
+// accumulated in place in the output (Z unused).
+template <int NB, int LPM, bool ZH, bool GZ = false>
 __device__ __forceinline__ void schur_one(zd *H, zd *Z, zd *taus, int n, int t, const zd *__restrict__ src,
                                           zd *__restrict__ out, zd *__restrict__ T, int8_t *flagp, int itmax)
 {
     const FullLayout lay{n | 1};
     const long long nn = (long long)n * n;
     zd *U = T + nn;
-    const int info = schur_in_lds<NB, LPM, ZH>(H, Z, taus, lay, src, n, t, itmax);
+    const int info = schur_in_lds<NB, LPM, ZH, GZ>(H, GZ ? U : Z, taus, lay, src, n, t, itmax);
     if (info < 0) {
         fill_nan<LPM>(out, n, t);
         fill_nan<LPM>(T, (int)(2 * nn), t);
@@ -73,7 +129,7 @@ __device__ __forceinline__ void schur_one(zd *H, zd *Z, zd *taus, int n, int t, 
     }
     for (int r = t; r < n; r += LPM) out[r] = H[lay(r, r)];
     store_full<LPM>(H, lay, T, n, t);
-    store_full<LPM>(Z, lay, U, n, t);
+    if constexpr (!GZ) store_full<LPM>(Z, lay, U, n, t);   // with GZ the QR updated U in place
     if (t == 0) *flagp = info != 0 ? 1 : 0;
 }
 
