@@ -142,6 +142,22 @@ __global__ __launch_bounds__(64) void k_zgeev_adjoint(int n, int batch, const zd
                                     flag + mat, itmax);
 }
 
+// Stored-Schur backward for n > 32: one matrix per workgroup of
+// TKZ_ADJ_WG_NT threads (the two LDS buffers allow one matrix per CU, so the
+// extra wavefronts share its work instead of idling the CU).
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_zgeev_adjoint_wg(int n, const zd *__restrict__ schur,
+                                                         const zd *__restrict__ gbar, zd *__restrict__ dA,
+                                                         const int8_t *__restrict__ flag)
+{
+    using G = AdjGeom<64>;
+    __shared__ zd lds[G::PER];
+    const long long mat = blockIdx.x, nn = (long long)n * n;
+    zd *B1 = lds, *B2 = lds + G::MAT, *gv = B2 + G::MAT;
+    adjoint_wg<NT>(B1, B2, gv, n, schur + mat * 2 * nn, gbar + mat * n, dA + mat * nn, flag[mat]);
+}
+
 #define TKZ_ADJ_NB(X) X(16, 16) X(32, 32) X(64, 64)
 
 static int adj_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
@@ -183,6 +199,11 @@ static hipError_t launch_vectors(int n, int batch, const zd *A, zd *eig, zd *VL,
 static hipError_t launch_adjoint(int n, int batch, const zd *A, const zd *schur, const zd *gbar, zd *dA,
                                  int8_t *flag, int itmax, hipStream_t s)
 {
+    if (schur && adj_nb(n) == 64) {
+        hipLaunchKernelGGL((k_zgeev_adjoint_wg<TKZ_ADJ_WG_NT>), dim3(batch), dim3(TKZ_ADJ_WG_NT), 0, s, n, schur,
+                           gbar, dA, flag);
+        return hipGetLastError();
+    }
     switch (adj_nb(n)) {
 #define X(nb, lpm)                                                                                               \
     case nb:                                                                                                     \

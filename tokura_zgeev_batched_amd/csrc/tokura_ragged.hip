@@ -313,12 +313,32 @@ __global__ __launch_bounds__(64) void k_ragged_adjoint(char *work, const zd *__r
     }
 }
 
-// resident one-wave workgroups of `k` per CU, queried once per kernel
+// class 0 (n > 32) of the backward: one matrix at a time per workgroup of
+// TKZ_ADJ_WG_NT threads (adjoint_wg)
+template <int NT>
+__global__ __launch_bounds__(NT) void k_ragged_adjoint_wg(char *work, const zd *__restrict__ schur,
+                                                          const zd *__restrict__ gbar, zd *dA,
+                                                          const int8_t *__restrict__ flag)
+{
+    using G = AdjGeom<64>;
+    __shared__ zd lds[G::PER];
+    const RaggedView V(work);
+    int lo, hi;
+    class_range(V, 0, lo, hi);
+    zd *B1 = lds, *B2 = lds + G::MAT, *gv = B2 + G::MAT;
+    for (int j = lo + (int)blockIdx.x; j < hi; j += (int)gridDim.x) {
+        const int b = V.order[j];
+        adjoint_wg<NT>(B1, B2, gv, V.nsz[b], schur + 2 * V.moff[b], gbar + V.eoff[b], dA + V.moff[b], flag[b]);
+        __syncthreads();
+    }
+}
+
+// resident workgroups of `k` (block of `threads`) per CU, queried once per kernel
 template <class K>
-static int resident_blocks(K k)
+static int resident_blocks(K k, int threads = 64)
 {
     int occ = 0;
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, 64, 0);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, threads, 0);
     return occ < 1 ? 1 : occ;
 }
 
@@ -340,10 +360,11 @@ static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schu
 static hipError_t launch_ragged_adjoint(const zd *schur, const zd *gbar, zd *dA, char *work, int8_t *flag, int itmax,
                                         int cus, hipStream_t s)
 {
-    static const int o0 = resident_blocks(k_ragged_adjoint<64, 64, 0>);
+    static const int o0 = resident_blocks(k_ragged_adjoint_wg<TKZ_ADJ_WG_NT>, TKZ_ADJ_WG_NT);
     static const int o1 = resident_blocks(k_ragged_adjoint<32, 32, 1>);
     static const int o2 = resident_blocks(k_ragged_adjoint<16, 16, 2>);
-    hipLaunchKernelGGL((k_ragged_adjoint<64, 64, 0>), dim3(cus * o0), dim3(64), 0, s, work, schur, gbar, dA, flag, itmax);
+    hipLaunchKernelGGL((k_ragged_adjoint_wg<TKZ_ADJ_WG_NT>), dim3(cus * o0), dim3(TKZ_ADJ_WG_NT), 0, s, work, schur,
+                       gbar, dA, flag);
     hipLaunchKernelGGL((k_ragged_adjoint<32, 32, 1>), dim3(cus * o1), dim3(64), 0, s, work, schur, gbar, dA, flag, itmax);
     hipLaunchKernelGGL((k_ragged_adjoint<16, 16, 2>), dim3(cus * o2), dim3(64), 0, s, work, schur, gbar, dA, flag, itmax);
     return hipGetLastError();

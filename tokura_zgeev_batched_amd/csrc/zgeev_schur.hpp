@@ -4,6 +4,11 @@
 #pragma once
 #include "zgeev_device.hpp"
 
+// threads per workgroup of the stored-Schur backward for n > 32 (adjoint_wg)
+#ifndef TKZ_ADJ_WG_NT
+#define TKZ_ADJ_WG_NT 256
+#endif
+
 namespace tkz {
 
 template <int NB>
@@ -221,6 +226,37 @@ __device__ __forceinline__ zd zgeev_normalise(zd v, bool live, int r)
     return v;
 }
 
+// X (unit upper triangular, into B2) of the right eigenvectors of the upper
+// triangular T in B1: (T - t_ii) x_i = 0 for rows r < i (ZTREVC without
+// rescaling; near-equal eigenvalues perturbed to ulp*|t_ii|).  Lane i owns
+// column i; only B1's upper part and diagonal are read.
+template <int LPM, int S>
+__device__ __forceinline__ void trevc_right_upper(const zd *B1, zd *B2, const FullLayout lay, int n, int t)
+{
+    const double ulp = DBL_EPSILON, smlnum = DBL_MIN * ((double)n / ulp);
+    for (int c = t; c < n; c += LPM)
+        for (int r = 0; r < n; ++r) B2[lay(r, c)] = mk(r == c ? 1.0 : 0.0, 0.0);
+    lds_order();
+    for (int r = n - 2; r >= 0; --r) {
+        const zd trr = B1[lay(r, r)];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int i = t + s * LPM;
+            if (i > r && i < n) {
+                zd acc = mk(0.0, 0.0);
+                for (int c = r + 1; c <= i; ++c) acc = zadd(acc, zmul(B1[lay(r, c)], B2[lay(c, i)]));
+                const zd lam = B1[lay(i, i)];
+                const double smin = fmax(ulp * cabs1(lam), smlnum);
+                zd dd = zsub(trr, lam);
+                if (cabs1(dd) < smin) dd = mk(smin, 0.0);
+                zd q = zdiv(acc, dd);
+                B2[lay(r, i)] = mk(-q.re, -q.im);
+            }
+        }
+        lds_order();
+    }
+}
+
 // dL/dA of one matrix into dst (n x n column-major) for cotangents gbar_m,
 // with the buffers B1, B2, taus, gv of AdjGeom<NB>.  RECOMPUTE: the Schur
 // form is rebuilt from Amat and the flag written to *flagp; otherwise it is
@@ -255,29 +291,7 @@ __device__ __forceinline__ void adjoint_one(zd *B1, zd *B2, zd *taus, zd *gv, in
         return;
     }
     for (int i = t; i < n; i += LPM) gv[i] = gbar_m[i];
-    const double ulp = DBL_EPSILON, smlnum = DBL_MIN * ((double)n / ulp);
-    // ---- X (buf2, upper, unit diagonal): (T - t_ii) x_i = 0, rows r < i
-    for (int c = t; c < n; c += LPM)
-        for (int r = 0; r < n; ++r) B2[lay(r, c)] = mk(r == c ? 1.0 : 0.0, 0.0);
-    lds_order();
-    for (int r = n - 2; r >= 0; --r) {
-        const zd trr = B1[lay(r, r)];
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            const int i = t + s * LPM;
-            if (i > r && i < n) {
-                zd acc = mk(0.0, 0.0);
-                for (int c = r + 1; c <= i; ++c) acc = zadd(acc, zmul(B1[lay(r, c)], B2[lay(c, i)]));
-                const zd lam = B1[lay(i, i)];
-                const double smin = fmax(ulp * cabs1(lam), smlnum);
-                zd dd = zsub(trr, lam);
-                if (cabs1(dd) < smin) dd = mk(smin, 0.0);
-                zd q = zdiv(acc, dd);
-                B2[lay(r, i)] = mk(-q.re, -q.im);
-            }
-        }
-        lds_order();
-    }
+    trevc_right_upper<LPM, S>(B1, B2, lay, n, t);
     // ---- Y (buf1 strictly lower, y_i(i) = 1): y_i^H (T - t_ii) = 0, rows r > i
     trevc_left_lower<LPM, S>(B1, lay, n, t);
     // ---- W = Y diag(g) X^H (lower triangular) into buf1, one row at a time
@@ -332,6 +346,73 @@ __device__ __forceinline__ void adjoint_one(zd *B1, zd *B2, zd *taus, zd *gv, in
                 for (int a = 0; a < n; ++a) acc = zadd(acc, zmul(B1[lay(r, a)], B2[lay(a, c)]));
                 dst[r + (long long)c * n] = acc;
             }
+        }
+    }
+}
+
+// dL/dA of one matrix (n <= 64) from its stored Schur form by a workgroup of
+// NT threads (NT/64 wavefronts): adjoint_one's quantities with the same
+// per-element arithmetic, so bitwise its results, spread over the
+// wavefronts.  X (wavefront 0) and Y (wavefront 1) are solved side by side;
+// then wavefront w forms rows a = w (mod NT/64) of W, rows of P = W U^H over
+// W's rows (a row of P needs only its own row of W, so no other wavefront
+// reads what is overwritten), and columns c = w (mod NT/64) of dA = U P.
+// Buffers (AdjGeom<64>): B1 = T/Y -> W -> P, B2 = X -> U, gv = the cotangent.
+// `f` is the matrix's forward flag (uniform over the workgroup).
+template <int NT>
+__device__ __forceinline__ void adjoint_wg(zd *B1, zd *B2, zd *gv, int n, const zd *__restrict__ schur_m,
+                                           const zd *__restrict__ gbar_m, zd *__restrict__ dst, int f)
+{
+    static_assert(NT % 64 == 0 && NT >= 128, "X and Y need one wavefront each");
+    constexpr int NW = NT / 64;
+    const int tid = (int)threadIdx.x, w = tid >> 6, t = tid & 63;
+    const FullLayout lay{n | 1};
+    const int nn = n * n;
+    if (f != 0) {
+        fill_nan<NT>(dst, nn, tid);
+        return;
+    }
+    for (int e = tid; e < nn; e += NT) B1[lay(e % n, e / n)] = schur_m[e];
+    for (int i = tid; i < n; i += NT) gv[i] = gbar_m[i];
+    __syncthreads();
+    if (w == 0) trevc_right_upper<64, 1>(B1, B2, lay, n, t);
+    else if (w == 1) trevc_left_lower<64, 1>(B1, lay, n, t);
+    __syncthreads();
+    // W = Y diag(g) X^H (lower triangular), row a: lanes b <= a
+    for (int a = w; a < n; a += NW) {
+        const int b = t;
+        zd acc = mk(0.0, 0.0);
+        if (b <= a) {
+            for (int i = b; i <= a; ++i) {
+                const zd y = (i == a) ? mk(1.0, 0.0) : B1[lay(a, i)];
+                const zd x = (i == b) ? mk(1.0, 0.0) : B2[lay(b, i)];
+                acc = zadd(acc, zmul(zmul(y, gv[i]), zconj(x)));
+            }
+        }
+        lds_order();
+        if (b <= a) B1[lay(a, b)] = acc;
+    }
+    __syncthreads();
+    const zd *Ug = schur_m + nn;
+    for (int e = tid; e < nn; e += NT) B2[lay(e % n, e / n)] = Ug[e];
+    __syncthreads();
+    // P = W U^H, row a over W's row a: lane c
+    for (int a = w; a < n; a += NW) {
+        const int c = t;
+        zd acc = mk(0.0, 0.0);
+        if (c < n)
+            for (int b = 0; b <= a; ++b) acc = zadd(acc, zmul(B1[lay(a, b)], zconj(B2[lay(c, b)])));
+        lds_order();
+        if (c < n) B1[lay(a, c)] = acc;
+    }
+    __syncthreads();
+    // dA = U P, column c: lane r
+    for (int c = w; c < n; c += NW) {
+        const int r = t;
+        if (r < n) {
+            zd acc = mk(0.0, 0.0);
+            for (int a = 0; a < n; ++a) acc = zadd(acc, zmul(B2[lay(r, a)], B1[lay(a, c)]));
+            dst[r + c * n] = acc;
         }
     }
 }
