@@ -3,9 +3,10 @@
 // the same Schur form.  No counterpart in the reference, which has no
 // gradient path (README.md:41-44: "No eigenvectors are computed").
 //
-// Forward (k_zgeev_schur): A = U T U^H by the same method as the eigenvalue
-// path, in Schur-form mode: Hessenberg with kept reflectors, Q accumulation
-// (ZUNGHR), QR with WANTT/WANTZ (ZLAHQR), T and U stored per matrix.
+// Forward (k_zgeev_schur; for n > 32 k_schur_hq + k_schur_qr): A = U T U^H
+// by the same method as the eigenvalue path, in Schur-form mode: Hessenberg
+// with kept reflectors, Q accumulation (ZUNGHR), QR with WANTT/WANTZ
+// (ZLAHQR), T and U stored per matrix.
 //
 // Backward (k_zgeev_adjoint): for L = Re sum_i conj(g_i) lambda_i,
 //   dL/dA = sum_i g_i Y_i X_i^H / conj(Y_i^H X_i)   (torch's convention)
@@ -79,49 +80,67 @@ __global__ __launch_bounds__(64) void k_zgeev_vectors(int n, int batch, const zd
         if (VL) fill_nan<LPM>(VL + mat * nn, (int)nn, t);
         return;
     }
-    const bool live = t < n;
-    const int r = t;
-    // v_i = U x_i = U(:, i) + sum_{c < i} U(:, c) x_i(c); x_i(c) at B1(i, c)
-    auto right = [&](int i) {
-        zd v = mk(0.0, 0.0);
-        if (live) {
-            for (int c = 0; c < i; ++c) v = zadd(v, zmul(U[ul(r, c)], B1[lay(i, c)]));
-            v = zadd(v, U[ul(r, i)]);
-        }
-        v = zgeev_normalise<LPM>(v, live, r);
-        if (live) VR[mat * nn + r + (long long)i * n] = v;
-    };
-    // u_i = U y_i = U(:, i) + sum_{c > i} U(:, c) y_i(c); y_i(c) at B1(c, i)
-    auto left = [&](int i) {
-        zd v = mk(0.0, 0.0);
-        if (live) {
-            v = U[ul(r, i)];
-            for (int c = i + 1; c < n; ++c) v = zadd(v, zmul(U[ul(r, c)], B1[lay(c, i)]));
-        }
-        v = zgeev_normalise<LPM>(v, live, r);
-        if (live) VL[mat * nn + r + (long long)i * n] = v;
-    };
-    if constexpr (GZ) {
-        if (VL) {
-            trevc_left_lower<LPM, 1>(B1, lay, n, t);
-            for (int i = 0; i < n; ++i) left(i);
-            lds_order();
-        }
-        if (VR) {
-            trevc_right_lower_t<LPM, 1>(B1, lay, n, t);
-            for (int i = n - 1; i >= 0; --i) right(i);
-        }
+    vectors_from_schur<LPM, GZ>(B1, lay, U, ul, n, t, VL ? VL + mat * nn : nullptr, VR ? VR + mat * nn : nullptr);
+}
+
+// Schur form for n > 32 in two launches (schur_hq_one, schur_qr_one).  The
+// matrix's T and U slots are Tout + mat*tstride and Uout + mat*ustride: the
+// Schur-form output (T then U per matrix) or, for eigenvectors, VL's and VR's
+// slots.  First half: one wavefront per matrix, full LDS layout; writes flag
+// 0, or 2 with NaN outputs for a non-finite input.
+__global__ __launch_bounds__(64) void k_schur_hq(int n, const zd *__restrict__ A, zd *__restrict__ eig, zd *Tout,
+                                                 long long tstride, zd *Uout, long long ustride,
+                                                 int8_t *__restrict__ flag)
+{
+    using G = AdjGeom<64>;
+    __shared__ zd lds[G::MAT + 64];
+    const int t = (int)threadIdx.x;
+    const long long mat = blockIdx.x, nn = (long long)n * n;
+    zd *Tm = Tout + mat * tstride, *Um = Uout + mat * ustride;
+    if (schur_hq_one<64>(lds, lds + G::MAT, A + mat * nn, n, t, Tm, Um)) {
+        if (t == 0) flag[mat] = 0;
     } else {
-        if (VR) {
-            trevc_right_lower_t<LPM, 1>(B1, lay, n, t);
-            for (int i = 0; i < n; ++i) right(i);
-            lds_order();
-        }
-        if (VL) {
-            trevc_left_lower<LPM, 1>(B1, lay, n, t);
-            for (int i = 0; i < n; ++i) left(i);
-        }
+        fill_nan<64>(eig + mat * n, n, t);
+        fill_nan<64>(Tm, (int)nn, t);
+        fill_nan<64>(Um, (int)nn, t);
+        if (t == 0) flag[mat] = 2;
     }
+}
+
+// Second half: the QR on the packed layout, four matrices per CU at n = 64;
+// eigenvalues, T, U in place, flag 0/1 (matrices flagged 2 are skipped).
+__global__ __launch_bounds__(64) void k_schur_qr(int n, zd *__restrict__ eig, zd *Tout, long long tstride, zd *Uout,
+                                                 long long ustride, int8_t *flag, int itmax)
+{
+    __shared__ zd lds[packed_size(64)];
+    const int t = (int)threadIdx.x;
+    const long long mat = blockIdx.x;
+    if (flag[mat] == 2) return;
+    const int info = schur_qr_one(lds, n, t, Tout + mat * tstride, Uout + mat * ustride, eig + mat * n, itmax);
+    if (t == 0) flag[mat] = info != 0 ? 1 : 0;
+}
+
+// Eigenvectors (both sides, n > 32) after k_schur_hq / k_schur_qr left T in
+// VL's slot and U in VR's: T into LDS (each lane its own rows, so the later
+// writes of VL by the same lanes need no fence), then vectors_from_schur in
+// place.  Flagged matrices get NaN vectors.
+__global__ __launch_bounds__(64) void k_vectors_from_schur(int n, zd *VL, zd *VR, const int8_t *__restrict__ flag)
+{
+    using G = AdjGeom<64>;
+    __shared__ zd lds[G::MAT];
+    const int t = (int)threadIdx.x;
+    const long long mat = blockIdx.x, nn = (long long)n * n;
+    zd *VLm = VL + mat * nn, *VRm = VR + mat * nn;
+    if (flag[mat] != 0) {
+        fill_nan<64>(VLm, (int)nn, t);
+        fill_nan<64>(VRm, (int)nn, t);
+        return;
+    }
+    const FullLayout lay{n | 1};
+    for (int r = t; r < n; r += 64)
+        for (int c = 0; c < n; ++c) lds[lay(r, c)] = VLm[r + c * n];
+    lds_order();
+    vectors_from_schur<64, true>(lds, lay, VRm, FullLayout{n}, n, t, VLm, VRm);
 }
 
 template <int NB, int LPM, bool RECOMPUTE>
@@ -163,20 +182,33 @@ __global__ __launch_bounds__(NT) void k_zgeev_adjoint_wg(int n, const zd *__rest
 static int adj_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 
 // Schur forward configurations (NB, lanes per matrix, Z on the upper
-// half-wave, Z in global memory)
+// half-wave, Z in global memory); n > 32 goes through k_schur_hq/k_schur_qr
+// except for one-sided eigenvector requests (no second slot for T).
 #define TKZ_SCHUR_CFG(X) X(16, 32, true, false) X(32, 64, true, false) X(64, 64, false, true)
+
+static void launch_schur_split(int n, int batch, const zd *A, zd *eig, zd *Tout, long long tstride, zd *Uout,
+                               long long ustride, int8_t *flag, int itmax, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_schur_hq, dim3(batch), dim3(64), 0, s, n, A, eig, Tout, tstride, Uout, ustride, flag);
+    hipLaunchKernelGGL(k_schur_qr, dim3(batch), dim3(64), 0, s, n, eig, Tout, tstride, Uout, ustride, flag, itmax);
+}
 
 static hipError_t launch_schur(int n, int batch, const zd *A, zd *eig, zd *schur, int8_t *flag, int itmax,
                                hipStream_t s)
 {
+    const long long nn = (long long)n * n;
     switch (adj_nb(n)) {
-#define X(nb, lpm, zh, gz)                                                                                       \
-    case nb:                                                                                                     \
-        hipLaunchKernelGGL((k_zgeev_schur<nb, lpm, zh, gz>), dim3((batch + 64 / lpm - 1) / (64 / lpm)), dim3(64), 0, \
-                           s, n, batch, A, eig, schur, flag, itmax);                                             \
+    case 16:
+        hipLaunchKernelGGL((k_zgeev_schur<16, 32, true, false>), dim3((batch + 1) / 2), dim3(64), 0, s, n, batch, A,
+                           eig, schur, flag, itmax);
         return hipGetLastError();
-        TKZ_SCHUR_CFG(X)
-#undef X
+    case 32:
+        hipLaunchKernelGGL((k_zgeev_schur<32, 64, true, false>), dim3(batch), dim3(64), 0, s, n, batch, A, eig,
+                           schur, flag, itmax);
+        return hipGetLastError();
+    case 64:
+        launch_schur_split(n, batch, A, eig, schur, 2 * nn, schur + nn, 2 * nn, flag, itmax, s);
+        return hipGetLastError();
     }
     return hipErrorInvalidValue;
 }
@@ -184,6 +216,12 @@ static hipError_t launch_schur(int n, int batch, const zd *A, zd *eig, zd *schur
 static hipError_t launch_vectors(int n, int batch, const zd *A, zd *eig, zd *VL, zd *VR, int8_t *flag, int itmax,
                                  hipStream_t s)
 {
+    const long long nn = (long long)n * n;
+    if (adj_nb(n) == 64 && VL && VR) {
+        launch_schur_split(n, batch, A, eig, VL, nn, VR, nn, flag, itmax, s);
+        hipLaunchKernelGGL(k_vectors_from_schur, dim3(batch), dim3(64), 0, s, n, VL, VR, flag);
+        return hipGetLastError();
+    }
     switch (adj_nb(n)) {
 #define X(nb, lpm, zh, gz)                                                                                        \
     case nb:                                                                                                      \

@@ -289,6 +289,51 @@ __global__ __launch_bounds__(64) void k_ragged_schur(char *work, const zd *__res
     }
 }
 
+// class 0 (n > 32) of the Schur forward in two launches, as the same-size
+// path: Hessenberg + Q (full LDS layout), then the QR on the packed layout
+// (four matrices per CU) with U in the output
+__global__ __launch_bounds__(64) void k_ragged_schur_hq(char *work, const zd *__restrict__ A, zd *__restrict__ eig,
+                                                        zd *schur, int8_t *__restrict__ flag)
+{
+    using G = AdjGeom<64>;
+    __shared__ zd lds[G::MAT + 64];
+    const RaggedView V(work);
+    int lo, hi;
+    class_range(V, 0, lo, hi);
+    const int t = (int)threadIdx.x;
+    for (int j = lo + (int)blockIdx.x; j < hi; j += (int)gridDim.x) {
+        const int b = V.order[j], n = V.nsz[b];
+        const long long nn = (long long)n * n;
+        zd *Tm = schur + 2 * V.moff[b], *Um = Tm + nn;
+        if (schur_hq_one<64>(lds, lds + G::MAT, A + V.moff[b], n, t, Tm, Um)) {
+            if (t == 0) flag[b] = 0;
+        } else {
+            fill_nan<64>(eig + V.eoff[b], n, t);
+            fill_nan<64>(Tm, (int)(2 * nn), t);
+            if (t == 0) flag[b] = 2;
+        }
+        lds_order();
+    }
+}
+
+__global__ __launch_bounds__(64) void k_ragged_schur_qr(char *work, zd *__restrict__ eig, zd *schur, int8_t *flag,
+                                                        int itmax)
+{
+    __shared__ zd lds[packed_size(64)];
+    const RaggedView V(work);
+    int lo, hi;
+    class_range(V, 0, lo, hi);
+    const int t = (int)threadIdx.x;
+    for (int j = lo + (int)blockIdx.x; j < hi; j += (int)gridDim.x) {
+        const int b = V.order[j], n = V.nsz[b];
+        if (flag[b] == 2) continue;
+        zd *Tm = schur + 2 * V.moff[b];
+        const int info = schur_qr_one(lds, n, t, Tm, Tm + (long long)n * n, eig + V.eoff[b], itmax);
+        if (t == 0) flag[b] = info != 0 ? 1 : 0;
+        lds_order();
+    }
+}
+
 template <int NB, int LPM, int CLS>
 __global__ __launch_bounds__(64) void k_ragged_adjoint(char *work, const zd *__restrict__ schur,
                                                        const zd *__restrict__ gbar, zd *dA,
@@ -345,11 +390,12 @@ static int resident_blocks(K k, int threads = 64)
 static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schur, int8_t *flag, int itmax, int cus,
                                       hipStream_t s)
 {
-    static const int o0 = resident_blocks(k_ragged_schur<64, 64, false, true, 0>);
+    static const int oh = resident_blocks(k_ragged_schur_hq);
+    static const int oq = resident_blocks(k_ragged_schur_qr);
     static const int o1 = resident_blocks(k_ragged_schur<32, 64, true, false, 1>);
     static const int o2 = resident_blocks(k_ragged_schur<16, 32, true, false, 2>);
-    hipLaunchKernelGGL((k_ragged_schur<64, 64, false, true, 0>), dim3(cus * o0), dim3(64), 0, s, work, A, eig, schur, flag,
-                       itmax);
+    hipLaunchKernelGGL(k_ragged_schur_hq, dim3(cus * oh), dim3(64), 0, s, work, A, eig, schur, flag);
+    hipLaunchKernelGGL(k_ragged_schur_qr, dim3(cus * oq), dim3(64), 0, s, work, eig, schur, flag, itmax);
     hipLaunchKernelGGL((k_ragged_schur<32, 64, true, false, 1>), dim3(cus * o1), dim3(64), 0, s, work, A, eig, schur, flag,
                        itmax);
     hipLaunchKernelGGL((k_ragged_schur<16, 32, true, false, 2>), dim3(cus * o2), dim3(64), 0, s, work, A, eig, schur, flag,

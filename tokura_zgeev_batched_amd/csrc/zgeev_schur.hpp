@@ -138,6 +138,56 @@ __device__ __forceinline__ void schur_one(zd *H, zd *Z, zd *taus, int n, int t, 
     if (t == 0) *flagp = info != 0 ? 1 : 0;
 }
 
+// The Schur form for n > 32 in two kernels, so that the QR, which dominates,
+// runs on the packed Hessenberg layout (34 KB of LDS at n = 64, four
+// matrices per CU) with U in global memory.  Same arithmetic as
+// schur_in_lds<NB, 64, false, true>, hence the same bits.
+//
+// First half (full LDS layout, one wavefront per matrix): Hessenberg with
+// kept reflectors; the Hessenberg matrix (zeros below the subdiagonal) to Th
+// and Q (ZUNGHR, formed in place of the reflectors) to Ug, both n x n
+// column-major in global memory and written row by row by the owning lane.
+// Returns false for a non-finite input (nothing written).
+template <int LPM>
+__device__ __forceinline__ bool schur_hq_one(zd *H, zd *taus, const zd *__restrict__ src, int n, int t, zd *Th,
+                                             zd *Ug)
+{
+    const FullLayout lay{n | 1};
+    const bool ok = load_full<LPM>(H, lay, src, n, t);
+    lds_order();
+    if (!ok) return false;
+    hessenberg<LPM, true>(H, lay, n, t, taus);
+    for (int r = t; r < n; r += LPM)
+        for (int c = 0; c < n; ++c) Th[r + c * n] = r > c + 1 ? mk(0.0, 0.0) : H[lay(r, c)];
+    lds_order();
+    form_q_inplace<LPM>(H, lay, taus, n, t);
+    for (int r = t; r < n; r += LPM)
+        for (int c = 0; c < n; ++c) Ug[r + c * n] = H[lay(r, c)];
+    lds_order();
+    return true;
+}
+
+// Second half (packed LDS layout, packed_size(n) complex): ZLAHQR with
+// WANTT/WANTZ on the Hessenberg matrix from Th, Z = Ug updated in place (lane
+// p owns row p, as it wrote it), then the eigenvalues to out and T (zeros
+// below the diagonal) back to Th, row by row by the lane that read it.
+// Returns the QR INFO.
+__device__ __forceinline__ int schur_qr_one(zd *H, int n, int t, zd *Th, zd *Ug, zd *__restrict__ out, int itmax)
+{
+    const PackedLayout lay{};
+    for (int r = t; r < n; r += 64)
+        for (int c = r > 0 ? r - 1 : 0; c < n; ++c) H[lay(r, c)] = Th[r + c * n];
+    lds_order();
+    int info = 0;
+    if (n > 1) info = zlahqr_core<64, 1, true, false>(H, lay, n, t, itmax, Ug, FullLayout{n});
+    lds_order();
+    for (int r = t; r < n; r += 64) {
+        out[r] = H[lay(r, r)];
+        for (int c = 0; c < n; ++c) Th[r + c * n] = r > c ? mk(0.0, 0.0) : H[lay(r, c)];
+    }
+    return info;
+}
+
 // Left eigenvectors of the upper triangular T held in the upper part of B
 // (ZTREVC, SIDE = 'L', without rescaling): y_i^H (T - t_ii) = 0 with
 // y_i(i) = 1 and y_i(r) = 0 for r < i; y_i(r), r > i, is written to the
@@ -224,6 +274,62 @@ __device__ __forceinline__ zd zgeev_normalise(zd v, bool live, int r)
     v = zmul(v, mk(vk.re * inv, -vk.im * inv));
     if (r == bi) v.im = 0.0;
     return v;
+}
+
+// Eigenvectors from a Schur form held as T in LDS (B1, full layout `lay`) and
+// U (layout `ul`, in LDS or global memory): ZTREVC, back-transform by U,
+// ZGEEV normalisation, into the matrix's VL and/or VR (n x n column-major;
+// null = not asked for).  INPLACE: U lives in VR's home (VL's when VR is
+// null), so the columns are formed in an order that reads each column of U
+// before overwriting it: VL ascending (column i needs U's columns >= i), VR
+// descending (columns <= i), VL first.
+template <int LPM, bool INPLACE>
+__device__ __forceinline__ void vectors_from_schur(zd *B1, const FullLayout lay, const zd *U, const FullLayout ul,
+                                                   int n, int t, zd *VLm, zd *VRm)
+{
+    const bool live = t < n;
+    const int r = t;
+    // v_i = U x_i = U(:, i) + sum_{c < i} U(:, c) x_i(c); x_i(c) at B1(i, c)
+    auto right = [&](int i) {
+        zd v = mk(0.0, 0.0);
+        if (live) {
+            for (int c = 0; c < i; ++c) v = zadd(v, zmul(U[ul(r, c)], B1[lay(i, c)]));
+            v = zadd(v, U[ul(r, i)]);
+        }
+        v = zgeev_normalise<LPM>(v, live, r);
+        if (live) VRm[r + (long long)i * n] = v;
+    };
+    // u_i = U y_i = U(:, i) + sum_{c > i} U(:, c) y_i(c); y_i(c) at B1(c, i)
+    auto left = [&](int i) {
+        zd v = mk(0.0, 0.0);
+        if (live) {
+            v = U[ul(r, i)];
+            for (int c = i + 1; c < n; ++c) v = zadd(v, zmul(U[ul(r, c)], B1[lay(c, i)]));
+        }
+        v = zgeev_normalise<LPM>(v, live, r);
+        if (live) VLm[r + (long long)i * n] = v;
+    };
+    if constexpr (INPLACE) {
+        if (VLm) {
+            trevc_left_lower<LPM, 1>(B1, lay, n, t);
+            for (int i = 0; i < n; ++i) left(i);
+            lds_order();
+        }
+        if (VRm) {
+            trevc_right_lower_t<LPM, 1>(B1, lay, n, t);
+            for (int i = n - 1; i >= 0; --i) right(i);
+        }
+    } else {
+        if (VRm) {
+            trevc_right_lower_t<LPM, 1>(B1, lay, n, t);
+            for (int i = 0; i < n; ++i) right(i);
+            lds_order();
+        }
+        if (VLm) {
+            trevc_left_lower<LPM, 1>(B1, lay, n, t);
+            for (int i = 0; i < n; ++i) left(i);
+        }
+    }
 }
 
 // X (unit upper triangular, into B2) of the right eigenvectors of the upper
