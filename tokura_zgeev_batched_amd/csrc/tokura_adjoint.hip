@@ -3,10 +3,10 @@
 // the same Schur form.  No counterpart in the reference, which has no
 // gradient path (README.md:41-44: "No eigenvectors are computed").
 //
-// Forward (k_zgeev_schur; for n > 32 k_schur_hq + k_schur_qr): A = U T U^H
-// by the same method as the eigenvalue path, in Schur-form mode: Hessenberg
-// with kept reflectors, Q accumulation (ZUNGHR), QR with WANTT/WANTZ
-// (ZLAHQR), T and U stored per matrix.
+// Forward (k_schur_hq + k_schur_qr): A = U T U^H by the same method as the
+// eigenvalue path, in Schur-form mode: Hessenberg with kept reflectors, Q
+// accumulation (ZUNGHR), QR with WANTT/WANTZ (ZLAHQR), T and U stored per
+// matrix.
 //
 // Backward (k_zgeev_adjoint): for L = Re sum_i conj(g_i) lambda_i,
 //   dL/dA = sum_i g_i Y_i X_i^H / conj(Y_i^H X_i)   (torch's convention)
@@ -24,25 +24,6 @@
 #include "zgeev_schur.hpp"
 
 namespace tkz {
-
-// ZH: the upper half of each lane group applies the reflectors to Z in the
-// same instructions that update H (n <= LPM/2), doubling the matrices per CU
-// compared with one group of n lanes doing both.  GZ: U is accumulated in
-// the output (the Z buffer lives in global memory), so the LDS holds only H.
-template <int NB, int LPM, bool ZH, bool GZ>
-__global__ __launch_bounds__(64) void k_zgeev_schur(int n, int batch, const zd *__restrict__ A, zd *__restrict__ eig,
-                                                    zd *__restrict__ schur, int8_t *__restrict__ flag, int itmax)
-{
-    using G = AdjGeom<NB>;
-    constexpr int GPW = 64 / LPM, PER = GZ ? G::PER_GZ : G::PER;
-    __shared__ zd lds[GPW * PER];
-    const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
-    const long long mat = (long long)blockIdx.x * GPW + g;
-    if (mat >= batch) return;
-    zd *H = lds + g * PER, *Z = GZ ? nullptr : H + G::MAT, *taus = H + (GZ ? 1 : 2) * G::MAT;
-    const long long nn = (long long)n * n;
-    schur_one<NB, LPM, ZH, GZ>(H, Z, taus, n, t, A + mat * nn, eig + mat * n, schur + mat * 2 * nn, flag + mat, itmax);
-}
 
 // Eigenvalues and right and/or left eigenvectors (LAPACK ZGEEV with
 // JOBVR/JOBVL = 'V' where VR/VL is non-null, no balancing): the Schur form
@@ -83,64 +64,74 @@ __global__ __launch_bounds__(64) void k_zgeev_vectors(int n, int batch, const zd
     vectors_from_schur<LPM, GZ>(B1, lay, U, ul, n, t, VL ? VL + mat * nn : nullptr, VR ? VR + mat * nn : nullptr);
 }
 
-// Schur form for n > 32 in two launches (schur_hq_one, schur_qr_one).  The
-// matrix's T and U slots are Tout + mat*tstride and Uout + mat*ustride: the
-// Schur-form output (T then U per matrix) or, for eigenvectors, VL's and VR's
-// slots.  First half: one wavefront per matrix, full LDS layout; writes flag
-// 0, or 2 with NaN outputs for a non-finite input.
-__global__ __launch_bounds__(64) void k_schur_hq(int n, const zd *__restrict__ A, zd *__restrict__ eig, zd *Tout,
-                                                 long long tstride, zd *Uout, long long ustride,
+// Schur form in two launches (schur_hq_one, schur_qr_one), LPM lanes per
+// matrix.  The matrix's T and U slots are Tout + mat*tstride and
+// Uout + mat*ustride: the Schur-form output (T then U per matrix) or, for
+// eigenvectors, VL's and VR's slots.  First half: full LDS layout; writes
+// flag 0, or 2 with NaN outputs for a non-finite input.
+template <int NB, int LPM>
+__global__ __launch_bounds__(64) void k_schur_hq(int n, int batch, const zd *__restrict__ A, zd *__restrict__ eig,
+                                                 zd *Tout, long long tstride, zd *Uout, long long ustride,
                                                  int8_t *__restrict__ flag)
 {
-    using G = AdjGeom<64>;
-    __shared__ zd lds[G::MAT + 64];
-    const int t = (int)threadIdx.x;
-    const long long mat = blockIdx.x, nn = (long long)n * n;
-    zd *Tm = Tout + mat * tstride, *Um = Uout + mat * ustride;
-    if (schur_hq_one<64>(lds, lds + G::MAT, A + mat * nn, n, t, Tm, Um)) {
+    using G = AdjGeom<NB>;
+    constexpr int GPW = 64 / LPM, PER = G::MAT + NB;
+    __shared__ zd lds[GPW * PER];
+    const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
+    const long long mat = (long long)blockIdx.x * GPW + g, nn = (long long)n * n;
+    if (mat >= batch) return;
+    zd *Tm = Tout + mat * tstride, *Um = Uout + mat * ustride, *H = lds + g * PER;
+    if (schur_hq_one<LPM>(H, H + G::MAT, A + mat * nn, n, t, Tm, Um)) {
         if (t == 0) flag[mat] = 0;
     } else {
-        fill_nan<64>(eig + mat * n, n, t);
-        fill_nan<64>(Tm, (int)nn, t);
-        fill_nan<64>(Um, (int)nn, t);
+        fill_nan<LPM>(eig + mat * n, n, t);
+        fill_nan<LPM>(Tm, (int)nn, t);
+        fill_nan<LPM>(Um, (int)nn, t);
         if (t == 0) flag[mat] = 2;
     }
 }
 
-// Second half: the QR on the packed layout, four matrices per CU at n = 64;
+// Second half: the QR on the packed layout (four matrices per CU at n = 64);
 // eigenvalues, T, U in place, flag 0/1 (matrices flagged 2 are skipped).
-__global__ __launch_bounds__(64) void k_schur_qr(int n, zd *__restrict__ eig, zd *Tout, long long tstride, zd *Uout,
-                                                 long long ustride, int8_t *flag, int itmax)
+template <int NB, int LPM>
+__global__ __launch_bounds__(64) void k_schur_qr(int n, int batch, zd *__restrict__ eig, zd *Tout, long long tstride,
+                                                 zd *Uout, long long ustride, int8_t *flag, int itmax)
 {
-    __shared__ zd lds[packed_size(64)];
-    const int t = (int)threadIdx.x;
-    const long long mat = blockIdx.x;
-    if (flag[mat] == 2) return;
-    const int info = schur_qr_one(lds, n, t, Tout + mat * tstride, Uout + mat * ustride, eig + mat * n, itmax);
+    constexpr int GPW = 64 / LPM, PER = packed_size(NB);
+    __shared__ zd lds[GPW * PER];
+    const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
+    const long long mat = (long long)blockIdx.x * GPW + g;
+    if (mat >= batch || flag[mat] == 2) return;
+    const int info = schur_qr_one<LPM>(lds + g * PER, n, t, Tout + mat * tstride, Uout + mat * ustride,
+                                       eig + mat * n, itmax);
     if (t == 0) flag[mat] = info != 0 ? 1 : 0;
 }
 
-// Eigenvectors (both sides, n > 32) after k_schur_hq / k_schur_qr left T in
-// VL's slot and U in VR's: T into LDS (each lane its own rows, so the later
-// writes of VL by the same lanes need no fence), then vectors_from_schur in
-// place.  Flagged matrices get NaN vectors.
-__global__ __launch_bounds__(64) void k_vectors_from_schur(int n, zd *VL, zd *VR, const int8_t *__restrict__ flag)
+// Eigenvectors (both sides) after k_schur_hq / k_schur_qr left T in VL's slot
+// and U in VR's: T into LDS (each lane its own rows, so the later writes of
+// VL by the same lanes need no fence), then vectors_from_schur in place.
+// Flagged matrices get NaN vectors.
+template <int NB, int LPM>
+__global__ __launch_bounds__(64) void k_vectors_from_schur(int n, int batch, zd *VL, zd *VR,
+                                                           const int8_t *__restrict__ flag)
 {
-    using G = AdjGeom<64>;
-    __shared__ zd lds[G::MAT];
-    const int t = (int)threadIdx.x;
-    const long long mat = blockIdx.x, nn = (long long)n * n;
-    zd *VLm = VL + mat * nn, *VRm = VR + mat * nn;
+    using G = AdjGeom<NB>;
+    constexpr int GPW = 64 / LPM;
+    __shared__ zd lds[GPW * G::MAT];
+    const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
+    const long long mat = (long long)blockIdx.x * GPW + g, nn = (long long)n * n;
+    if (mat >= batch) return;
+    zd *VLm = VL + mat * nn, *VRm = VR + mat * nn, *B1 = lds + g * G::MAT;
     if (flag[mat] != 0) {
-        fill_nan<64>(VLm, (int)nn, t);
-        fill_nan<64>(VRm, (int)nn, t);
+        fill_nan<LPM>(VLm, (int)nn, t);
+        fill_nan<LPM>(VRm, (int)nn, t);
         return;
     }
     const FullLayout lay{n | 1};
-    for (int r = t; r < n; r += 64)
-        for (int c = 0; c < n; ++c) lds[lay(r, c)] = VLm[r + c * n];
+    for (int r = t; r < n; r += LPM)
+        for (int c = 0; c < n; ++c) B1[lay(r, c)] = VLm[r + c * n];
     lds_order();
-    vectors_from_schur<64, true>(lds, lay, VRm, FullLayout{n}, n, t, VLm, VRm);
+    vectors_from_schur<LPM, true>(B1, lay, VRm, FullLayout{n}, n, t, VLm, VRm);
 }
 
 template <int NB, int LPM, bool RECOMPUTE>
@@ -181,47 +172,47 @@ __global__ __launch_bounds__(NT) void k_zgeev_adjoint_wg(int n, const zd *__rest
 
 static int adj_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 
-// Schur forward configurations (NB, lanes per matrix, Z on the upper
-// half-wave, Z in global memory); n > 32 goes through k_schur_hq/k_schur_qr
-// except for one-sided eigenvector requests (no second slot for T).
+// One-kernel Schur configurations (NB, lanes per matrix, Z on the upper
+// half-wave, Z in global memory), used by one-sided eigenvector requests
+// (there is no second slot to park T in); everything else goes through
+// k_schur_hq / k_schur_qr with NB lanes per matrix.
 #define TKZ_SCHUR_CFG(X) X(16, 32, true, false) X(32, 64, true, false) X(64, 64, false, true)
+#define TKZ_SPLIT_NB(X) X(16) X(32) X(64)
 
-static void launch_schur_split(int n, int batch, const zd *A, zd *eig, zd *Tout, long long tstride, zd *Uout,
-                               long long ustride, int8_t *flag, int itmax, hipStream_t s)
+static hipError_t launch_schur_split(int n, int batch, const zd *A, zd *eig, zd *Tout, long long tstride,
+                                     zd *Uout, long long ustride, int8_t *flag, int itmax, hipStream_t s,
+                                     zd *VLv = nullptr, zd *VRv = nullptr)
 {
-    hipLaunchKernelGGL(k_schur_hq, dim3(batch), dim3(64), 0, s, n, A, eig, Tout, tstride, Uout, ustride, flag);
-    hipLaunchKernelGGL(k_schur_qr, dim3(batch), dim3(64), 0, s, n, eig, Tout, tstride, Uout, ustride, flag, itmax);
+    switch (adj_nb(n)) {
+#define X(nb)                                                                                                   \
+    case nb: {                                                                                                  \
+        const dim3 grid((batch + 64 / nb - 1) / (64 / nb));                                                     \
+        hipLaunchKernelGGL((k_schur_hq<nb, nb>), grid, dim3(64), 0, s, n, batch, A, eig, Tout, tstride, Uout,    \
+                           ustride, flag);                                                                      \
+        hipLaunchKernelGGL((k_schur_qr<nb, nb>), grid, dim3(64), 0, s, n, batch, eig, Tout, tstride, Uout,       \
+                           ustride, flag, itmax);                                                               \
+        if (VLv)                                                                                                \
+            hipLaunchKernelGGL((k_vectors_from_schur<nb, nb>), grid, dim3(64), 0, s, n, batch, VLv, VRv, flag);  \
+        return hipGetLastError();                                                                               \
+    }
+        TKZ_SPLIT_NB(X)
+#undef X
+    }
+    return hipErrorInvalidValue;
 }
 
 static hipError_t launch_schur(int n, int batch, const zd *A, zd *eig, zd *schur, int8_t *flag, int itmax,
                                hipStream_t s)
 {
     const long long nn = (long long)n * n;
-    switch (adj_nb(n)) {
-    case 16:
-        hipLaunchKernelGGL((k_zgeev_schur<16, 32, true, false>), dim3((batch + 1) / 2), dim3(64), 0, s, n, batch, A,
-                           eig, schur, flag, itmax);
-        return hipGetLastError();
-    case 32:
-        hipLaunchKernelGGL((k_zgeev_schur<32, 64, true, false>), dim3(batch), dim3(64), 0, s, n, batch, A, eig,
-                           schur, flag, itmax);
-        return hipGetLastError();
-    case 64:
-        launch_schur_split(n, batch, A, eig, schur, 2 * nn, schur + nn, 2 * nn, flag, itmax, s);
-        return hipGetLastError();
-    }
-    return hipErrorInvalidValue;
+    return launch_schur_split(n, batch, A, eig, schur, 2 * nn, schur + nn, 2 * nn, flag, itmax, s);
 }
 
 static hipError_t launch_vectors(int n, int batch, const zd *A, zd *eig, zd *VL, zd *VR, int8_t *flag, int itmax,
                                  hipStream_t s)
 {
     const long long nn = (long long)n * n;
-    if (adj_nb(n) == 64 && VL && VR) {
-        launch_schur_split(n, batch, A, eig, VL, nn, VR, nn, flag, itmax, s);
-        hipLaunchKernelGGL(k_vectors_from_schur, dim3(batch), dim3(64), 0, s, n, VL, VR, flag);
-        return hipGetLastError();
-    }
+    if (VL && VR) return launch_schur_split(n, batch, A, eig, VL, nn, VR, nn, flag, itmax, s, VL, VR);
     switch (adj_nb(n)) {
 #define X(nb, lpm, zh, gz)                                                                                        \
     case nb:                                                                                                      \

@@ -31,10 +31,11 @@ namespace {
 // Default variant per n before any tuning (ids of zgeev_kernels.hip's table).
 int default_variant(int n)
 {
+    // what tokuraAutotune picks on an MI355X at batches of 20,000 and 50,000
     if (n <= 4) return 0;    // fused, 4 lanes/matrix, 16 matrices/wave
     if (n <= 8) return 1;    // fused 8 x 4
-    if (n <= 16) return 4;   // fused 16 x 8
-    if (n <= 32) return 8;   // fused 32 x 32
+    if (n <= 16) return 5;   // fused 16 x 16
+    if (n <= 32) return 14;  // split 32 x 32
     if (n <= 48) return 19;  // split 48 x 64, two bulges per sweep
     return 20;               // split 64 x 64, two bulges per sweep
 }

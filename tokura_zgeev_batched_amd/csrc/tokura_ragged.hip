@@ -266,70 +266,60 @@ __device__ __forceinline__ void class_range(const RaggedView &V, int cls, int &l
     hi = cls == 0 ? V.h->nbig : (cls == 1 ? V.h->n16 : V.h->batch);
 }
 
-template <int NB, int LPM, bool ZH, bool GZ, int CLS>
-__global__ __launch_bounds__(64) void k_ragged_schur(char *work, const zd *__restrict__ A, zd *__restrict__ eig,
-                                                     zd *__restrict__ schur, int8_t *__restrict__ flag, int itmax)
+// The Schur forward of size class CLS in two launches, as the same-size path:
+// Hessenberg + Q (full LDS layout), then the QR on the packed layout with U
+// in the output; LPM lanes per matrix, 64/LPM matrices per wavefront.
+template <int NB, int LPM, int CLS>
+__global__ __launch_bounds__(64) void k_ragged_schur_hq(char *work, const zd *__restrict__ A, zd *__restrict__ eig,
+                                                        zd *schur, int8_t *__restrict__ flag)
 {
     using G = AdjGeom<NB>;
-    constexpr int GPW = 64 / LPM, PER = GZ ? G::PER_GZ : G::PER;
+    constexpr int GPW = 64 / LPM, PER = G::MAT + NB;
     __shared__ zd lds[GPW * PER];
     const RaggedView V(work);
     int lo, hi;
     class_range(V, CLS, lo, hi);
-    const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
-    zd *H = lds + g * PER, *Z = GZ ? nullptr : H + G::MAT, *taus = H + (GZ ? 1 : 2) * G::MAT;
+    const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
+    zd *H = lds + g * PER;
     for (int base = lo + (int)blockIdx.x * GPW; base < hi; base += (int)gridDim.x * GPW) {
         const int j = base + g;
         if (j < hi) {
-            const int b = V.order[j];
-            schur_one<NB, LPM, ZH, GZ>(H, Z, taus, V.nsz[b], t, A + V.moff[b], eig + V.eoff[b],
-                                       schur + 2 * V.moff[b], flag + b, itmax);
+            const int b = V.order[j], n = V.nsz[b];
+            const long long nn = (long long)n * n;
+            zd *Tm = schur + 2 * V.moff[b], *Um = Tm + nn;
+            if (schur_hq_one<LPM>(H, H + G::MAT, A + V.moff[b], n, t, Tm, Um)) {
+                if (t == 0) flag[b] = 0;
+            } else {
+                fill_nan<LPM>(eig + V.eoff[b], n, t);
+                fill_nan<LPM>(Tm, (int)(2 * nn), t);
+                if (t == 0) flag[b] = 2;
+            }
         }
         lds_order();
     }
 }
 
-// class 0 (n > 32) of the Schur forward in two launches, as the same-size
-// path: Hessenberg + Q (full LDS layout), then the QR on the packed layout
-// (four matrices per CU) with U in the output
-__global__ __launch_bounds__(64) void k_ragged_schur_hq(char *work, const zd *__restrict__ A, zd *__restrict__ eig,
-                                                        zd *schur, int8_t *__restrict__ flag)
-{
-    using G = AdjGeom<64>;
-    __shared__ zd lds[G::MAT + 64];
-    const RaggedView V(work);
-    int lo, hi;
-    class_range(V, 0, lo, hi);
-    const int t = (int)threadIdx.x;
-    for (int j = lo + (int)blockIdx.x; j < hi; j += (int)gridDim.x) {
-        const int b = V.order[j], n = V.nsz[b];
-        const long long nn = (long long)n * n;
-        zd *Tm = schur + 2 * V.moff[b], *Um = Tm + nn;
-        if (schur_hq_one<64>(lds, lds + G::MAT, A + V.moff[b], n, t, Tm, Um)) {
-            if (t == 0) flag[b] = 0;
-        } else {
-            fill_nan<64>(eig + V.eoff[b], n, t);
-            fill_nan<64>(Tm, (int)(2 * nn), t);
-            if (t == 0) flag[b] = 2;
-        }
-        lds_order();
-    }
-}
-
+template <int NB, int LPM, int CLS>
 __global__ __launch_bounds__(64) void k_ragged_schur_qr(char *work, zd *__restrict__ eig, zd *schur, int8_t *flag,
                                                         int itmax)
 {
-    __shared__ zd lds[packed_size(64)];
+    constexpr int GPW = 64 / LPM, PER = packed_size(NB);
+    __shared__ zd lds[GPW * PER];
     const RaggedView V(work);
     int lo, hi;
-    class_range(V, 0, lo, hi);
-    const int t = (int)threadIdx.x;
-    for (int j = lo + (int)blockIdx.x; j < hi; j += (int)gridDim.x) {
-        const int b = V.order[j], n = V.nsz[b];
-        if (flag[b] == 2) continue;
-        zd *Tm = schur + 2 * V.moff[b];
-        const int info = schur_qr_one(lds, n, t, Tm, Tm + (long long)n * n, eig + V.eoff[b], itmax);
-        if (t == 0) flag[b] = info != 0 ? 1 : 0;
+    class_range(V, CLS, lo, hi);
+    const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
+    for (int base = lo + (int)blockIdx.x * GPW; base < hi; base += (int)gridDim.x * GPW) {
+        const int j = base + g;
+        if (j < hi) {
+            const int b = V.order[j], n = V.nsz[b];
+            if (flag[b] != 2) {
+                zd *Tm = schur + 2 * V.moff[b];
+                const int info = schur_qr_one<LPM>(lds + g * PER, n, t, Tm, Tm + (long long)n * n,
+                                                   eig + V.eoff[b], itmax);
+                if (t == 0) flag[b] = info != 0 ? 1 : 0;
+            }
+        }
         lds_order();
     }
 }
@@ -390,16 +380,19 @@ static int resident_blocks(K k, int threads = 64)
 static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schur, int8_t *flag, int itmax, int cus,
                                       hipStream_t s)
 {
-    static const int oh = resident_blocks(k_ragged_schur_hq);
-    static const int oq = resident_blocks(k_ragged_schur_qr);
-    static const int o1 = resident_blocks(k_ragged_schur<32, 64, true, false, 1>);
-    static const int o2 = resident_blocks(k_ragged_schur<16, 32, true, false, 2>);
-    hipLaunchKernelGGL(k_ragged_schur_hq, dim3(cus * oh), dim3(64), 0, s, work, A, eig, schur, flag);
-    hipLaunchKernelGGL(k_ragged_schur_qr, dim3(cus * oq), dim3(64), 0, s, work, eig, schur, flag, itmax);
-    hipLaunchKernelGGL((k_ragged_schur<32, 64, true, false, 1>), dim3(cus * o1), dim3(64), 0, s, work, A, eig, schur, flag,
-                       itmax);
-    hipLaunchKernelGGL((k_ragged_schur<16, 32, true, false, 2>), dim3(cus * o2), dim3(64), 0, s, work, A, eig, schur, flag,
-                       itmax);
+#define TKZ_RS(nb, cls)                                                                                       \
+    {                                                                                                          \
+        static const int oh = resident_blocks(k_ragged_schur_hq<nb, nb, cls>);                                 \
+        static const int oq = resident_blocks(k_ragged_schur_qr<nb, nb, cls>);                                 \
+        hipLaunchKernelGGL((k_ragged_schur_hq<nb, nb, cls>), dim3(cus * oh), dim3(64), 0, s, work, A, eig, schur, \
+                           flag);                                                                              \
+        hipLaunchKernelGGL((k_ragged_schur_qr<nb, nb, cls>), dim3(cus * oq), dim3(64), 0, s, work, eig, schur,  \
+                           flag, itmax);                                                                       \
+    }
+    TKZ_RS(64, 0)
+    TKZ_RS(32, 1)
+    TKZ_RS(16, 2)
+#undef TKZ_RS
     return hipGetLastError();
 }
 

@@ -114,38 +114,14 @@ __device__ __forceinline__ int schur_in_lds(zd *H, zd *Z, zd *taus, const FullLa
     return info;
 }
 
-// Schur form of one matrix (src, n x n column-major) with the buffers H, Z,
-// taus of AdjGeom<NB>: eigenvalues to out[0..n), T then U (n*n each) to T,
-// the flag to *flagp (2: non-finite input, 1: QR did not converge).  GZ: U is
-// accumulated in place in the output (Z unused).
-template <int NB, int LPM, bool ZH, bool GZ = false>
-__device__ __forceinline__ void schur_one(zd *H, zd *Z, zd *taus, int n, int t, const zd *__restrict__ src,
-                                          zd *__restrict__ out, zd *__restrict__ T, int8_t *flagp, int itmax)
-{
-    const FullLayout lay{n | 1};
-    const long long nn = (long long)n * n;
-    zd *U = T + nn;
-    const int info = schur_in_lds<NB, LPM, ZH, GZ>(H, GZ ? U : Z, taus, lay, src, n, t, itmax);
-    if (info < 0) {
-        fill_nan<LPM>(out, n, t);
-        fill_nan<LPM>(T, (int)(2 * nn), t);
-        if (t == 0) *flagp = 2;
-        return;
-    }
-    for (int r = t; r < n; r += LPM) out[r] = H[lay(r, r)];
-    store_full<LPM>(H, lay, T, n, t);
-    if constexpr (!GZ) store_full<LPM>(Z, lay, U, n, t);   // with GZ the QR updated U in place
-    if (t == 0) *flagp = info != 0 ? 1 : 0;
-}
-
-// The Schur form for n > 32 in two kernels, so that the QR, which dominates,
-// runs on the packed Hessenberg layout (34 KB of LDS at n = 64, four
-// matrices per CU) with U in global memory.  Same arithmetic as
-// schur_in_lds<NB, 64, false, true>, hence the same bits.
+// The Schur form in two kernels, so that the QR, which dominates, runs on
+// the packed Hessenberg layout (34 KB of LDS at n = 64, four matrices per CU;
+// 8.7 KB at n = 32) with U in global memory.  Same arithmetic as
+// schur_in_lds, hence the same bits.
 //
-// First half (full LDS layout, one wavefront per matrix): Hessenberg with
-// kept reflectors; the Hessenberg matrix (zeros below the subdiagonal) to Th
-// and Q (ZUNGHR, formed in place of the reflectors) to Ug, both n x n
+// First half (full LDS layout, LPM lanes per matrix): Hessenberg with kept
+// reflectors; the Hessenberg matrix (zeros below the subdiagonal) to Th and
+// Q (ZUNGHR, formed in place of the reflectors) to Ug, both n x n
 // column-major in global memory and written row by row by the owning lane.
 // Returns false for a non-finite input (nothing written).
 template <int LPM>
@@ -167,21 +143,23 @@ __device__ __forceinline__ bool schur_hq_one(zd *H, zd *taus, const zd *__restri
     return true;
 }
 
-// Second half (packed LDS layout, packed_size(n) complex): ZLAHQR with
-// WANTT/WANTZ on the Hessenberg matrix from Th, Z = Ug updated in place (lane
-// p owns row p, as it wrote it), then the eigenvalues to out and T (zeros
-// below the diagonal) back to Th, row by row by the lane that read it.
-// Returns the QR INFO.
+// Second half (packed LDS layout, packed_size(n) complex; n <= LPM): ZLAHQR
+// with WANTT/WANTZ on the Hessenberg matrix from Th, Z = Ug updated in place
+// (lane p owns row p, as it wrote it), then the eigenvalues to out and T
+// (zeros below the diagonal) back to Th, row by row by the lane that read
+// it.  Returns the QR INFO.
+template <int LPM>
 __device__ __forceinline__ int schur_qr_one(zd *H, int n, int t, zd *Th, zd *Ug, zd *__restrict__ out, int itmax)
 {
     const PackedLayout lay{};
-    for (int r = t; r < n; r += 64)
+    const int r = t;
+    if (r < n)
         for (int c = r > 0 ? r - 1 : 0; c < n; ++c) H[lay(r, c)] = Th[r + c * n];
     lds_order();
     int info = 0;
-    if (n > 1) info = zlahqr_core<64, 1, true, false>(H, lay, n, t, itmax, Ug, FullLayout{n});
+    if (n > 1) info = zlahqr_core<LPM, 1, true, false>(H, lay, n, t, itmax, Ug, FullLayout{n});
     lds_order();
-    for (int r = t; r < n; r += 64) {
+    if (r < n) {
         out[r] = H[lay(r, r)];
         for (int c = 0; c < n; ++c) Th[r + c * n] = r > c ? mk(0.0, 0.0) : H[lay(r, c)];
     }
