@@ -78,9 +78,9 @@ extern "C" int tokuraDestroy(tokurablas_t *handle)
     if (handle == nullptr) return 0;
     ragged_release(handle);
     for (auto &ev : handle->timed_calls)
-        for (auto &x : ev) hipEventDestroy(x);
+        for (auto &x : ev) (void)hipEventDestroy(x);   // teardown: nothing to undo on failure
     for (auto &ev : handle->event_pool)
-        for (auto &x : ev) hipEventDestroy(x);
+        for (auto &x : ev) (void)hipEventDestroy(x);
     delete handle;
     return 0;
 }
@@ -110,29 +110,38 @@ extern "C" int tokura_zgeev_batched_gpu(tokurablas_t *handle, int n, int batch_c
     std::array<hipEvent_t, 3> *ev = handle->stage_timing ? handle->timing_begin(stream) : nullptr;
     hipError_t e = launch_plain(v, n, batch_count, A, eigenvalues, flag, work, handle->max_sweeps, handle->balance,
                                 stream, &stage, ev ? (*ev)[1] : nullptr);
-    if (ev) {
-        if (variant_info(v)->kind == VK_FUSED) hipEventRecord((*ev)[1], stream);
+    if (ev) {   // stage timing is diagnostic: a failed record only spoils that reading
+        if (variant_info(v)->kind == VK_FUSED) (void)hipEventRecord((*ev)[1], stream);
         handle->timing_end(ev, stream);
     }
     if (e != hipSuccess) return stage;
     return 0;
 }
 
+// Events for one timed call (nullptr, and the call goes untimed, when they
+// cannot be created or recorded).
 std::array<hipEvent_t, 3> *tokurablas_t::timing_begin(hipStream_t s)
 {
-    std::array<hipEvent_t, 3> ev;
+    std::array<hipEvent_t, 3> ev{};
     if (!event_pool.empty()) {
         ev = event_pool.back();
         event_pool.pop_back();
     } else {
-        for (auto &x : ev) hipEventCreate(&x);
+        for (int k = 0; k < 3; ++k)
+            if (hipEventCreate(&ev[k]) != hipSuccess) {
+                for (int j = 0; j < k; ++j) (void)hipEventDestroy(ev[j]);
+                return nullptr;
+            }
+    }
+    if (hipEventRecord(ev[0], s) != hipSuccess) {
+        event_pool.push_back(ev);
+        return nullptr;
     }
     timed_calls.push_back(ev);
-    hipEventRecord(timed_calls.back()[0], s);
     return &timed_calls.back();
 }
 
-void tokurablas_t::timing_end(std::array<hipEvent_t, 3> *ev, hipStream_t s) { hipEventRecord((*ev)[2], s); }
+void tokurablas_t::timing_end(std::array<hipEvent_t, 3> *ev, hipStream_t s) { (void)hipEventRecord((*ev)[2], s); }
 
 extern "C" int tokuraSetStageTiming(tokurablas_t *handle, int enable)
 {
@@ -149,8 +158,8 @@ extern "C" int tokuraGetStageTimes(tokurablas_t *handle, int *calls, float *hess
     for (auto &ev : handle->timed_calls) {
         if (hipEventSynchronize(ev[2]) != hipSuccess) rc = 1;
         float a = 0.f, b = 0.f;
-        hipEventElapsedTime(&a, ev[0], ev[1]);
-        hipEventElapsedTime(&b, ev[1], ev[2]);
+        if (hipEventElapsedTime(&a, ev[0], ev[1]) != hipSuccess || hipEventElapsedTime(&b, ev[1], ev[2]) != hipSuccess)
+            rc = 1;
         s1 += a;
         s2 += b;
         handle->event_pool.push_back(ev);
@@ -241,7 +250,7 @@ extern "C" int tokuraAutotune(tokurablas_t *handle, int n_lo, int n_hi, int batc
     if (hipMalloc(&dE, maxn * batch_count * sizeof(hipDoubleComplex)) != hipSuccess ||
         hipMalloc(&dW, tokura_get_zgeeveigenvaluesgetworspacesize((int)maxn, batch_count)) != hipSuccess ||
         hipMalloc(&dF, (size_t)batch_count) != hipSuccess) {
-        hipFree(dA); hipFree(dE); hipFree(dW); hipFree(dF);
+        (void)hipFree(dA); (void)hipFree(dE); (void)hipFree(dW); (void)hipFree(dF);
         return 3;
     }
     // seeded N(0,1) entries, the distribution of the benchmark workloads
@@ -249,11 +258,11 @@ extern "C" int tokuraAutotune(tokurablas_t *handle, int n_lo, int n_hi, int batc
     std::mt19937_64 rng(12345);
     std::normal_distribution<double> nd(0.0, 1.0);
     for (auto &z : hA) z = make_hipDoubleComplex(nd(rng), nd(rng));
-    hipMemcpyAsync(dA, hA.data(), nA * sizeof(hipDoubleComplex), hipMemcpyHostToDevice, stream);
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = 0;
+    if (hipMemcpyAsync(dA, hA.data(), nA * sizeof(hipDoubleComplex), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = 4;
     for (int n = n_lo; n <= n_hi && rc == 0; ++n) {
         float best = 1e30f;
         int bestv = default_variant(n);
@@ -265,15 +274,14 @@ extern "C" int tokuraAutotune(tokurablas_t *handle, int n_lo, int n_hi, int batc
             int stage;
             float tms = 0.f;
             for (int rep = 0; rep < 3; ++rep) {
-                hipEventRecord(e0, stream);
-                if (launch_plain(v, n, batch_count, dA, dE, dF, dW, -1, handle->balance, stream, &stage) != hipSuccess) {
+                float ms = 0.f;
+                if (hipEventRecord(e0, stream) != hipSuccess ||
+                    launch_plain(v, n, batch_count, dA, dE, dF, dW, -1, handle->balance, stream, &stage) != hipSuccess ||
+                    hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
                     rc = 4;
                     break;
                 }
-                hipEventRecord(e1, stream);
-                hipEventSynchronize(e1);
-                float ms;
-                hipEventElapsedTime(&ms, e0, e1);
                 if (rep > 0) tms += ms;   // the first run warms up
             }
             if (rc) break;
@@ -281,9 +289,9 @@ extern "C" int tokuraAutotune(tokurablas_t *handle, int n_lo, int n_hi, int batc
         }
         handle->variant_for_n[n] = bestv;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipFree(dA); hipFree(dE); hipFree(dW); hipFree(dF);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(dA); (void)hipFree(dE); (void)hipFree(dW); (void)hipFree(dF);
     if (rc == 0 && cache_path != nullptr) {
         FILE *f = std::fopen(cache_path, "w");
         if (!f) return 5;

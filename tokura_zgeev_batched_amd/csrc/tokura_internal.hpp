@@ -49,5 +49,15 @@ struct tokurablas_t {
     void timing_end(std::array<hipEvent_t, 3> *ev, hipStream_t s);
 };
 
+// Resident workgroups of kernel `k` per CU at `threads` threads per block
+// (1 when the query fails or reports 0), for sizing persistent grids.
+template <class K>
+int resident_blocks(K k, int threads = 64)
+{
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, threads, 0) != hipSuccess) return 1;
+    return occ < 1 ? 1 : occ;
+}
+
 void ragged_release(tokurablas_t *h);
 int ragged_num_configs();

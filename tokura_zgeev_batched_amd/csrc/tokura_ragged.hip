@@ -223,15 +223,9 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     // resident workgroups per CU, queried once (outside any graph capture the
     // first call is made from tokura_zgeev_ragged_graph's capture stream, where
     // occupancy queries are not stream operations and are allowed)
-    static int occ_h = 0, occ_q = 0, occ_w = 0;
-    if (occ_h == 0) {
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_h, k_ragged_hess<LB, LC>, 64, 0);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_w, k_ragged_hess_wg, 256, 0);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_q, k_ragged_qr<LA1, LA2, LB, LC>, 64, 0);
-        if (occ_h < 1) occ_h = 1;
-        if (occ_w < 1) occ_w = 1;
-        if (occ_q < 1) occ_q = 1;
-    }
+    static const int occ_h = resident_blocks(k_ragged_hess<LB, LC>);
+    static const int occ_w = resident_blocks(k_ragged_hess_wg, 256);
+    static const int occ_q = resident_blocks(k_ragged_qr<LA1, LA2, LB, LC>);
     hipLaunchKernelGGL(k_ragged_hess_wg, dim3(cus * occ_w * 4), dim3(256), 0, s, A, work, flag, bal);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -368,14 +362,7 @@ __global__ __launch_bounds__(NT) void k_ragged_adjoint_wg(char *work, const zd *
     }
 }
 
-// resident workgroups of `k` (block of `threads`) per CU, queried once per kernel
-template <class K>
-static int resident_blocks(K k, int threads = 64)
-{
-    int occ = 0;
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, threads, 0);
-    return occ < 1 ? 1 : occ;
-}
+
 
 static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schur, int8_t *flag, int itmax, int cus,
                                       hipStream_t s)
@@ -487,10 +474,10 @@ using namespace tkz;
 
 void ragged_release(tokurablas_t *h)
 {
-    if (h->graph.exec) hipGraphExecDestroy(h->graph.exec);
-    if (h->graph.graph) hipGraphDestroy(h->graph.graph);
+    if (h->graph.exec) (void)hipGraphExecDestroy(h->graph.exec);   // teardown: nothing to undo on failure
+    if (h->graph.graph) (void)hipGraphDestroy(h->graph.graph);
     h->graph = RaggedGraphCache();
-    if (h->ragged_host_stage) hipHostFree(h->ragged_host_stage);
+    if (h->ragged_host_stage) (void)hipHostFree(h->ragged_host_stage);
     h->ragged_host_stage = nullptr;
     h->ragged_host_bytes = 0;
     h->prepared.clear();
@@ -526,9 +513,9 @@ extern "C" int tokura_zgeev_ragged_prepare(tokurablas_t *handle, int batch_count
     if ((long long)work_bytes < S.hdr.total_bytes) return -5;
     const size_t meta = (size_t)S.hdr.off_H;
     // the staging buffer may still feed an earlier copy: wait for it
-    if (handle->ragged_host_stage) hipStreamSynchronize(handle->ragged_stage_stream);
+    if (handle->ragged_host_stage && hipStreamSynchronize(handle->ragged_stage_stream) != hipSuccess) return 3;
     if (handle->ragged_host_bytes < meta) {
-        if (handle->ragged_host_stage) hipHostFree(handle->ragged_host_stage);
+        if (handle->ragged_host_stage) (void)hipHostFree(handle->ragged_host_stage);
         handle->ragged_host_stage = nullptr;
         handle->ragged_host_bytes = 0;
         if (hipHostMalloc(&handle->ragged_host_stage, meta, hipHostMallocDefault) != hipSuccess) return 3;
@@ -593,8 +580,8 @@ extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, 
                      G.flag == flag && G.batch == batch_count && G.cfg == cfg && G.max_sweeps == handle->max_sweeps &&
                      G.balance == handle->balance;
     if (!hit) {
-        if (G.exec) hipGraphExecDestroy(G.exec);
-        if (G.graph) hipGraphDestroy(G.graph);
+        if (G.exec) (void)hipGraphExecDestroy(G.exec);
+        if (G.graph) (void)hipGraphDestroy(G.graph);
         G = RaggedGraphCache();
         hipStream_t cs;
         if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return 3;
@@ -607,9 +594,9 @@ extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, 
             if (el != hipSuccess) e = el;
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0);
-        hipStreamDestroy(cs);
+        (void)hipStreamDestroy(cs);
         if (e != hipSuccess) {
-            if (G.graph) hipGraphDestroy(G.graph);
+            if (G.graph) (void)hipGraphDestroy(G.graph);
             G = RaggedGraphCache();
             return 2;
         }
@@ -647,30 +634,29 @@ extern "C" int tokuraAutotuneRagged(tokurablas_t *handle, int batch_count, const
 {
     if (handle == nullptr) return -1;
     const int keep = handle->ragged_cfg;
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     float best = 1e30f;
     int bestc = keep, rc = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = 3;
     for (int c = 0; c < kNumRaggedCfg && rc == 0; ++c) {
         handle->ragged_cfg = c;
         rc = tokura_zgeev_ragged_prepare(handle, batch_count, offsets_host, work, work_bytes, stream);
         if (rc) break;
         float ms = 0.f;
         for (int rep = 0; rep < 2 && rc == 0; ++rep) {
-            hipEventRecord(e0, stream);
+            if (hipEventRecord(e0, stream) != hipSuccess) { rc = 3; break; }
             rc = tokura_zgeev_ragged_gpu(handle, batch_count, A, eigenvalues, work, flag, stream);
-            hipEventRecord(e1, stream);
-            hipEventSynchronize(e1);
-            hipEventElapsedTime(&ms, e0, e1);
+            if (rc == 0 && (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = 3;
         }
         if (rc == 0 && ms < best) { best = ms; bestc = c; }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     handle->ragged_cfg = rc == 0 ? bestc : keep;
     int rc2 = tokura_zgeev_ragged_prepare(handle, batch_count, offsets_host, work, work_bytes, stream);
-    hipStreamSynchronize(stream);
+    if (rc2 == 0 && hipStreamSynchronize(stream) != hipSuccess) rc2 = 3;
     return rc ? rc : rc2;
 }
 
