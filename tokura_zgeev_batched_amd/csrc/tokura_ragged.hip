@@ -1,16 +1,17 @@
 // Ragged batches: n varies per matrix (1..64).  No counterpart in the
 // reference, which requires one n per call (README.md:5, tokura_blas.h:12).
 //
-// Pipeline (both launches are captured together by tokura_zgeev_ragged_graph):
-//  1. k_ragged_reset    zeroes the queue head (a fill kernel, graph-safe).
+// Pipeline (all launches are captured together by tokura_zgeev_ragged_graph):
+//  1. k_ragged_reset    zeroes the queue heads (a fill kernel, graph-safe).
 //  2. Householder Hessenberg of every matrix, packed result written to `work`:
 //     k_ragged_hess_wg  one 256-thread workgroup per matrix for n > 32,
 //     k_ragged_hess     one wavefront per tile of small matrices (n <= 32).
-//  3. k_ragged_qr       ONE persistent kernel: every wavefront pulls the next
-//                       tile from an atomic queue until the queue is empty, so
-//                       matrices that need more QR sweeps never hold up a
-//                       whole launch; tiles are queued in decreasing size
-//                       (longest-processing-time first).
+//  3. k_ragged_qr       one persistent launch per size class, largest first,
+//                       each with the LDS of its class only: every wavefront
+//                       pulls the class's next tile from an atomic queue until
+//                       the queue is empty, so matrices that need more QR
+//                       sweeps never hold up a whole launch; tiles are queued
+//                       in decreasing size (longest-processing-time first).
 // A tile is the work of one wavefront: 64/LPM matrices of one size class,
 // A1 (n 49..64), A2 (33..48), B (17..32) or C (1..16); the lanes-per-matrix of
 // each class form a "class configuration" that the autotuner picks.  Smaller
@@ -34,26 +35,26 @@
 namespace tkz {
 
 struct RaggedHeader {
-    int counter;   // QR queue head
+    int qctr[4];       // QR queue heads, one per size class
     int ntiles;
     int batch;
     int cfg;
+    int nbig;          // matrices of classes A1/A2 (n > 32): order[0 .. nbig)
     long long off_tiles, off_order, off_n, off_moff, off_eoff, off_hoff, off_H;
     long long total_bytes;
-    int nbig;          // matrices of classes A1/A2 (n > 32): order[0 .. nbig)
     int tile_small0;   // first tile of classes B/C
     int n16;           // matrices with n > 16: order[0 .. n16)
+    int tile_cls[5];   // tiles of class c: [tile_cls[c], tile_cls[c+1])
     int pad_i;
-    long long pad[4];
 };
 static_assert(sizeof(RaggedHeader) == 128, "header size");
 
 // lanes per matrix of classes A1 (49..64), A2 (33..48), B (17..32), C (1..16)
 #define TKZ_RAGGED_CFGS(X) \
-    X(0, 64, 32, 16, 4)    \
+    X(0, 64, 64, 16, 8)    \
     X(1, 64, 64, 32, 8)    \
     X(2, 64, 32, 16, 8)    \
-    X(3, 64, 32, 32, 4)
+    X(3, 64, 64, 32, 16)
 static const int g_cfg_lpm[][4] = {
 #define X(id, a1, a2, b, c) {a1, a2, b, c},
     TKZ_RAGGED_CFGS(X)
@@ -82,7 +83,11 @@ struct RaggedView {
     }
 };
 
-__global__ void k_ragged_reset(char *work) { ((RaggedHeader *)work)->counter = 0; }
+__global__ void k_ragged_reset(char *work)
+{
+    RaggedHeader *h = (RaggedHeader *)work;
+    for (int c = 0; c < 4; ++c) h->qctr[c] = 0;
+}
 
 template <int NB, int LPM>
 __device__ __forceinline__ void hess_tile(zd *lds, const RaggedView &V, int4 td, const zd *__restrict__ A,
@@ -191,25 +196,23 @@ __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A
     }
 }
 
-template <int LA1, int LA2, int LB, int LC>
+// QR of one size class: a persistent kernel whose LDS is sized for that
+// class alone (so the small classes run more wavefronts per CU than the
+// n = 64 class); every wavefront pulls the class's next tile from its queue
+// until the queue is empty.
+template <int CLS, int NB, int LPM>
 __global__ __launch_bounds__(64) void k_ragged_qr(char *work, zd *__restrict__ eig, int8_t *__restrict__ flag, int itmax)
 {
-    constexpr int SZ = cmax(cmax((64 / LA1) * packed_size(64), (64 / LA2) * packed_size(48)),
-                            cmax((64 / LB) * packed_size(32), (64 / LC) * packed_size(16)));
-    __shared__ zd lds[SZ];
+    __shared__ zd lds[(64 / LPM) * packed_size(NB)];
     const RaggedView V(work);
-    const int nt = V.h->ntiles;
+    const int t0 = V.h->tile_cls[CLS], t1 = V.h->tile_cls[CLS + 1];
     const int lane = threadIdx.x & 63;
     for (;;) {
-        int tile = 0;
-        if (lane == 0) tile = atomicAdd(&V.h->counter, 1);
-        tile = __shfl(tile, 0);
-        if (tile >= nt) break;
-        const int4 td = V.tiles[tile];
-        if (td.x == 0) qr_tile<64, LA1>(lds, V, td, eig, flag, itmax);
-        else if (td.x == 1) qr_tile<48, LA2>(lds, V, td, eig, flag, itmax);
-        else if (td.x == 2) qr_tile<32, LB>(lds, V, td, eig, flag, itmax);
-        else qr_tile<16, LC>(lds, V, td, eig, flag, itmax);
+        int k = 0;
+        if (lane == 0) k = atomicAdd(&V.h->qctr[CLS], 1);
+        const int tile = t0 + __shfl(k, 0);
+        if (tile >= t1) break;
+        qr_tile<NB, LPM>(lds, V, V.tiles[tile], eig, flag, itmax);
     }
 }
 
@@ -225,14 +228,21 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     // occupancy queries are not stream operations and are allowed)
     static const int occ_h = resident_blocks(k_ragged_hess<LB, LC>);
     static const int occ_w = resident_blocks(k_ragged_hess_wg, 256);
-    static const int occ_q = resident_blocks(k_ragged_qr<LA1, LA2, LB, LC>);
+    static const int occ_q0 = resident_blocks(k_ragged_qr<0, 64, LA1>);
+    static const int occ_q1 = resident_blocks(k_ragged_qr<1, 48, LA2>);
+    static const int occ_q2 = resident_blocks(k_ragged_qr<2, 32, LB>);
+    static const int occ_q3 = resident_blocks(k_ragged_qr<3, 16, LC>);
     hipLaunchKernelGGL(k_ragged_hess_wg, dim3(cus * occ_w * 4), dim3(256), 0, s, A, work, flag, bal);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag, bal);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ragged_qr<LA1, LA2, LB, LC>), dim3(cus * occ_q), dim3(64), 0, s, work, eig, flag, itmax);
+    // largest class first: the tail of one launch is at most one tile of it
+    hipLaunchKernelGGL((k_ragged_qr<0, 64, LA1>), dim3(cus * occ_q0), dim3(64), 0, s, work, eig, flag, itmax);
+    hipLaunchKernelGGL((k_ragged_qr<1, 48, LA2>), dim3(cus * occ_q1), dim3(64), 0, s, work, eig, flag, itmax);
+    hipLaunchKernelGGL((k_ragged_qr<2, 32, LB>), dim3(cus * occ_q2), dim3(64), 0, s, work, eig, flag, itmax);
+    hipLaunchKernelGGL((k_ragged_qr<3, 16, LC>), dim3(cus * occ_q3), dim3(64), 0, s, work, eig, flag, itmax);
     return hipGetLastError();
 }
 
@@ -453,6 +463,10 @@ static int build_schedule(int batch, const int64_t *offsets, int cfg, Schedule &
     while (S.hdr.n16 < batch && S.nsz[S.order[S.hdr.n16]] > 16) ++S.hdr.n16;
     S.hdr.tile_small0 = 0;
     while (S.hdr.tile_small0 < ntiles && S.tiles[4 * S.hdr.tile_small0] < 2) ++S.hdr.tile_small0;
+    for (int c = 0, k = 0; c <= 4; ++c) {   // tiles are in class order
+        while (k < ntiles && S.tiles[4 * k] < c) ++k;
+        S.hdr.tile_cls[c] = c == 4 ? ntiles : k;
+    }
     S.hdr.ntiles = ntiles;
     S.hdr.batch = batch;
     S.hdr.cfg = cfg;
