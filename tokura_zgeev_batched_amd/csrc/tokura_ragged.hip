@@ -45,7 +45,7 @@ struct RaggedHeader {
     int tile_small0;   // first tile of classes B/C
     int n16;           // matrices with n > 16: order[0 .. n16)
     int tile_cls[5];   // tiles of class c: [tile_cls[c], tile_cls[c+1])
-    int pad_i;
+    int n48;           // matrices with n > 48: order[0 .. n48)
 };
 static_assert(sizeof(RaggedHeader) == 128, "header size");
 
@@ -158,17 +158,19 @@ __global__ __launch_bounds__(64) void k_ragged_hess(const zd *__restrict__ A, ch
     }
 }
 
-// Hessenberg of the large classes (A1, A2): one 256-thread workgroup per
-// matrix, quads of threads sharing each row/column.
+// Hessenberg of a large class (NB 64: A1, n 49..64; NB 48: A2, n 33..48):
+// one 256-thread workgroup per matrix, quads of threads sharing each
+// row/column, LDS sized for the class.
+template <int NB>
 __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A, char *work,
                                                         int8_t *__restrict__ flag, int bal)
 {
-    __shared__ zd lds[64 * 65];
+    __shared__ zd lds[NB * (NB | 1)];
     __shared__ double red[8];
     const RaggedView V(work);
-    const int nbig = V.h->nbig;
+    const int lo = NB == 64 ? 0 : V.h->n48, hi = NB == 64 ? V.h->n48 : V.h->nbig;
     const int tid = threadIdx.x;
-    for (int j = blockIdx.x; j < nbig; j += gridDim.x) {
+    for (int j = lo + (int)blockIdx.x; j < hi; j += gridDim.x) {
         const int b = V.order[j];
         const int n = V.nsz[b];
         const FullLayout lay{n | 1};
@@ -227,12 +229,14 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     // first call is made from tokura_zgeev_ragged_graph's capture stream, where
     // occupancy queries are not stream operations and are allowed)
     static const int occ_h = resident_blocks(k_ragged_hess<LB, LC>);
-    static const int occ_w = resident_blocks(k_ragged_hess_wg, 256);
+    static const int occ_w1 = resident_blocks(k_ragged_hess_wg<64>, 256);
+    static const int occ_w2 = resident_blocks(k_ragged_hess_wg<48>, 256);
     static const int occ_q0 = resident_blocks(k_ragged_qr<0, 64, LA1>);
     static const int occ_q1 = resident_blocks(k_ragged_qr<1, 48, LA2>);
     static const int occ_q2 = resident_blocks(k_ragged_qr<2, 32, LB>);
     static const int occ_q3 = resident_blocks(k_ragged_qr<3, 16, LC>);
-    hipLaunchKernelGGL(k_ragged_hess_wg, dim3(cus * occ_w * 4), dim3(256), 0, s, A, work, flag, bal);
+    hipLaunchKernelGGL((k_ragged_hess_wg<64>), dim3(cus * occ_w1 * 4), dim3(256), 0, s, A, work, flag, bal);
+    hipLaunchKernelGGL((k_ragged_hess_wg<48>), dim3(cus * occ_w2 * 4), dim3(256), 0, s, A, work, flag, bal);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag, bal);
@@ -457,7 +461,9 @@ static int build_schedule(int batch, const int64_t *offsets, int cfg, Schedule &
     }
     const int ntiles = (int)(S.tiles.size() / 4);
     std::memset(&S.hdr, 0, sizeof S.hdr);
-    S.hdr.nbig = 0;
+    S.hdr.n48 = 0;
+    while (S.hdr.n48 < batch && S.nsz[S.order[S.hdr.n48]] > 48) ++S.hdr.n48;
+    S.hdr.nbig = S.hdr.n48;
     while (S.hdr.nbig < batch && S.nsz[S.order[S.hdr.nbig]] > 32) ++S.hdr.nbig;
     S.hdr.n16 = S.hdr.nbig;
     while (S.hdr.n16 < batch && S.nsz[S.order[S.hdr.n16]] > 16) ++S.hdr.n16;
