@@ -65,7 +65,10 @@ def backward_batched(n: int, batch: int, gbar: torch.Tensor, schur: torch.Tensor
 _RECOMPUTE_HANDLE = None
 
 
-def _recompute_handle() -> Handle:
+def recompute_handle() -> Handle:
+    """Handle whose eigenvalue variants (fused, lanes = the backward's NB)
+    share the arithmetic of the recompute backward's Schur form, so the
+    forward's eigenvalue order is the order the backward differentiates."""
     global _RECOMPUTE_HANDLE
     if _RECOMPUTE_HANDLE is None:
         h = Handle()
@@ -85,7 +88,7 @@ class _Eigvals(torch.autograd.Function):
             eig, schur, flag = schur_batched(A, n, B)
             ctx.save_for_backward(schur, flag)
         elif mode == "recompute":
-            h = _recompute_handle()
+            h = recompute_handle()
             eig = torch.empty((B, n), dtype=torch.complex128, device=M.device)
             flag = torch.empty((B,), dtype=torch.int8, device=M.device)
             work = torch.empty((workspace_size(n, B),), dtype=torch.uint8, device=M.device)
@@ -110,7 +113,7 @@ class _Eigvals(torch.autograd.Function):
             dA, _ = backward_batched(ctx.n, ctx.B, geig, schur=saved, flag=flag.clone())
         else:
             dA, _ = backward_batched(ctx.n, ctx.B, geig, A_colmajor=saved, flag=torch.empty_like(flag),
-                                     handle=_recompute_handle())
+                                     handle=recompute_handle())
         return dA.transpose(-1, -2), None
 
 
