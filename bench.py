@@ -288,15 +288,24 @@ def bench_vectors(dev, cases=((32, 50_000), (64, 20_000)), steps=3, seed=11):
             if rc != 0:
                 raise RuntimeError(f"tokura_zgeev_batched_vectors_gpu returned {rc}")
 
-        def right():
+        work = torch.empty(tk.vectors_workspace_size(n, B), dtype=torch.uint8, device=dev)
+
+        def right():          # T parked in the optional workspace
+            rc = tk.zgeev_vectors(A, n, B, eig, None, VR, flag, h, work=work)
+            if rc != 0:
+                raise RuntimeError(f"tokura_zgeev_batched_vectors_gpu returned {rc}")
+
+        def right_no_ws():    # one-kernel form, no workspace
             tk.zgeev_vectors(A, n, B, eig, None, VR, flag, h)
 
         t_both = timed(both, steps, 1)
         t_right = timed(right, steps, 1)
+        t_right1 = timed(right_no_ws, steps, 1)
         out[f"n{n}"] = {"batch": B, "left_right_ms": t_both * 1e3, "right_only_ms": t_right * 1e3,
+                        "right_only_no_workspace_ms": t_right1 * 1e3,
                         "matrices_per_s_left_right": B / t_both, "matrices_per_s_right": B / t_right,
                         "flags_nonzero": int((flag != 0).sum())}
-        del A, eig, flag, VL, VR
+        del A, eig, flag, VL, VR, work
         torch.cuda.empty_cache()
     return out
 

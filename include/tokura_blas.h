@@ -217,11 +217,17 @@ int tokura_zgeev_ragged_backward_gpu(tokurablas_t *handle, int batch_count, cons
  * ZGEEV does: unit Euclidean norm, the component of largest modulus real and
  * non-negative.  Eigenvalues are in the order of the Schur diagonal.  VL and
  * VR may not both be NULL (-5).  flag as in tokura_zgeev_batched_gpu; a
- * matrix with flag != 0 gets NaN vectors.  A is not modified; no workspace. */
+ * matrix with flag != 0 gets NaN vectors.  A is not modified.
+ * `work` (may be NULL) is used only when one side is asked for: with
+ * tokura_get_zgeev_vectors_worksize(n, batch) bytes the Schur factor T is
+ * parked there and the faster split kernels run (both sides park T in VL);
+ * without it a slower one-kernel form keeps T in LDS.  Results are the same
+ * bits either way. */
+size_t tokura_get_zgeev_vectors_worksize(int n, int batch_count);
 int tokura_zgeev_batched_vectors_gpu(tokurablas_t *handle, int n, int batch_count,
                                      const hipDoubleComplex *A, hipDoubleComplex *eigenvalues,
-                                     hipDoubleComplex *VL, hipDoubleComplex *VR, char *flag,
-                                     hipStream_t stream);
+                                     hipDoubleComplex *VL, hipDoubleComplex *VR, hipDoubleComplex *work,
+                                     char *flag, hipStream_t stream);
 
 #ifdef __cplusplus
 }

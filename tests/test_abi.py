@@ -108,18 +108,23 @@ def test_vectors_entry_validation():
     L = tk.lib()
     h = make_handle()
     P = ctypes.c_void_p
-    a, e, vl, vr, f = P(0x1000), P(0x2000), P(0x3000), P(0x4000), P(0x5000)
+    a, e, vl, vr, f, w = P(0x1000), P(0x2000), P(0x3000), P(0x4000), P(0x5000), P(0x6000)
     s = P(0)
     g = L.tokura_zgeev_batched_vectors_gpu
-    assert g(None, 4, 1, a, e, vl, vr, f, s) == -1
-    assert g(h, 0, 1, a, e, vl, vr, f, s) == -2
-    assert g(h, 65, 1, a, e, vl, vr, f, s) == -2
-    assert g(h, 4, 1, None, e, vl, vr, f, s) == -3
-    assert g(h, 4, 1, a, None, vl, vr, f, s) == -4
-    assert g(h, 4, 1, a, e, None, None, f, s) == -5     # neither left nor right vectors asked for
-    assert g(h, 4, 0, a, e, vl, vr, f, s) == -6
-    assert g(h, 4, 1, a, e, vl, vr, None, s) == -7
+    assert g(None, 4, 1, a, e, vl, vr, w, f, s) == -1
+    assert g(h, 0, 1, a, e, vl, vr, w, f, s) == -2
+    assert g(h, 65, 1, a, e, vl, vr, None, f, s) == -2
+    assert g(h, 4, 1, None, e, vl, vr, w, f, s) == -3
+    assert g(h, 4, 1, a, None, vl, vr, w, f, s) == -4
+    assert g(h, 4, 1, a, e, None, None, w, f, s) == -5     # neither left nor right vectors asked for
+    assert g(h, 4, 0, a, e, vl, vr, w, f, s) == -6
+    assert g(h, 4, 1, a, e, vl, vr, w, None, s) == -7
     assert L.tokuraDestroy(h) == 0
+    ws = L.tokura_get_zgeev_vectors_worksize
+    assert ws(64, 1000) == 16 * 64 * 64 * 1000            # T of every matrix, n*n complex
+    assert ws(0, 10) == ctypes.c_size_t(-1).value
+    assert ws(65, 10) == ctypes.c_size_t(-1).value
+    assert ws(8, 0) == ctypes.c_size_t(-2).value
 
 
 def test_balancing_switch():

@@ -67,16 +67,20 @@ def test_vectors_one_side_and_schur_order(gpu, n):
     M = torch.tensor(rand(rng, 16, n), device=gpu)
     print(f"[case] vectors one side n={n}", flush=True)
     w_lr, VL, VR, f0 = tk.eig(M, left=True, right=True)
-    w_r, _, VR_only, f1 = tk.eig(M, left=False, right=True)
+    w_r, _, VR_only, f1 = tk.eig(M, left=False, right=True)                    # T parked in the workspace
     w_l, VL_only, _, f2 = tk.eig(M, left=True, right=False)
+    w_r1, _, VR_1k, f4 = tk.eig(M, left=False, right=True, workspace=False)    # one-kernel form
+    w_l1, VL_1k, _, f5 = tk.eig(M, left=True, right=False, workspace=False)
     A = M.transpose(-1, -2).contiguous()
     e_s, _, f3 = schur_batched(A, n, 16)
     torch.cuda.synchronize()
-    for f in (f0, f1, f2, f3):
+    for f in (f0, f1, f2, f3, f4, f5):
         assert int(f.abs().sum()) == 0
     # one Schur form behind every call: identical eigenvalues and vectors
-    assert torch.equal(w_lr, e_s) and torch.equal(w_r, e_s) and torch.equal(w_l, e_s)
+    for w in (w_lr, w_r, w_l, w_r1, w_l1):
+        assert torch.equal(w, e_s)
     assert torch.equal(VR, VR_only) and torch.equal(VL, VL_only)
+    assert torch.equal(VR, VR_1k) and torch.equal(VL, VL_1k)
 
 
 def test_vectors_closed_forms(gpu):
@@ -101,3 +105,9 @@ def test_vectors_nan_flag(gpu):
     assert flag.cpu().tolist() == [0, 2, 0]
     assert torch.isnan(VR[1].real).all() and torch.isnan(VL[1].real).all() and torch.isnan(w[1].real).all()
     assert not torch.isnan(VR[0].real).any() and not torch.isnan(VR[2].real).any()
+    for M2 in (M, rand(rng, 3, 40)):          # one-sided, both size classes of the split path
+        M2[1, 4, 5] = np.nan
+        w, VL, VR, flag = tk.eig(torch.tensor(M2, device=gpu), left=False, right=True)
+        torch.cuda.synchronize()
+        assert flag.cpu().tolist() == [0, 2, 0] and VL is None
+        assert torch.isnan(VR[1].real).all() and not torch.isnan(VR[[0, 2]].real).any()

@@ -62,7 +62,8 @@ _SIGNATURES = {
     "tokura_get_zgeev_schur_worksize": ([_I, _I], _SZ),
     "tokura_zgeev_batched_schur_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P], _I),
     "tokura_zgeev_batched_backward_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P, _P], _I),
-    "tokura_zgeev_batched_vectors_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P, _P], _I),
+    "tokura_get_zgeev_vectors_worksize": ([_I, _I], ctypes.c_size_t),
+    "tokura_zgeev_batched_vectors_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P, _P, _P], _I),
     "tokura_get_zgeev_ragged_schur_worksize": ([_I, _P], _SZ),
     "tokura_zgeev_ragged_schur_gpu": ([_P, _I, _P, _P, _P, _P, _P, _P], _I),
     "tokura_zgeev_ragged_backward_gpu": ([_P, _I, _P, _P, _P, _P, _P, _P], _I),
@@ -221,24 +222,34 @@ def eigvals(M: torch.Tensor, handle: Handle | None = None, colmajor: bool = Fals
 
 
 def zgeev_vectors(A_colmajor: torch.Tensor, n: int, batch: int, eig: torch.Tensor, VL: torch.Tensor | None,
-                  VR: torch.Tensor | None, flag: torch.Tensor, handle: Handle | None = None, stream=None) -> int:
+                  VR: torch.Tensor | None, flag: torch.Tensor, handle: Handle | None = None, stream=None,
+                  work: torch.Tensor | None = None) -> int:
     """Raw tokura_zgeev_batched_vectors_gpu: A column-major per matrix, eig
     batch*n, VL/VR batch*n*n column-major (either may be None), flag batch
-    int8.  Returns the C code."""
+    int8, work (optional, used for one-sided requests) at least
+    vectors_workspace_size(n, batch) bytes.  Returns the C code."""
     h = handle or default_handle()
     return int(lib().tokura_zgeev_batched_vectors_gpu(
         h.ptr, n, batch, _P(A_colmajor.data_ptr()), _P(eig.data_ptr()),
         _P(VL.data_ptr()) if VL is not None else None, _P(VR.data_ptr()) if VR is not None else None,
-        _P(flag.data_ptr()), _stream_ptr(stream)))
+        _P(work.data_ptr()) if work is not None else None, _P(flag.data_ptr()), _stream_ptr(stream)))
 
 
-def eig(M: torch.Tensor, left: bool = False, right: bool = True, handle: Handle | None = None, stream=None):
+def vectors_workspace_size(n: int, batch: int) -> int:
+    """Bytes of the optional workspace of a one-sided eigenvector call."""
+    return int(lib().tokura_get_zgeev_vectors_worksize(n, batch))
+
+
+def eig(M: torch.Tensor, left: bool = False, right: bool = True, handle: Handle | None = None, stream=None,
+        workspace: bool = True):
     """Eigenvalues and eigenvectors of a (B, n, n) complex128 device tensor.
 
     Returns (w (B, n), VL (B, n, n) or None, VR (B, n, n) or None, flag (B,)),
     with VR[b, :, i] the right eigenvector of w[b, i] (M[b] v = w v) and
     VL[b, :, i] the left one (u^H M[b] = w u^H), normalised as LAPACK zgeev
-    (unit 2-norm, largest component real).  At least one of left/right."""
+    (unit 2-norm, largest component real).  At least one of left/right.
+    A one-sided call allocates the optional workspace unless workspace=False
+    (same results, slower kernels, no extra memory)."""
     _dev_check(M, "M", torch.complex128)
     if M.dim() == 2:
         M = M.unsqueeze(0)
@@ -252,7 +263,10 @@ def eig(M: torch.Tensor, left: bool = False, right: bool = True, handle: Handle 
     flag = torch.empty((B,), dtype=torch.int8, device=M.device)
     VL = torch.empty((B, n, n), dtype=torch.complex128, device=M.device) if left else None
     VR = torch.empty((B, n, n), dtype=torch.complex128, device=M.device) if right else None
-    _check(zgeev_vectors(A, n, B, w, VL, VR, flag, handle, stream), "tokura_zgeev_batched_vectors_gpu")
+    work = None
+    if workspace and not (left and right):
+        work = torch.empty(vectors_workspace_size(n, B), dtype=torch.uint8, device=M.device)
+    _check(zgeev_vectors(A, n, B, w, VL, VR, flag, handle, stream, work), "tokura_zgeev_batched_vectors_gpu")
     # the library writes column-major: stored[b, i, r] = V(r, i)
     return (w, VL.transpose(-1, -2) if left else None, VR.transpose(-1, -2) if right else None, flag)
 

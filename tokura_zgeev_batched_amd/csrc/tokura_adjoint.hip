@@ -107,12 +107,13 @@ __global__ __launch_bounds__(64) void k_schur_qr(int n, int batch, zd *__restric
     if (t == 0) flag[mat] = info != 0 ? 1 : 0;
 }
 
-// Eigenvectors (both sides) after k_schur_hq / k_schur_qr left T in VL's slot
-// and U in VR's: T into LDS (each lane its own rows, so the later writes of
-// VL by the same lanes need no fence), then vectors_from_schur in place.
+// Eigenvectors after k_schur_hq / k_schur_qr left T in its slot (VL's when
+// both sides are asked for, else the caller's workspace) and U in VR's (VL's
+// when VR is NULL): T into LDS (each lane its own rows, so the later writes
+// of VL by the same lanes need no fence), then vectors_from_schur in place.
 // Flagged matrices get NaN vectors.
 template <int NB, int LPM>
-__global__ __launch_bounds__(64) void k_vectors_from_schur(int n, int batch, zd *VL, zd *VR,
+__global__ __launch_bounds__(64) void k_vectors_from_schur(int n, int batch, const zd *Tslot, zd *VL, zd *VR,
                                                            const int8_t *__restrict__ flag)
 {
     using G = AdjGeom<NB>;
@@ -121,17 +122,18 @@ __global__ __launch_bounds__(64) void k_vectors_from_schur(int n, int batch, zd 
     const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
     const long long mat = (long long)blockIdx.x * GPW + g, nn = (long long)n * n;
     if (mat >= batch) return;
-    zd *VLm = VL + mat * nn, *VRm = VR + mat * nn, *B1 = lds + g * G::MAT;
+    zd *VLm = VL ? VL + mat * nn : nullptr, *VRm = VR ? VR + mat * nn : nullptr, *B1 = lds + g * G::MAT;
     if (flag[mat] != 0) {
-        fill_nan<LPM>(VLm, (int)nn, t);
-        fill_nan<LPM>(VRm, (int)nn, t);
+        if (VLm) fill_nan<LPM>(VLm, (int)nn, t);
+        if (VRm) fill_nan<LPM>(VRm, (int)nn, t);
         return;
     }
     const FullLayout lay{n | 1};
+    const zd *Tm = Tslot + mat * nn;
     for (int r = t; r < n; r += LPM)
-        for (int c = 0; c < n; ++c) B1[lay(r, c)] = VLm[r + c * n];
+        for (int c = 0; c < n; ++c) B1[lay(r, c)] = Tm[r + c * n];
     lds_order();
-    vectors_from_schur<LPM, true>(B1, lay, VRm, FullLayout{n}, n, t, VLm, VRm);
+    vectors_from_schur<LPM, true>(B1, lay, VRm ? VRm : VLm, FullLayout{n}, n, t, VLm, VRm);
 }
 
 template <int NB, int LPM, bool RECOMPUTE>
@@ -155,7 +157,6 @@ __global__ __launch_bounds__(64) void k_zgeev_adjoint(int n, int batch, const zd
 // Stored-Schur backward for n > 32: one matrix per workgroup of
 // TKZ_ADJ_WG_NT threads (the two LDS buffers allow one matrix per CU, so the
 // extra wavefronts share its work instead of idling the CU).
-
 template <int NT>
 __global__ __launch_bounds__(NT) void k_zgeev_adjoint_wg(int n, const zd *__restrict__ schur,
                                                          const zd *__restrict__ gbar, zd *__restrict__ dA,
@@ -179,9 +180,11 @@ static int adj_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 #define TKZ_SCHUR_CFG(X) X(16, 32, true, false) X(32, 64, true, false) X(64, 64, false, true)
 #define TKZ_SPLIT_NB(X) X(16) X(32) X(64)
 
+// Tslot receives the Hessenberg matrix, then T; Uslot receives Q, then U.
+// With VEC, k_vectors_from_schur follows (eigenvectors into VLv / VRv).
 static hipError_t launch_schur_split(int n, int batch, const zd *A, zd *eig, zd *Tout, long long tstride,
                                      zd *Uout, long long ustride, int8_t *flag, int itmax, hipStream_t s,
-                                     zd *VLv = nullptr, zd *VRv = nullptr)
+                                     bool vec = false, zd *VLv = nullptr, zd *VRv = nullptr)
 {
     switch (adj_nb(n)) {
 #define X(nb)                                                                                                   \
@@ -191,8 +194,9 @@ static hipError_t launch_schur_split(int n, int batch, const zd *A, zd *eig, zd 
                            ustride, flag);                                                                      \
         hipLaunchKernelGGL((k_schur_qr<nb, nb>), grid, dim3(64), 0, s, n, batch, eig, Tout, tstride, Uout,       \
                            ustride, flag, itmax);                                                               \
-        if (VLv)                                                                                                \
-            hipLaunchKernelGGL((k_vectors_from_schur<nb, nb>), grid, dim3(64), 0, s, n, batch, VLv, VRv, flag);  \
+        if (vec)                                                                                                \
+            hipLaunchKernelGGL((k_vectors_from_schur<nb, nb>), grid, dim3(64), 0, s, n, batch, Tout, VLv, VRv,   \
+                               flag);                                                                           \
         return hipGetLastError();                                                                               \
     }
         TKZ_SPLIT_NB(X)
@@ -208,11 +212,14 @@ static hipError_t launch_schur(int n, int batch, const zd *A, zd *eig, zd *schur
     return launch_schur_split(n, batch, A, eig, schur, 2 * nn, schur + nn, 2 * nn, flag, itmax, s);
 }
 
-static hipError_t launch_vectors(int n, int batch, const zd *A, zd *eig, zd *VL, zd *VR, int8_t *flag, int itmax,
-                                 hipStream_t s)
+// Two sides: T parks in VL's slot.  One side: T parks in `work` when given,
+// else the one-kernel form runs (T in LDS).
+static hipError_t launch_vectors(int n, int batch, const zd *A, zd *eig, zd *VL, zd *VR, zd *work, int8_t *flag,
+                                 int itmax, hipStream_t s)
 {
     const long long nn = (long long)n * n;
-    if (VL && VR) return launch_schur_split(n, batch, A, eig, VL, nn, VR, nn, flag, itmax, s, VL, VR);
+    zd *Tslot = (VL && VR) ? VL : work;
+    if (Tslot) return launch_schur_split(n, batch, A, eig, Tslot, nn, VR ? VR : VL, nn, flag, itmax, s, true, VL, VR);
     switch (adj_nb(n)) {
 #define X(nb, lpm, zh, gz)                                                                                        \
     case nb:                                                                                                      \
@@ -293,10 +300,17 @@ extern "C" int tokura_zgeev_batched_backward_gpu(tokurablas_t *handle, int n, in
     return e == hipSuccess ? 0 : 2;
 }
 
+extern "C" size_t tokura_get_zgeev_vectors_worksize(int n, int batch_count)
+{
+    if (!(0 < n && n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE)) return (size_t)-1;
+    if (!(0 < batch_count)) return (size_t)-2;
+    return sizeof(hipDoubleComplex) * (size_t)n * (size_t)n * (size_t)batch_count;
+}
+
 extern "C" int tokura_zgeev_batched_vectors_gpu(tokurablas_t *handle, int n, int batch_count,
                                                 const hipDoubleComplex *A, hipDoubleComplex *eigenvalues,
-                                                hipDoubleComplex *VL, hipDoubleComplex *VR, char *flag,
-                                                hipStream_t stream)
+                                                hipDoubleComplex *VL, hipDoubleComplex *VR, hipDoubleComplex *work,
+                                                char *flag, hipStream_t stream)
 {
     if (handle == nullptr) return -1;
     if (!(n > 0 && n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE)) return -2;
@@ -305,7 +319,7 @@ extern "C" int tokura_zgeev_batched_vectors_gpu(tokurablas_t *handle, int n, int
     if (VL == nullptr && VR == nullptr) return -5;
     if (!(batch_count > 0)) return -6;
     if (flag == nullptr) return -7;
-    hipError_t e = launch_vectors(n, batch_count, (const zd *)A, (zd *)eigenvalues, (zd *)VL, (zd *)VR,
+    hipError_t e = launch_vectors(n, batch_count, (const zd *)A, (zd *)eigenvalues, (zd *)VL, (zd *)VR, (zd *)work,
                                   (int8_t *)flag, handle->max_sweeps, stream);
     return e == hipSuccess ? 0 : 2;
 }
