@@ -19,6 +19,11 @@ Contents
 * ``method_vectors`` — eigenvectors by the same method (ZTREVC on the Schur
   form, back-transform, ZGEEV normalisation); ``lapack_zgeev_vectors`` —
   host LAPACK zgeev('V', 'V') to pin it.
+* ``method_dgeev``  — the C restatement of the real path (dgehd2 + dlahqr
+  + dlanv2, eigenvalues only, no balancing), ``dgeev_method.c``;
+  ``lapack_dgeev`` / ``lapack_dhrd_hseqr`` — host LAPACK dgeev and
+  dgehrd + dhseqr; ``dlanv2`` / ``dlahqr`` — the restated steps, or LAPACK's
+  own routines with ``use_lapack=True``, to pin them.
 * ``match_max_relerr`` — optimal-assignment eigenvalue matching; the
   reference's harness sorts lexicographically instead
   (stream_test_with_MKL/main.cu:16-181), which mis-pairs close eigenvalues.
@@ -96,6 +101,22 @@ def lib():
     L.lapack_zgebal.restype = I
     L.lapack_zgeevx_rcond.argtypes = [I, I, P, P, P, P, I]
     L.lapack_zgeevx_rcond.restype = I
+    L.oracle_dgeev_batched.argtypes = [I, I, P, P, P, P, I, I]
+    L.oracle_dgeev_batched.restype = None
+    L.oracle_dgehd2.argtypes = [I, P]
+    L.oracle_dgehd2.restype = None
+    L.oracle_dlahqr.argtypes = [I, P, I, P, P, I]
+    L.oracle_dlahqr.restype = I
+    L.oracle_dlanv2.argtypes = [P] * 10
+    L.oracle_dlanv2.restype = None
+    for name in ("lapack_dgeev_batched", "lapack_dhrd_hseqr_batched"):
+        f = getattr(L, name)
+        f.argtypes = [I, I, P, P, P, P, I]
+        f.restype = I
+    L.lapack_dlanv2.argtypes = [P, P]
+    L.lapack_dlanv2.restype = I
+    L.lapack_dlahqr.argtypes = [I, P, P, P]
+    L.lapack_dlahqr.restype = I
     rc = L.lapack_open(_lapack_path().encode())
     if rc != 0:
         raise RuntimeError(f"lapack_open failed ({rc})")
@@ -112,6 +133,84 @@ def _colmajor(M: np.ndarray) -> np.ndarray:
 
 def _p(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---- real matrices (DGEEV analogue) ----------------------------------------
+
+def _colmajor_real(M: np.ndarray) -> np.ndarray:
+    M = np.asarray(M, dtype=np.float64)
+    if M.ndim == 2:
+        M = M[None]
+    return np.ascontiguousarray(M.transpose(0, 2, 1))
+
+
+def method_dgeev(M, itmax: int = -1, nthreads: int = 0):
+    """Eigenvalues of real matrices by the restated real method (DGEHD2 +
+    DLAHQR + DLANV2, no balancing): (w = wr + i wi (b, n), info (b,)), info
+    0, the DLAHQR INFO, or -1 for a non-finite input."""
+    A = _colmajor_real(M)
+    b, n, _ = A.shape
+    wr = np.zeros((b, n))
+    wi = np.zeros((b, n))
+    info = np.zeros(b, np.int32)
+    lib().oracle_dgeev_batched(n, b, _p(A), _p(wr), _p(wi), _p(info), nthreads, itmax)
+    return wr + 1j * wi, info
+
+
+def lapack_dgeev(M, nthreads: int = 0):
+    """Host LAPACK dgeev('N', 'N') (balanced, LAPACK default): (w, info)."""
+    A = _colmajor_real(M)
+    b, n, _ = A.shape
+    wr = np.zeros((b, n))
+    wi = np.zeros((b, n))
+    info = np.zeros(b, np.int32)
+    lib().lapack_dgeev_batched(n, b, _p(A), _p(wr), _p(wi), _p(info), nthreads)
+    return wr + 1j * wi, info
+
+
+def lapack_dhrd_hseqr(M, nthreads: int = 0):
+    """Host LAPACK dgehrd + dhseqr('E', 'N'), no balancing: (w, info)."""
+    A = _colmajor_real(M)
+    b, n, _ = A.shape
+    wr = np.zeros((b, n))
+    wi = np.zeros((b, n))
+    info = np.zeros(b, np.int32)
+    lib().lapack_dhrd_hseqr_batched(n, b, _p(A), _p(wr), _p(wi), _p(info), nthreads)
+    return wr + 1j * wi, info
+
+
+def dgehd2(M):
+    """DGEHD2 restatement on one real matrix: H (n x n, zeros below the sub-diagonal)."""
+    A = _colmajor_real(M)[0].copy()
+    lib().oracle_dgehd2(A.shape[0], _p(A))
+    return A.T.copy()
+
+
+def dlahqr(H, use_lapack: bool = False):
+    """Eigenvalues of a real upper Hessenberg matrix by the DLAHQR
+    restatement or by LAPACK's DLAHQR itself: (wr + i wi, info)."""
+    A = _colmajor_real(H)[0].copy()
+    n = A.shape[0]
+    wr = np.zeros(n)
+    wi = np.zeros(n)
+    if use_lapack:
+        info = lib().lapack_dlahqr(n, _p(A), _p(wr), _p(wi))
+    else:
+        info = lib().oracle_dlahqr(n, _p(A), n, _p(wr), _p(wi), -1)
+    return wr + 1j * wi, info
+
+
+def dlanv2(a, b, c, d, use_lapack: bool = False):
+    """DLANV2 restatement (or LAPACK's): returns
+    (a, b, c, d, rt1r, rt1i, rt2r, rt2i, cs, sn) as doubles."""
+    if use_lapack:
+        abcd = np.array([a, b, c, d], np.float64)
+        out = np.zeros(6)
+        lib().lapack_dlanv2(_p(abcd), _p(out))
+        return tuple(abcd) + tuple(out)
+    v = [ctypes.c_double(x) for x in (a, b, c, d)] + [ctypes.c_double() for _ in range(6)]
+    lib().oracle_dlanv2(*[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
 
 
 def method_zgeev(M, itmax: int = -1, with_sweeps: bool = False, balance: str = "N"):

@@ -16,6 +16,10 @@
  *                          the exact method of the GPU path, run by LAPACK
  *   lapack_zgeevx_rcond    eigenvalue condition numbers (RCONDE) of the
  *                          unbalanced matrix, for condition-scaled tolerances
+ *   lapack_dgeev_batched   real dgeev (balanced, LAPACK default): wr, wi
+ *   lapack_dhrd_hseqr_batched  dgehrd + dhseqr('E','N') without balancing
+ *   lapack_dlanv2 / lapack_dlahqr  the LAPACK routines themselves (Fortran
+ *                          interface), to pin the real restatement
  */
 #include <complex.h>
 #include <dlfcn.h>
@@ -30,6 +34,13 @@ typedef int (*zhseqr_t)(int, char, char, int, int, int, zc *, int, zc *, zc *, i
 typedef int (*zgebal_t)(int, char, int, zc *, int, int *, int *, double *);
 typedef int (*zgeevx_t)(int, char, char, char, char, int, zc *, int, zc *, zc *, int, zc *, int,
                         int *, int *, double *, double *, double *, double *);
+typedef int (*dgeev_t)(int, char, char, int, double *, int, double *, double *, double *, int, double *, int);
+typedef int (*dgehrd_t)(int, int, int, int, double *, int, double *);
+typedef int (*dhseqr_t)(int, char, char, int, int, int, double *, int, double *, double *, double *, int);
+typedef void (*dlanv2_t)(double *, double *, double *, double *, double *, double *, double *, double *, double *,
+                         double *);
+typedef void (*dlahqr_t)(const int *, const int *, const int *, const int *, const int *, double *, const int *,
+                         double *, double *, const int *, const int *, double *, const int *, int *);
 typedef void (*setthr_t)(int);
 
 static void *g_lib;
@@ -38,6 +49,11 @@ static zgehrd_t p_zgehrd;
 static zhseqr_t p_zhseqr;
 static zgeevx_t p_zgeevx;
 static zgebal_t p_zgebal;
+static dgeev_t p_dgeev;
+static dgehrd_t p_dgehrd;
+static dhseqr_t p_dhseqr;
+static dlanv2_t p_dlanv2;
+static dlahqr_t p_dlahqr;
 
 #define COLMAJOR 102
 
@@ -51,9 +67,15 @@ int lapack_open(const char *path)
     p_zhseqr = (zhseqr_t)dlsym(g_lib, "scipy_LAPACKE_zhseqr");
     p_zgeevx = (zgeevx_t)dlsym(g_lib, "scipy_LAPACKE_zgeevx");
     p_zgebal = (zgebal_t)dlsym(g_lib, "scipy_LAPACKE_zgebal");
+    p_dgeev = (dgeev_t)dlsym(g_lib, "scipy_LAPACKE_dgeev");
+    p_dgehrd = (dgehrd_t)dlsym(g_lib, "scipy_LAPACKE_dgehrd");
+    p_dhseqr = (dhseqr_t)dlsym(g_lib, "scipy_LAPACKE_dhseqr");
+    p_dlanv2 = (dlanv2_t)dlsym(g_lib, "scipy_dlanv2_");
+    p_dlahqr = (dlahqr_t)dlsym(g_lib, "scipy_dlahqr_");
     setthr_t st = (setthr_t)dlsym(g_lib, "scipy_openblas_set_num_threads");
     if (st) st(1);
     if (!p_zgeev || !p_zgehrd || !p_zhseqr || !p_zgeevx || !p_zgebal) return -2;
+    if (!p_dgeev || !p_dgehrd || !p_dhseqr || !p_dlanv2 || !p_dlahqr) return -2;
     return 0;
 }
 
@@ -152,4 +174,65 @@ int lapack_zgebal(char job, int n, zc *a, int *ilo, int *ihi, double *scale)
 {
     if (!p_zgebal) return -1;
     return p_zgebal(COLMAJOR, job, n, a, n, ilo, ihi, scale);
+}
+
+/* wr/wi[i + j*n]; info[j] = LAPACK INFO. */
+int lapack_dgeev_batched(int n, int batch, const double *A, double *wr, double *wi, int *info, int nthreads)
+{
+    if (!p_dgeev) return -1;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    #pragma omp parallel num_threads(nthreads)
+    {
+        double *a = (double *)malloc(sizeof(double) * (size_t)n * n);
+        #pragma omp for schedule(dynamic, 8)
+        for (int b = 0; b < batch; ++b) {
+            memcpy(a, A + (size_t)b * n * n, sizeof(double) * (size_t)n * n);
+            int r = p_dgeev(COLMAJOR, 'N', 'N', n, a, n, wr + (size_t)b * n, wi + (size_t)b * n, NULL, 1, NULL, 1);
+            if (info) info[b] = r;
+        }
+        free(a);
+    }
+    return 0;
+}
+
+int lapack_dhrd_hseqr_batched(int n, int batch, const double *A, double *wr, double *wi, int *info, int nthreads)
+{
+    if (!p_dgehrd) return -1;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    #pragma omp parallel num_threads(nthreads)
+    {
+        double *a = (double *)malloc(sizeof(double) * (size_t)n * n);
+        double *tau = (double *)malloc(sizeof(double) * (size_t)(n > 1 ? n : 1));
+        double zdummy = 0;
+        #pragma omp for schedule(dynamic, 8)
+        for (int b = 0; b < batch; ++b) {
+            memcpy(a, A + (size_t)b * n * n, sizeof(double) * (size_t)n * n);
+            int r = p_dgehrd(COLMAJOR, n, 1, n, a, n, tau);
+            if (r == 0)
+                r = p_dhseqr(COLMAJOR, 'E', 'N', n, 1, n, a, n, wr + (size_t)b * n, wi + (size_t)b * n, &zdummy, 1);
+            if (info) info[b] = r;
+        }
+        free(a); free(tau);
+    }
+    return 0;
+}
+
+/* LAPACK DLANV2 on (a, b, c, d) in place; out = rt1r, rt1i, rt2r, rt2i, cs, sn. */
+int lapack_dlanv2(double *abcd, double *out)
+{
+    if (!p_dlanv2) return -1;
+    p_dlanv2(&abcd[0], &abcd[1], &abcd[2], &abcd[3], &out[0], &out[1], &out[2], &out[3], &out[4], &out[5]);
+    return 0;
+}
+
+/* LAPACK DLAHQR (WANTT = WANTZ = false, ILO = 1, IHI = n) on an upper
+ * Hessenberg H (column-major, ld n, overwritten); returns INFO. */
+int lapack_dlahqr(int n, double *H, double *wr, double *wi)
+{
+    if (!p_dlahqr) return -1;
+    const int f = 0, one = 1;
+    double zd = 0.0;
+    int info = 0;
+    p_dlahqr(&f, &f, &n, &one, &n, H, &n, wr, wi, &one, &n, &zd, &one, &info);
+    return info;
 }

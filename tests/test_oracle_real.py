@@ -1,0 +1,157 @@
+"""Pins the oracle's restatement of the real path (DGEHD2 + DLAHQR + DLANV2,
+oracle/dgeev_method.c) before it is trusted as the checker of the GPU's real
+kernels:
+
+* DLANV2 bitwise against LAPACK's own DLANV2 (random and special 2x2
+  blocks: c = 0, b = 0, equal diagonal with opposite off-diagonal signs,
+  nearly equal real eigenvalues, huge and tiny scales);
+* DLAHQR against LAPACK's own DLAHQR on Hessenberg inputs (<= 1e-10
+  relative per eigenvalue; rounding differs only through DNRM2);
+* the whole method against LAPACK dgeev and dgehrd + dhseqr on seeded
+  batches and the committed golden vectors (tests/golden/dgeev_golden.npz),
+  <= 1e-10 after optimal matching;
+* closed forms: triangular input (the diagonal, no sweep), rotations
+  (cos t +- i sin t), the companion of x^n - 1 (roots of unity), symmetric
+  input (eigvalsh), the 2x2 quadratic formula;
+* structure: complex eigenvalues come in exact conjugate pairs, positive
+  imaginary part first, equal real parts;
+* flags: a non-finite input gives info -1, a sweep cap of 0 gives info > 0.
+A plausible slip (a sign in the double-shift vector, a wrong index in the
+3x3 reflector, a dropped exceptional shift) breaks the LAPACK comparisons.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dgeev_golden.npz")
+
+
+def test_dlanv2_bitwise_vs_lapack():
+    rng = np.random.default_rng(70)
+    cases = []
+    for k in range(4000):
+        a, b, c, d = rng.standard_normal(4) * (10.0 ** rng.integers(-8, 9, 4))
+        cases.append((a, b, c, d))
+    cases += [(1.0, 2.0, 0.0, 3.0), (1.0, 0.0, 2.0, 3.0), (2.0, 1.0, -1.0, 2.0), (2.0, -1.0, 1.0, 2.0),
+              (1.0, 1e-20, 1e-20, 1.0 + 1e-17), (0.0, 1e300, -1e-300, 0.0), (5.0, 3.0, 3.0, 5.0),
+              (1e-300, 1e-300, 1e-300, -1e-300), (0.0, 0.0, 0.0, 0.0), (-0.0, 1.0, -1.0, 0.0)]
+    for a, b, c, d in cases:
+        mine = oracle.dlanv2(a, b, c, d)
+        ref = oracle.dlanv2(a, b, c, d, use_lapack=True)
+        assert mine == ref, ((a, b, c, d), mine, ref)
+
+
+def test_dlahqr_vs_lapack_dlahqr():
+    rng = np.random.default_rng(71)
+    for n in (2, 3, 4, 7, 16, 31, 50, 64):
+        for _ in range(10):
+            H = oracle.dgehd2(rng.standard_normal((n, n)))
+            w1, i1 = oracle.dlahqr(H)
+            w2, i2 = oracle.dlahqr(H, use_lapack=True)
+            assert i1 == 0 and i2 == 0
+            assert oracle.max_relerr(w2[None], w1[None]) <= 1e-10
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 13, 16, 24, 33, 48, 64])
+def test_method_vs_lapack(n):
+    rng = np.random.default_rng(700 + n)
+    M = rng.standard_normal((40, n, n))
+    w, info = oracle.method_dgeev(M)
+    assert not info.any()
+    wl, il = oracle.lapack_dgeev(M)
+    wh, ih = oracle.lapack_dhrd_hseqr(M)
+    assert oracle.max_relerr(wl, w) <= 1e-10
+    assert oracle.max_relerr(wh, w) <= 1e-10
+
+
+def test_golden_vectors():
+    g = np.load(GOLD)
+    for n in g["sizes"]:
+        M = g[f"A_{n}"]
+        w, info = oracle.method_dgeev(M)
+        assert not info.any()
+        assert oracle.max_relerr(g[f"eig_dgeev_{n}"], w) <= 1e-10
+        assert oracle.max_relerr(g[f"eig_hseqr_{n}"], w) <= 1e-10
+        wl, _ = oracle.lapack_dgeev(M)           # the fixture reproduces with this LAPACK
+        assert oracle.max_relerr(g[f"eig_dgeev_{n}"], wl) <= 1e-13
+
+
+def test_conjugate_pairs_exact():
+    rng = np.random.default_rng(72)
+    for n in (4, 9, 20, 64):
+        w, info = oracle.method_dgeev(rng.standard_normal((20, n, n)))
+        for row in w:
+            j = 0
+            while j < n:
+                if row[j].imag != 0.0:
+                    assert row[j].imag > 0 and row[j + 1].imag == -row[j].imag and row[j + 1].real == row[j].real
+                    j += 2
+                else:
+                    j += 1
+
+
+def test_closed_forms():
+    rng = np.random.default_rng(73)
+    # triangular: the diagonal, exactly
+    T = np.triu(rng.standard_normal((30, 30)))
+    w, info = oracle.method_dgeev(T)
+    assert not info.any()
+    np.testing.assert_array_equal(np.sort(w[0].real), np.sort(np.diag(T)))
+    assert not w[0].imag.any()
+    # block diagonal of rotations r (cos t, sin t): r e^{+-i t}
+    ts = rng.uniform(0.1, 3.0, 10)
+    rs = rng.uniform(0.5, 2.0, 10)
+    R = np.zeros((20, 20))
+    for k, (t, r) in enumerate(zip(ts, rs)):
+        R[2 * k:2 * k + 2, 2 * k:2 * k + 2] = r * np.array([[np.cos(t), -np.sin(t)], [np.sin(t), np.cos(t)]])
+    Q, _ = np.linalg.qr(rng.standard_normal((20, 20)))
+    w, info = oracle.method_dgeev(Q @ R @ Q.T)
+    exact = np.concatenate([rs * np.exp(1j * ts), rs * np.exp(-1j * ts)])
+    assert oracle.max_relerr(exact[None], w) <= 1e-12
+    # companion of x^n - 1: roots of unity
+    for n in (5, 16, 40):
+        C = np.zeros((n, n))
+        C[1:, :-1] = np.eye(n - 1)
+        C[0, -1] = 1.0
+        w, info = oracle.method_dgeev(C)
+        assert not info.any()
+        assert oracle.max_relerr(np.exp(2j * np.pi * np.arange(n) / n)[None], w) <= 1e-11
+    # symmetric: real eigenvalues = eigvalsh
+    S = rng.standard_normal((25, 25))
+    S = S + S.T
+    w, info = oracle.method_dgeev(S)
+    assert np.abs(w.imag).max() == 0.0
+    assert oracle.max_relerr(np.linalg.eigvalsh(S)[None].astype(complex), w) <= 1e-12
+    # 2x2 quadratic formula
+    for _ in range(200):
+        a, b, c, d = rng.standard_normal(4)
+        disc = np.sqrt(complex((a - d) ** 2 + 4 * b * c))
+        exact = np.array([(a + d + disc) / 2, (a + d - disc) / 2])
+        w, info = oracle.method_dgeev(np.array([[a, b], [c, d]]))
+        assert oracle.max_relerr(exact[None], w) <= 1e-13
+
+
+def test_hessenberg_invariants():
+    rng = np.random.default_rng(74)
+    M = rng.standard_normal((40, 40))
+    H = oracle.dgehd2(M)
+    assert np.abs(np.tril(H, -2)).max() == 0.0
+    assert abs(np.trace(H) - np.trace(M)) <= 1e-12 * np.abs(M).sum()
+    assert abs(np.linalg.norm(H) - np.linalg.norm(M)) <= 1e-12 * np.linalg.norm(M)
+    wh, _ = oracle.dlahqr(H, use_lapack=True)
+    wm, _ = oracle.lapack_dgeev(M)
+    assert oracle.max_relerr(wm, wh[None]) <= 1e-10
+
+
+def test_flags():
+    rng = np.random.default_rng(75)
+    M = rng.standard_normal((3, 10, 10))
+    M[1, 3, 4] = np.nan
+    w, info = oracle.method_dgeev(M)
+    assert list(info) == [0, -1, 0]
+    assert np.isnan(w[1].real).all()
+    w, info = oracle.method_dgeev(rng.standard_normal((2, 10, 10)), itmax=0)
+    assert (info > 0).all()
