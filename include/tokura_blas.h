@@ -229,6 +229,20 @@ int tokura_zgeev_batched_vectors_gpu(tokurablas_t *handle, int n, int batch_coun
                                      hipDoubleComplex *VL, hipDoubleComplex *VR, hipDoubleComplex *work,
                                      char *flag, hipStream_t stream);
 
+/* ---- real matrices (LAPACK DGEEV analogue, eigenvalues only) ------------
+ * No counterpart in the reference, which solves complex matrices only.  A
+ * holds batch_count real n x n matrices (double), column-major, matrix after
+ * matrix, and is not modified.  Eigenvalue i of matrix j is
+ * wr[i + j*n] + i*wi[i + j*n]; complex conjugate pairs are consecutive,
+ * positive imaginary part first, with equal real parts (as DGEEV).  Method:
+ * Householder Hessenberg (DGEHD2) + Francis double-shift QR (DLAHQR, DLANV2
+ * for 2x2 blocks), no balancing.  work: tokura_get_dgeev_worksize(n, batch)
+ * bytes; flag and return codes as tokura_zgeev_batched_gpu (-4: wr or wi
+ * NULL).  The sweep cap of tokuraSetMaxSweeps applies. */
+size_t tokura_get_dgeev_worksize(int n, int batch_count);
+int tokura_dgeev_batched_gpu(tokurablas_t *handle, int n, int batch_count, const double *A, double *wr, double *wi,
+                             double *work, char *flag, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

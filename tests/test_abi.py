@@ -135,3 +135,28 @@ def test_balancing_switch():
     assert L.tokuraSetBalancing(h, b"X") == -2
     assert L.tokuraSetBalancing(None, b"B") == -1
     assert L.tokuraDestroy(h) == 0
+
+
+def test_real_entry_validation_and_workspace():
+    L = tk.lib()
+    h = make_handle()
+    P = ctypes.c_void_p
+    a, wr, wi, w, f = P(0x1000), P(0x2000), P(0x2800), P(0x3000), P(0x4000)
+    s = P(0)
+    g = L.tokura_dgeev_batched_gpu
+    assert g(None, 4, 1, a, wr, wi, w, f, s) == -1
+    assert g(h, 0, 1, a, wr, wi, w, f, s) == -2
+    assert g(h, 65, 1, a, wr, wi, w, f, s) == -2
+    assert g(h, 4, 1, None, wr, wi, w, f, s) == -3
+    assert g(h, 4, 1, a, None, wi, w, f, s) == -4
+    assert g(h, 4, 1, a, wr, None, w, f, s) == -4
+    assert g(h, 4, 1, a, wr, wi, None, f, s) == -5
+    assert g(h, 4, 0, a, wr, wi, w, f, s) == -6
+    assert g(h, 4, 1, a, wr, wi, w, None, s) == -7
+    assert L.tokuraDestroy(h) == 0
+    ws = L.tokura_get_dgeev_worksize
+    # packed upper Hessenberg, (n-1)(n+2)/2 + n doubles per matrix
+    assert ws(64, 1000) == 8 * (63 * 66 // 2 + 64) * 1000
+    assert ws(1, 3) == 8 * 3
+    M = (1 << 64) - 1
+    assert ws(0, 5) == M and ws(65, 5) == M and ws(4, 0) == M - 1

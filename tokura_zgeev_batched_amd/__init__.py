@@ -67,6 +67,8 @@ _SIGNATURES = {
     "tokura_get_zgeev_ragged_schur_worksize": ([_I, _P], _SZ),
     "tokura_zgeev_ragged_schur_gpu": ([_P, _I, _P, _P, _P, _P, _P, _P], _I),
     "tokura_zgeev_ragged_backward_gpu": ([_P, _I, _P, _P, _P, _P, _P, _P], _I),
+    "tokura_get_dgeev_worksize": ([_I, _I], _SZ),
+    "tokura_dgeev_batched_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P, _P], _I),
 }
 
 
@@ -219,6 +221,41 @@ def eigvals(M: torch.Tensor, handle: Handle | None = None, colmajor: bool = Fals
     work = torch.empty((workspace_size(n, B),), dtype=torch.uint8, device=M.device)
     _check(zgeev_batched(A, n, B, eig, work, flag, handle, stream), "tokura_zgeev_batched_gpu")
     return eig, flag
+
+
+def dgeev_workspace_size(n: int, batch: int) -> int:
+    return int(lib().tokura_get_dgeev_worksize(n, batch))
+
+
+def dgeev_batched(A_colmajor: torch.Tensor, n: int, batch: int, wr: torch.Tensor, wi: torch.Tensor,
+                  work: torch.Tensor, flag: torch.Tensor, handle: Handle | None = None, stream=None) -> int:
+    """Raw tokura_dgeev_batched_gpu: A batch*n*n float64 column-major per
+    matrix; wr, wi batch*n float64; work dgeev_workspace_size(n, batch) bytes;
+    flag batch int8.  Returns the C code."""
+    h = handle or default_handle()
+    return int(lib().tokura_dgeev_batched_gpu(h.ptr, n, batch, _P(A_colmajor.data_ptr()), _P(wr.data_ptr()),
+                                               _P(wi.data_ptr()), _P(work.data_ptr()), _P(flag.data_ptr()),
+                                               _stream_ptr(stream)))
+
+
+def eigvals_real(M: torch.Tensor, handle: Handle | None = None, stream=None):
+    """Eigenvalues of a (B, n, n) float64 device tensor (the DGEEV analogue).
+
+    Returns (w (B, n) complex128, flag (B,) int8): complex conjugate pairs
+    are consecutive, positive imaginary part first."""
+    _dev_check(M, "M", torch.float64)
+    if M.dim() == 2:
+        M = M.unsqueeze(0)
+    B, n, n2 = M.shape
+    if n != n2 or not (1 <= n <= MAX_N):
+        raise TokuraError(f"matrices must be n x n with 1 <= n <= {MAX_N}, got {tuple(M.shape)}")
+    A = M.transpose(-1, -2).contiguous()
+    wr = torch.empty((B, n), dtype=torch.float64, device=M.device)
+    wi = torch.empty((B, n), dtype=torch.float64, device=M.device)
+    flag = torch.empty((B,), dtype=torch.int8, device=M.device)
+    work = torch.empty((dgeev_workspace_size(n, B),), dtype=torch.uint8, device=M.device)
+    _check(dgeev_batched(A, n, B, wr, wi, work, flag, handle, stream), "tokura_dgeev_batched_gpu")
+    return torch.complex(wr, wi), flag
 
 
 def zgeev_vectors(A_colmajor: torch.Tensor, n: int, batch: int, eig: torch.Tensor, VL: torch.Tensor | None,
