@@ -310,6 +310,58 @@ def bench_vectors(dev, cases=((32, 50_000), (64, 20_000)), steps=3, seed=11):
     return out
 
 
+def bench_real(dev, cases=((64, 100_000), (32, 100_000), (16, 100_000)), steps=3, seed=19, cpu_seconds=4.0):
+    """Real matrices (tokura_dgeev_batched_gpu, the DGEEV analogue): N(0,1)
+    entries, eigenvalues only; accuracy vs host LAPACK dgeev on a sample and
+    a bounded host LAPACK dgeev baseline (16 threads) at each n."""
+    import oracle
+    out = {"workload": "real matrices, same-size batches, eigenvalues only (DGEEV analogue)"}
+    h = tk.Handle()
+    for n, B in cases:
+        g = torch.Generator(device=dev).manual_seed(seed)
+        M = torch.randn(B, n, n, generator=g, device=dev, dtype=torch.float64)
+        A = M.transpose(-1, -2).contiguous()
+        wr = torch.empty((B, n), dtype=torch.float64, device=dev)
+        wi = torch.empty_like(wr)
+        flag = torch.empty(B, dtype=torch.int8, device=dev)
+        work = torch.empty(tk.dgeev_workspace_size(n, B), dtype=torch.uint8, device=dev)
+
+        def step():
+            rc = tk.dgeev_batched(A, n, B, wr, wi, work, flag, h)
+            if rc != 0:
+                raise RuntimeError(f"tokura_dgeev_batched_gpu returned {rc}")
+
+        t = timed(step, steps, 1)
+        k = min(B, 2000)
+        Ms = M[:k].cpu().numpy()
+        w_gpu = (wr[:k] + 1j * wi[:k]).cpu().numpy()
+        t0 = time.perf_counter()
+        done = 0
+        wl_parts = []
+        while done < k and time.perf_counter() - t0 < cpu_seconds:
+            wl, _ = oracle.lapack_dgeev(Ms[done:done + 500])
+            wl_parts.append(wl)
+            done += wl.shape[0]
+        cpu_rate = done / (time.perf_counter() - t0)
+        wl_all = np.concatenate(wl_parts)
+        err = oracle.max_relerr(wl_all, w_gpu[:done])
+        # real Gaussian matrices have real eigenvalues near 0, where any
+        # backward-stable method's relative error grows: also in units of
+        # kappa_i eps ||A||_F and relative to ||A||_F
+        ks = min(done, 300)
+        cs = float(oracle.condscaled_err(Ms[:ks], w_gpu[:ks]).max())
+        na = max(float(np.abs(w_gpu[b][oracle.match(wl_all[b], w_gpu[b])] - wl_all[b]).max()
+                       / np.linalg.norm(Ms[b])) for b in range(done))
+        out[f"n{n}_b{B}"] = {"matrices_per_s": B / t, "ms": t * 1e3, "flags_nonzero": int((flag != 0).sum()),
+                             "max_rel_err_vs_lapack_dgeev": err, "condition_scaled_err_max": cs,
+                             "max_abs_err_over_norm_A": na, "accuracy_sample": done,
+                             "cpu_lapack_dgeev_matrices_per_s": cpu_rate, "cpu_threads": oracle.max_threads(),
+                             "model_tflops_10n3": 10.0 * n ** 3 * B / t / 1e12}
+        del M, A, wr, wi, flag, work
+        torch.cuda.empty_cache()
+    return out
+
+
 def bench_small(dev, cases=((16, 10_000), (16, 200_000), (32, 100_000)), steps=5, seed=13):
     """Plain same-size path at small n (the reference's CPU-runnable case is
     n = 16 x 10,000), autotuned per n, accuracy vs host LAPACK zgeev."""
@@ -343,7 +395,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the ragged and differentiation legs")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch table")
-    ap.add_argument("--only", choices=["ragged", "diff", "hard", "e2e", "vectors", "small"], default=None,
+    ap.add_argument("--only", choices=["ragged", "diff", "hard", "e2e", "vectors", "small", "real"], default=None,
                     help="run only this extra leg (for profiling); prints its JSON object")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates the "
@@ -358,7 +410,7 @@ def main():
     dev = torch.device("cuda:0")
     if args.only:
         fn = {"ragged": bench_ragged, "diff": bench_diff, "hard": bench_hard, "e2e": bench_end_to_end,
-              "vectors": bench_vectors, "small": bench_small}[args.only]
+              "vectors": bench_vectors, "small": bench_small, "real": bench_real}[args.only]
         print(json.dumps({args.only: fn(dev)}), flush=True)
         return
     n, B = args.n, args.batch
@@ -450,7 +502,8 @@ def main():
         torch.cuda.empty_cache()
         result["extra"] = {"ragged": bench_ragged(dev), "differentiation": bench_diff(dev),
                            "hard_spectra": bench_hard(dev), "end_to_end": bench_end_to_end(dev),
-                           "eigenvectors": bench_vectors(dev), "small_n": bench_small(dev)}
+                           "eigenvectors": bench_vectors(dev), "small_n": bench_small(dev),
+                           "real_matrices": bench_real(dev)}
     print(json.dumps(result), flush=True)
 
 

@@ -6,8 +6,9 @@ matching against the oracle's restatement and against host LAPACK dgeev;
 complex eigenvalues in exact conjugate pairs (positive imaginary part first,
 equal real parts); closed forms (triangular input, rotations, roots of
 unity, symmetric input); flags for non-finite input and the sweep cap; at
-the full 100,000 x 64 size the trace invariant sum(wr) = trace(A) and a
-LAPACK sample.
+the full 100,000 x 64 size the trace invariant sum(wr) = trace(A) and, on a
+sample, the condition-scaled error (<= 100 units of kappa_i eps ||A||_F) and
+the error relative to ||A||_F (<= 1e-13) against LAPACK dgeev.
 """
 import numpy as np
 import pytest
@@ -115,5 +116,12 @@ def test_real_full_size_invariants():
     idx = torch.arange(0, B, 499)
     ws = w[idx].cpu().numpy()
     check_pairs(ws)                                          # exact conjugate pairs
-    wl, _ = oracle.lapack_dgeev(M[idx].cpu().numpy())
-    assert oracle.max_relerr(wl, ws) <= TOL
+    Ms = M[idx].cpu().numpy()
+    # real Gaussian matrices have real eigenvalues arbitrarily close to 0,
+    # where the relative error of any backward-stable method grows: judged in
+    # units of the first-order bound kappa_i eps ||A||_F (as hard spectra)
+    assert float(oracle.condscaled_err(Ms, ws).max()) <= 100.0
+    wl, _ = oracle.lapack_dgeev(Ms)
+    for b in range(len(Ms)):
+        g = ws[b][oracle.match(wl[b], ws[b])]
+        assert np.abs(g - wl[b]).max() <= 1e-13 * np.linalg.norm(Ms[b])

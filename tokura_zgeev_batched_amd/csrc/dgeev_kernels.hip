@@ -3,7 +3,8 @@
 // counterpart in the reference, which solves complex matrices only.
 //
 // Two launches, as the complex split path:
-//  1. k_dhess  loads each matrix into LDS (full layout), checks it is finite,
+//  1. k_dhess (k_dhess_wg for n > 32: a 256-thread workgroup per matrix)
+//              loads each matrix into LDS (full layout), checks it is finite,
 //              reduces it to upper Hessenberg form (DGEHD2) and writes the
 //              packed Hessenberg matrix (column c: rows 0..c+1) to `work`;
 //  2. k_dqr    loads the packed matrix into a BulgeLayout LDS buffer (two
@@ -45,6 +46,36 @@ __global__ __launch_bounds__(64) void k_dhess(int n, int batch, const double *__
         for (int r = t; r < rows; r += LPM) dst[pk(r, c)] = H[lay(r, c)];
     }
     if (t == 0) flag[mat] = ok ? 0 : 2;
+}
+
+// Stage 1 for n > 32: one matrix per workgroup of NT threads (dhessenberg_wg).
+template <int NB, int NT>
+__global__ __launch_bounds__(NT) void k_dhess_wg(int n, int /*batch: one workgroup per matrix*/, const double *__restrict__ A,
+                                                 double *__restrict__ work, int8_t *__restrict__ flag)
+{
+    constexpr int LDP = NB | 1;
+    __shared__ double lds[NB * LDP];
+    __shared__ double red[2];
+    const long long mat = blockIdx.x;
+    const int tid = (int)threadIdx.x;
+    const FullLayout lay{n | 1};
+    const int nn = n * n;
+    const double *src = A + mat * nn;
+    bool bad = false;
+    for (int e = tid; e < nn; e += NT) {
+        const double v = src[e];
+        bad |= !isfinite(v);
+        lds[lay(e % n, e / n)] = v;
+    }
+    const int ok = __syncthreads_or(bad) == 0;
+    if (ok) dhessenberg_wg<NT, NT / 64>(lds, lay, n, red);
+    double *dst = work + mat * (long long)packed_size(n);
+    const PackedLayout pk;
+    for (int c = 0; c < n; ++c) {
+        const int rows = c + 2 < n ? c + 2 : n;
+        for (int r = tid; r < rows; r += NT) dst[pk(r, c)] = lds[lay(r, c)];
+    }
+    if (tid == 0) flag[mat] = ok ? 0 : 2;
 }
 
 template <int NB>
@@ -90,7 +121,10 @@ static hipError_t launch_dgeev(int n, int batch, const double *A, double *wr, do
     case nb: {                                                                                             \
         const dim3 grid((batch + 64 / nb - 1) / (64 / nb));                                                \
         *stage = 1;                                                                                        \
-        hipLaunchKernelGGL((k_dhess<nb>), grid, dim3(64), 0, s, n, batch, A, work, flag);                  \
+        if (nb == 64)                                                                                      \
+            hipLaunchKernelGGL((k_dhess_wg<64, 256>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag); \
+        else                                                                                               \
+            hipLaunchKernelGGL((k_dhess<nb>), grid, dim3(64), 0, s, n, batch, A, work, flag);              \
         hipError_t e = hipGetLastError();                                                                  \
         if (e != hipSuccess) return e;                                                                     \
         *stage = 2;                                                                                        \
