@@ -8,7 +8,9 @@ equal real parts); closed forms (triangular input, rotations, roots of
 unity, symmetric input); flags for non-finite input and the sweep cap; at
 the full 100,000 x 64 size the trace invariant sum(wr) = trace(A) and, on a
 sample, the condition-scaled error (<= 100 units of kappa_i eps ||A||_F) and
-the error relative to ||A||_F (<= 1e-13) against LAPACK dgeev.
+the error relative to ||A||_F (<= 1e-13) against LAPACK dgeev; hard spectra
+(real companions, clustered ill-conditioned real spectra) with the
+condition-scaled error <= 100 and no flag.
 """
 import numpy as np
 import pytest
@@ -16,6 +18,7 @@ import torch
 
 import oracle
 import tokura_zgeev_batched_amd as tk
+from tokura_zgeev_batched_amd.workloads import hard_batch_real
 
 pytestmark = pytest.mark.gpu
 
@@ -125,3 +128,16 @@ def test_real_full_size_invariants():
     for b in range(len(Ms)):
         g = ws[b][oracle.match(wl[b], ws[b])]
         assert np.abs(g - wl[b]).max() <= 1e-13 * np.linalg.norm(Ms[b])
+
+
+@pytest.mark.parametrize("n", [8, 17, 33, 50, 64])
+def test_real_hard_spectra(n):
+    M, kinds = hard_batch_real(200, n, 410 + n)
+    w, f = eigs(M)
+    assert not f.any()
+    check_pairs(w)
+    err = oracle.condscaled_err(M, w).max(axis=1)
+    print(f"real hard n={n}: companion {err[0::2].max():.3g} clustered {err[1::2].max():.3g}", flush=True)
+    assert float(err.max()) <= 100.0
+    wo, _ = oracle.method_dgeev(M)
+    assert float(oracle.condscaled_err(M, wo).max()) <= 100.0

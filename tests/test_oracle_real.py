@@ -15,7 +15,9 @@ kernels:
   input (eigvalsh), the 2x2 quadratic formula;
 * structure: complex eigenvalues come in exact conjugate pairs, positive
   imaginary part first, equal real parts;
-* flags: a non-finite input gives info -1, a sweep cap of 0 gives info > 0.
+* flags: a non-finite input gives info -1, a sweep cap of 0 gives info > 0;
+* hard spectra (real companions, clustered ill-conditioned real spectra):
+  condition-scaled error <= 100 against LAPACK zgeevx, as the GPU bar.
 A plausible slip (a sign in the double-shift vector, a wrong index in the
 3x3 reflector, a dropped exceptional shift) breaks the LAPACK comparisons.
 """
@@ -155,3 +157,18 @@ def test_flags():
     assert np.isnan(w[1].real).all()
     w, info = oracle.method_dgeev(rng.standard_normal((2, 10, 10)), itmax=0)
     assert (info > 0).all()
+
+
+@pytest.mark.parametrize("n", [8, 33, 64])
+def test_hard_spectra(n):
+    from tokura_zgeev_batched_amd.workloads import hard_batch_real
+    M, kinds = hard_batch_real(30, n, 310 + n)
+    w, info = oracle.method_dgeev(M)
+    assert not info.any()
+    err = oracle.condscaled_err(M, w).max(axis=1)
+    assert float(err.max()) <= 100.0
+    # the companions' polynomials: the monic polynomial with the computed roots
+    # reproduces the coefficients to the conditioning of the problem
+    for b in range(0, len(M), 2):
+        coeffs = np.poly(w[b])
+        assert np.abs(coeffs.imag).max() <= 1e-8 * np.abs(coeffs).max()

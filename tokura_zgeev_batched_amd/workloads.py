@@ -100,3 +100,48 @@ def hard_batch(B: int, lo: int, hi: int, seed: int, cond: float = 1e4, spread: f
             for j, b in enumerate(clus):
                 mats[b] = A[j]
     return mats, kinds
+
+
+def _orthogonal_batch(k: int, n: int, rng) -> np.ndarray:
+    Q, R = np.linalg.qr(rng.standard_normal((k, n, n)))
+    return Q * np.sign(np.diagonal(R, axis1=1, axis2=2))[:, None, :]
+
+
+def hard_batch_real(B: int, n: int, seed: int, cond: float = 1e4, spread: float = 1e-3):
+    """Real hard spectra, B matrices of size n: even indices are companions of
+    monic polynomials with iid real N(0,1) coefficients; odd indices
+    V blockdiag(...) V^-1 with a real orthogonal-times-log-spaced V
+    (kappa_2(V) = cond) and a real-Schur-form core whose spectrum sits in
+    three clusters of the given spread: one on the real axis around a real
+    centre and one conjugate pair of clusters around c +- i d (2x2 blocks
+    [[a, b], [-b, a]] with eigenvalues a +- i b).  Returns (M[B, n, n], kinds)."""
+    rng = np.random.default_rng(seed)
+    M = np.zeros((B, n, n))
+    kinds = ["companion" if b % 2 == 0 else "clustered" for b in range(B)]
+    comp = np.arange(0, B, 2)
+    clus = np.arange(1, B, 2)
+    if n > 1:
+        M[comp[:, None], np.arange(1, n)[None, :], np.arange(n - 1)[None, :]] = 1.0
+    M[comp, :, -1] = -rng.standard_normal((len(comp), n))
+    k = len(clus)
+    if k:
+        D = np.zeros((k, n, n))
+        npair = n // 3                       # conjugate pairs use 2 * npair slots
+        nreal = n - 2 * npair
+        creal = rng.standard_normal(k)
+        cre, cim = rng.standard_normal(k), np.abs(rng.standard_normal(k)) + 0.5
+        lam_r = creal[:, None] + spread * rng.standard_normal((k, nreal))
+        D[:, np.arange(nreal), np.arange(nreal)] = lam_r
+        a = cre[:, None] + spread * rng.standard_normal((k, npair))
+        b = cim[:, None] + spread * rng.standard_normal((k, npair))
+        r0 = nreal + 2 * np.arange(npair)
+        D[:, r0, r0] = a
+        D[:, r0 + 1, r0 + 1] = a
+        D[:, r0, r0 + 1] = b
+        D[:, r0 + 1, r0] = -b
+        U, W = _orthogonal_batch(k, n, rng), _orthogonal_batch(k, n, rng)
+        sig = np.logspace(0, np.log10(cond), n) if n > 1 else np.ones(1)
+        V = (U * sig[None, None, :]) @ W.transpose(0, 2, 1)
+        Vinv = (W * (1.0 / sig)[None, None, :]) @ U.transpose(0, 2, 1)
+        M[clus] = V @ D @ Vinv
+    return M, kinds
