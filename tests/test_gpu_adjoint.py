@@ -22,7 +22,7 @@ def rand(rng, b, n):
     return rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n))
 
 
-@pytest.mark.parametrize("n", [1, 2, 5, 16, 17, 32, 33, 64])
+@pytest.mark.parametrize("n", [1, 2, 5, 16, 17, 18, 32, 33, 40, 64])
 def test_schur_forward(n):
     rng = np.random.default_rng(300 + n)
     M = rand(rng, 20, n)

@@ -1,5 +1,6 @@
-"""GPU parity of the two-bulge QR variants (ids 19: NB 48, 20: NB 64; one
-matrix per wavefront, two shifts per sweep pipelined in one sweep).
+"""GPU parity of the two-bulge QR variants (ids 19: NB 48, 20: NB 64, one
+matrix per wavefront; id 21: NB 32, two matrices per wavefront with a
+16-lane chain per bulge; two shifts per sweep pipelined in one sweep).
 
 Bars as the plain path: max relative error <= 1e-10 against host LAPACK
 zgeev after optimal matching; condition-scaled error <= 100 on hard spectra
@@ -19,15 +20,17 @@ pytestmark = pytest.mark.gpu
 
 TOL_LAPACK = 1e-10
 DUAL = {48: 19, 64: 20}
+DUAL32 = 21
+DUAL_1W = 22   # NB 64, one-wavefront Hessenberg
 
 
 def rand(rng, b, n):
     return rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n))
 
 
-def dual_handle(n):
+def dual_handle(n, v=None):
     h = tk.Handle()
-    v = DUAL[48] if n <= 48 else DUAL[64]
+    v = v if v is not None else (DUAL[48] if n <= 48 else DUAL[64])
     assert tk.lib().tokuraSetVariant(h.ptr, n, v) == 0
     return h
 
@@ -144,3 +147,64 @@ def test_dual_structured_matrices():
     perm = oracle.match(refJ[0], e[4])
     assert np.abs(e[4][perm][far] - refJ[0][far]).max() <= 1e-10 * np.abs(refJ[0]).max()
     assert np.abs(e[4][perm][~far] - 2.0).max() <= 1e-3
+
+
+@pytest.mark.parametrize("n", [2, 9, 10, 11, 16, 17, 24, 31, 32])
+def test_dual32_vs_lapack_and_oracle(n):
+    # 131 matrices: the last wavefront holds one matrix, its second lane group idle
+    rng = np.random.default_rng(4100 + n)
+    M = rand(rng, 131, n)
+    h = dual_handle(n, DUAL32)
+    e, f = eigs(M, h)
+    h.close()
+    assert not f.any()
+    ref, _ = oracle.lapack_zgeev(M)
+    assert oracle.max_relerr(ref, e) <= TOL_LAPACK
+    me, _ = oracle.method_zgeev(M)
+    assert oracle.max_relerr(me, e) <= TOL_LAPACK
+
+
+def test_dual32_matches_dual64_geometry():
+    # the same one-wavefront Hessenberg and two-bulge sweeps on 16-lane chains
+    # (two matrices per wavefront, id 21) and on 32-lane chains (one matrix per
+    # wavefront, id 22): same arithmetic per step, hence the same bits
+    rng = np.random.default_rng(4200)
+    for n in (12, 25, 32):
+        M = rand(rng, 64, n)
+        h32, h64 = dual_handle(n, DUAL32), dual_handle(n, DUAL_1W)
+        e32, f32 = eigs(M, h32)
+        e64, f64 = eigs(M, h64)
+        h32.close()
+        h64.close()
+        assert not f32.any() and not f64.any()
+        np.testing.assert_array_equal(e32, e64)
+
+
+def test_dual32_hard_spectra_and_flags():
+    mats, kinds = hard_batch(300, 10, 32, 35)
+    sizes = np.array([m.shape[0] for m in mats])
+    worst = 0.0
+    for n in np.unique(sizes):
+        idx = np.nonzero(sizes == n)[0]
+        M = np.stack([mats[b] for b in idx])
+        h = dual_handle(int(n), DUAL32)
+        e, f = eigs(M, h)
+        h.close()
+        assert not f.any()
+        worst = max(worst, float(oracle.condscaled_err(M, e).max()))
+    print("dual32 condition-scaled worst:", worst, flush=True)
+    assert worst <= 100.0
+    rng = np.random.default_rng(43)
+    M = rand(rng, 5, 30)
+    M[1, 7, 3] = np.nan
+    M[2, 0, 29] = np.inf
+    h = dual_handle(30, DUAL32)
+    e, f = eigs(M, h)
+    assert list(f) == [0, 2, 2, 0, 0]
+    assert np.isnan(e[1]).all() and np.isnan(e[2]).all()
+    ref, _ = oracle.lapack_zgeev(M[[0, 3, 4]])
+    assert oracle.max_relerr(ref, e[[0, 3, 4]]) <= TOL_LAPACK
+    h.set_max_sweeps(0)
+    e, f = eigs(rand(rng, 3, 30), h)
+    assert (f == 1).all()
+    h.close()

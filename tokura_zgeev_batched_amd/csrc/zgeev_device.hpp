@@ -659,9 +659,10 @@ __device__ __forceinline__ void store_packed(const zd *H, const FullLayout lay, 
     }
 }
 
-// Ahues-Kressner negligibility test of H(k, k-1) (ZLAHQR loop 40).
+// Ahues-Kressner negligibility test of H(k, k-1) (ZLAHQR loop 40); lo, hi:
+// the ILO, IHI of the call.
 template <class Lay>
-__device__ __forceinline__ bool subdiag_negligible(const zd *H, const Lay &lay, int k, int n,
+__device__ __forceinline__ bool subdiag_negligible(const zd *H, const Lay &lay, int k, int lo, int hi,
                                                    double smlnum, double ulp)
 {
     zd h = H[lay(k, k - 1)];
@@ -669,8 +670,8 @@ __device__ __forceinline__ bool subdiag_negligible(const zd *H, const Lay &lay, 
     zd hkm = H[lay(k - 1, k - 1)], hkk = H[lay(k, k)];
     double tst = cabs1(hkm) + cabs1(hkk);
     if (tst == 0.0) {
-        if (k - 2 >= 0) tst += fabs(H[lay(k - 1, k - 2)].re);
-        if (k + 1 <= n - 1) tst += fabs(H[lay(k + 1, k)].re);
+        if (k - 2 >= lo) tst += fabs(H[lay(k - 1, k - 2)].re);
+        if (k + 1 <= hi) tst += fabs(H[lay(k + 1, k)].re);
     }
     if (fabs(h.re) <= ulp * tst) {
         zd hup = H[lay(k - 1, k)];
@@ -744,14 +745,17 @@ __device__ __forceinline__ zd zlahqr_shift(const zd *H, const Lay &lay, int l, i
 // ZH ("Z on the upper half-wave", requires S == 1 and n <= LPM/2): lanes
 // t < LPM/2 are the H positions, lanes t >= LPM/2 own row t - LPM/2 of Z and
 // apply the reflectors to it in the same instructions that update H.
+// RANGE: only the eigenvalues ilo..ihi are computed (ZLAHQR's ILO/IHI with
+// H(ilo, ilo-1) = 0); with WANTT the transformations still span the whole of
+// T and Z, so the routine finishes a block that a caller has split off.
 struct NoZ {
     __device__ __forceinline__ int cb(int) const { return 0; }
     __device__ __forceinline__ int operator()(int, int) const { return 0; }
 };
 
-template <int LPM, int S, bool WANTT, bool ZH = false, class Lay, class ZLay = NoZ>
+template <int LPM, int S, bool WANTT, bool ZH = false, class Lay, class ZLay = NoZ, bool RANGE = false>
 __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_override, zd *Z = nullptr,
-                           const ZLay zlay = ZLay())
+                           const ZLay zlay = ZLay(), int ihi = 0, int ilo = 0)
 {
     static_assert(!ZH || (S == 1 && WANTT), "half-wave Z needs one position per lane");
     constexpr int HALF = LPM / 2;
@@ -760,18 +764,21 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
     const bool zact = zlane && zrow < n;
     const int tp = zlane ? (1 << 20) : t;          // H position base (none for Z lanes)
     const double ulp = DBL_EPSILON, safmin = DBL_MIN;
-    const double smlnum = safmin * ((double)n / ulp);
-    const int itmax = itmax_override >= 0 ? itmax_override : 30 * (n > 10 ? n : 10);
+    const int nh = RANGE ? ihi - ilo + 1 : n;
+    const double smlnum = safmin * ((double)nh / ulp);
+    const int dlo = RANGE ? ilo : 0, dhi = RANGE ? ihi : n - 1;   // bounds of the deflation test
+    const int itmax = itmax_override >= 0 ? itmax_override : 30 * (nh > 10 ? nh : 10);
     const bool wantz = WANTT && Z != nullptr;
+    const int lfloor = RANGE ? ilo : 0;
     int kdefl = 0;
-    int i = n - 1;
-    while (i >= 1) {
-        int l = 0;
+    int i = RANGE ? ihi : n - 1;
+    while (i >= lfloor + 1) {
+        int l = lfloor;
         bool conv = false;
         for (int its = 0; its <= itmax; ++its) {
             // ---- look for a single small sub-diagonal element
             const int k = group_last_true<LPM, S>(t, l + 1, i, l, [&](int p) {
-                return subdiag_negligible(H, lay, p, n, smlnum, ulp);
+                return subdiag_negligible(H, lay, p, dlo, dhi, smlnum, ulp);
             });
             l = k;
             if (l > 0 && t == 0) H[lay(l, l - 1)] = mk(0.0, 0.0);
@@ -1081,6 +1088,31 @@ __device__ __forceinline__ void bcast_halves(double v, double &lo, double &hi)
     hi = b.d;
 }
 
+// The same within each 32-lane group of a wavefront (v_permlane16_swap): lo
+// receives the group's lanes 0..15 value, hi its lanes 16..31 value, in every
+// lane of the group (the 16-lane halves are uniform where this is used).
+__device__ __forceinline__ void bcast_halves16(double v, double &lo, double &hi)
+{
+    union { double d; unsigned w[2]; } u, a, b;
+    u.d = v;
+    const auto r0 = __builtin_amdgcn_permlane16_swap(u.w[0], u.w[0], false, false);
+    const auto r1 = __builtin_amdgcn_permlane16_swap(u.w[1], u.w[1], false, false);
+    a.w[0] = r0[0];
+    a.w[1] = r1[0];
+    b.w[0] = r0[1];
+    b.w[1] = r1[1];
+    lo = a.d;
+    hi = b.d;
+}
+
+// Halves of a lane group of LPM (32 or 64) lanes, broadcast to the group.
+template <int LPM>
+__device__ __forceinline__ void group_halves(double v, double &lo, double &hi)
+{
+    if constexpr (LPM == 64) bcast_halves(v, lo, hi);
+    else bcast_halves16(v, lo, hi);
+}
+
 // x with its sign bit xor-ed with m (0 or 0x80000000).
 __device__ __forceinline__ double flip_sign(double x, unsigned m)
 {
@@ -1091,13 +1123,15 @@ __device__ __forceinline__ double flip_sign(double x, unsigned m)
 }
 
 
-// ---- eigenvalues-only QR with two bulges per sweep (one matrix per wave) ----
+// ---- QR with two bulges per sweep (LPM = 64: one matrix per wavefront;
+// LPM = 32: two, n <= 32) ----
 // A "dual sweep" over an active block [l, i] is two consecutive single-shift
 // ZLAHQR sweeps with the shifts s1 (the Wilkinson shift) and s2 (the other
 // eigenvalue of the trailing 2x2), pipelined: bulge A (s1) starts at step 0,
-// bulge B (s2) three positions behind it.  Lanes 0..31 compute A's reflector
-// chain and window, lanes 32..63 B's, in the same instructions; every lane p
-// then applies both reflectors at its position (B's first, then A's).
+// bulge B (s2) three positions behind it.  The first half of the group's
+// lanes computes A's reflector chain and window, the second half B's, in the
+// same instructions; every lane p then applies both reflectors at its
+// position (B's first, then A's).
 //  * The two bulges act on disjoint rows/columns except the 2x2 block rows
 //    {kB, kB+1} x columns {kA, kA+1}, which B changes from the left and A
 //    from the right; those products commute, and the phases run in order
@@ -1107,13 +1141,27 @@ __device__ __forceinline__ double flip_sign(double x, unsigned m)
 //  * Windows: A's window (rows/cols kA..kA+2) is untouched by B, B's by A;
 //    B reads H(kA, kA-1) as its next window element after A wrote it.
 //  * Each bulge ends with ZLAHQR's "make H(i,i-1) real" scaling of column i.
+//  * WANTT: the Schur form T, ZLAHQR's WANTT ranges (left updates to column
+//    n-1, right updates from row 0); the extra elements of the two bulges'
+//    updates stay disjoint but for the same 2x2 block.  With Z (global
+//    memory, column-major with leading dimension zld, lane p owns row p)
+//    every reflector is also applied from the right to Z.  Per lane, A reads
+//    its Z columns from memory two steps ahead, hands each column it
+//    finishes to B through registers (B needs column kA-1 one step after A
+//    finished it), and B stores the columns it finishes: one load and one
+//    store per lane and step.
 // Blocks smaller than DUAL_MIN and the last rows go to the single-shift code
-// (zlahqr_core) on the sub-block, which is independent once H(l, l-1) = 0.
+// (zlahqr_core) on the sub-block, which is independent once H(l, l-1) = 0
+// (with WANTT on the whole of T and Z, limited to the block's eigenvalues).
 // Returns 0 or 1 + the 0-based index of the first unconverged eigenvalue.
 //  * Packed layout only: column c starts c + 2 elements after column c - 1,
 //    so every address of a step is an increment of the previous step's.
-__device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n, int t, int itmax_override)
+template <int LPM = 64, bool WANTT = false>
+__device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n, int t, int itmax_override,
+                                           zd *Z = nullptr, int zld = 0)
 {
+    static_assert(LPM == 32 || LPM == 64, "two-bulge QR: 32 or 64 lanes per matrix");
+    constexpr int HALF = LPM / 2;
 #ifndef TKZ_DUAL_MIN
 #define TKZ_DUAL_MIN 10
 #endif
@@ -1122,18 +1170,20 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
     const double ulp = DBL_EPSILON, safmin = DBL_MIN;
     const double smlnum = safmin * ((double)n / ulp);
     const int itmax = itmax_override >= 0 ? itmax_override : 30 * (n > 10 ? n : 10);
-    const int half = t >> 5;           // 0: bulge A's chain, 1: bulge B's
-    const bool leader = (t & 31) == 0; // lanes 0 and 32 store their half's window
+    const int half = t >> (LPM == 64 ? 5 : 4);   // 0: bulge A's chain, 1: bulge B's
+    const bool leader = (t & (HALF - 1)) == 0;   // the first lane of each half stores its window
     const int p = t;                   // bulk position of this lane
     const int cbp = lay.cb(p < n ? p : n - 1);
+    const bool zp = WANTT && Z != nullptr && p < n;   // this lane updates row p of Z
+    zd *const zr = zp ? Z + p : Z;     // element (p, c) of Z at zr[c * zld]
     int kdefl = 0;
     int i = n - 1;
     while (i >= 1) {
         int l = 0;
         bool conv = false;
         for (int its = 0; its <= itmax; ++its) {
-            const int kf = group_last_true<64, 1>(t, l + 1, i, l, [&](int q) {
-                return subdiag_negligible(H, lay, q, n, smlnum, ulp);
+            const int kf = group_last_true<LPM, 1>(t, l + 1, i, l, [&](int q) {
+                return subdiag_negligible(H, lay, q, 0, n - 1, smlnum, ulp);
             });
             l = kf;
             if (l > 0 && t == 0) H[lay(l, l - 1)] = mk(0.0, 0.0);
@@ -1141,9 +1191,15 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
             if (l >= i) { conv = true; break; }
             if (i - l + 1 < DUAL_MIN) {
                 // the whole block [l, i] by single-shift sweeps
-                const int info = zlahqr_core<64, 1, false>(H, SubLayout<PackedLayout>{lay, l}, i - l + 1, t,
-                                                           itmax_override);
-                if (info != 0) return l + info;
+                if constexpr (WANTT) {
+                    const int info = zlahqr_core<LPM, 1, true, false, PackedLayout, FullLayout, true>(
+                        H, lay, n, t, itmax_override, Z, FullLayout{zld}, i, l);
+                    if (info != 0) return info;
+                } else {
+                    const int info = zlahqr_core<LPM, 1, false>(H, SubLayout<PackedLayout>{lay, l}, i - l + 1, t,
+                                                                itmax_override);
+                    if (info != 0) return l + info;
+                }
                 conv = true;
                 break;
             }
@@ -1164,6 +1220,17 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
             zd x1 = mk(1.0, 0.0), a = mk(0.0, 0.0), b = a, c = a, d = a, e = a;
             double x2 = 0.0;
             int kk = l, cbk = lay.cb(l);
+            // Z state of this lane's row (WANTT with Z): A's carried column
+            // kA and the next two (read ahead), the two columns A finished
+            // last (zA0 newest) and the first one, B's carried and next column
+            zd zcA = mk(0.0, 0.0), zyA = zcA, zyA2 = zcA, zA0 = zcA, zA1 = zcA, zF = zcA, zcB = zcA, zyB = zcA;
+            if constexpr (WANTT) {
+                if (zp) {
+                    zcA = zr[l * zld];
+                    zyA = zr[(l + 1) * zld];
+                    zyA2 = zr[(l + 2) * zld];
+                }
+            }
             auto start = [&]() {
                 kk = l;
                 cbk = lay.cb(l);
@@ -1183,8 +1250,13 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
             // rows k, k+1 of columns p >= k+2, from the right on columns k, k+1
             // of rows l <= p <= k-1
             auto phase_addr = [&](const int k, const int cbks, bool &row, bool &col, int &ax, int &ay) {
-                row = pin && p >= k + 2;
-                col = pin && p <= k - 1;
+                if constexpr (WANTT) {        // columns k+2..n-1, rows 0..k-1
+                    row = p >= k + 2 && p < n;
+                    col = p <= k - 1;
+                } else {
+                    row = pin && p >= k + 2;
+                    col = pin && p <= k - 1;
+                }
                 ax = row ? cbp + k : cbks + p;
                 ay = ax + (row ? 1 : k + 2);
             };
@@ -1240,15 +1312,33 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                     H[cbk + kk] = a;
                     H[cbk + 2 * kk + 2] = b;
                 }
-                // both reflectors to every lane as wave-uniform values (A from
-                // lane 0, B from lane 32)
-                const zd tA = mk(readlane_d(t1.re, 0), readlane_d(t1.im, 0));
-                const zd vA = mk(readlane_d(v2.re, 0), readlane_d(v2.im, 0));
-                const double t2A = readlane_d(t2, 0);
-                const zd tB = mk(readlane_d(t1.re, 32), readlane_d(t1.im, 32));
-                const zd vB = mk(readlane_d(v2.re, 32), readlane_d(v2.im, 32));
-                const double t2B = readlane_d(t2, 32);
+                // both reflectors to every lane of the group (one matrix per
+                // wavefront: wave-uniform values read from lanes 0 and 32)
+                zd tA, vA, tB, vB;
+                double t2A, t2B;
+                if constexpr (LPM == 64) {
+                    tA = mk(readlane_d(t1.re, 0), readlane_d(t1.im, 0));
+                    vA = mk(readlane_d(v2.re, 0), readlane_d(v2.im, 0));
+                    t2A = readlane_d(t2, 0);
+                    tB = mk(readlane_d(t1.re, 32), readlane_d(t1.im, 32));
+                    vB = mk(readlane_d(v2.re, 32), readlane_d(v2.im, 32));
+                    t2B = readlane_d(t2, 32);
+                } else {
+                    group_halves<LPM>(t1.re, tA.re, tB.re);
+                    group_halves<LPM>(t1.im, tA.im, tB.im);
+                    group_halves<LPM>(v2.re, vA.re, vB.re);
+                    group_halves<LPM>(v2.im, vA.im, vB.im);
+                    group_halves<LPM>(t2, t2A, t2B);
+                }
                 if constexpr (hasB) phase_apply(rowB, colB, axB, ayB, xB, yB, tB, vB, t2B);
+                if constexpr (WANTT && hasB) {   // Z(p, kB), Z(p, kB+1): column kB is final
+                    if (zp) {
+                        zd x = zcB, y = zyB;
+                        pair_op(x, y, tB, t2B, zconj(vB));
+                        zr[kB * zld] = x;
+                        zcB = y;
+                    }
+                }
                 lds_order();
                 if constexpr (hasA) {         // phase 2 reads after phase 1's writes
                     bool rowA, colA;
@@ -1260,6 +1350,16 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                         yA = H[ayA];
                     }
                     phase_apply(rowA, colA, axA, ayA, xA, yA, tA, vA, t2A);
+                    if constexpr (WANTT) {       // Z(p, kA), Z(p, kA+1): column kA goes to B
+                        zd x = zcA, y = zyA;
+                        pair_op(x, y, tA, t2A, zconj(vA));
+                        if constexpr (!hasB) {
+                            if (tau == 0) zF = x;
+                        }
+                        zA1 = zA0;
+                        zA0 = x;
+                        zcA = y;
+                    }
                 }
                 lds_order();
                 // advance to kk+1: b, d, e are rows kk+1 .. kk+3 of column kk+2,
@@ -1278,30 +1378,55 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                         kk += 1;
                     }
                 }
+                if constexpr (WANTT) {
+                    // A's next columns: kA+2 (read last step), kA+3 (if <= i)
+                    if constexpr ((adv & 1) != 0) {
+                        zyA = zyA2;
+                        if (zp && kA + 3 <= i) zyA2 = zr[(kA + 3) * zld];
+                    }
+                    // B's next column kB+2 = kA-1: the column A finished last step
+                    if constexpr ((adv & 2) != 0) zyB = zA1;
+                    if constexpr (!hasA) zA1 = zA0;   // after A's end: column i next
+                }
             };
             // end of a bulge's sweep (ZLAHQR): H(i,i-1) made real, column i
             // rows l..i-1 scaled; c, d of half `hb` hold H(i,i-1), H(i,i)
             auto finish = [&](const int hb) {
                 double cre, cim, dre, dim, o;
                 if (hb == 0) {
-                    bcast_halves(c.re, cre, o);
-                    bcast_halves(c.im, cim, o);
-                    bcast_halves(d.re, dre, o);
-                    bcast_halves(d.im, dim, o);
+                    group_halves<LPM>(c.re, cre, o);
+                    group_halves<LPM>(c.im, cim, o);
+                    group_halves<LPM>(d.re, dre, o);
+                    group_halves<LPM>(d.im, dim, o);
                 } else {
-                    bcast_halves(c.re, o, cre);
-                    bcast_halves(c.im, o, cim);
-                    bcast_halves(d.re, o, dre);
-                    bcast_halves(d.im, o, dim);
+                    group_halves<LPM>(c.re, o, cre);
+                    group_halves<LPM>(c.im, o, cim);
+                    group_halves<LPM>(d.re, o, dre);
+                    group_halves<LPM>(d.im, o, dim);
                 }
                 const zd ce = mk(cre, cim), de = mk(dre, dim);
-                if (ce.im != 0.0) {
+                const bool rot = ce.im != 0.0;
+                zd temp = mk(1.0, 0.0);
+                if (rot) {
                     const double r = zabs(ce);
-                    const zd temp = zscale(1.0 / r, ce);
-                    if (p >= l && p <= i - 1) H[lay(p, i)] = zmul(H[lay(p, i)], temp);
+                    temp = zscale(1.0 / r, ce);
+                    if (p >= (WANTT ? 0 : l) && p <= i - 1) H[lay(p, i)] = zmul(H[lay(p, i)], temp);
+                    if constexpr (WANTT) {
+                        if (p >= i + 1 && p < n) H[lay(i, p)] = zmul(H[lay(i, p)], zconj(temp));
+                    }
                     if (t == 0) H[lay(i, i - 1)] = mk(r, 0.0);
                 } else if (t == 0) {
                     H[lay(i, i - 1)] = ce;
+                }
+                if constexpr (WANTT) {
+                    // column i of Z: A's goes to B as the column after i-1,
+                    // B's is final
+                    if (hb == 0) {
+                        zA1 = zA0;
+                        zA0 = rot ? zmul(zcA, temp) : zcA;
+                    } else if (zp) {
+                        zr[i * zld] = rot ? zmul(zcB, temp) : zcB;
+                    }
                 }
                 if (t == 0) H[lay(i, i)] = de;
                 lds_order();
@@ -1312,6 +1437,10 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
             if (half == 0) start();
             for (int tau = 0; tau < 3; ++tau) dstep(T_(), F_(), std::integral_constant<int, 1>(), tau);
             if (half == 1) start();
+            if constexpr (WANTT) {            // B's columns l, l+1: A's first two
+                zcB = zF;
+                zyB = zA1;
+            }
             for (int tau = 3; tau < nsteps - 1; ++tau) dstep(T_(), T_(), std::integral_constant<int, 3>(), tau);
             dstep(T_(), T_(), std::integral_constant<int, 2>(), nsteps - 1);   // A's last step
             finish(0);

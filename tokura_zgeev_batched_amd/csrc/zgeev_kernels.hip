@@ -127,7 +127,7 @@ __global__ __launch_bounds__(NT) void k_hessenberg_packed_wg(int n, int batch, c
 }
 
 // Stage 2 of the split path: QR on the packed Hessenberg matrix (DUAL: two
-// bulges per sweep, requires LPM == 64).
+// bulges per sweep, LPM == 64, or LPM == 32 with n <= 32).
 template <int NB, int LPM, bool DUAL = false>
 __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__restrict__ work,
                                                   zd *__restrict__ eig, int8_t *__restrict__ flag,
@@ -152,8 +152,8 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
     const PackedLayout pk;
     int info = 0;
     if constexpr (DUAL) {
-        static_assert(LPM == 64, "two-bulge QR: one matrix per wavefront");
-        if (n > 1) info = zlahqr_dual(H, pk, n, t, itmax);
+        static_assert(LPM == 64 || (LPM == 32 && NB <= 32), "two-bulge QR: a lane per position");
+        if (n > 1) info = zlahqr_dual<LPM>(H, pk, n, t, itmax);
     } else {
         if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
     }
@@ -185,7 +185,9 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
     X(17, SPLIT, 64, 32)                \
     X(18, SPLIT, 64, 64)                \
     X(19, DUAL, 48, 64)                 \
-    X(20, DUAL, 64, 64)
+    X(20, DUAL, 64, 64)                 \
+    X(21, DUAL, 32, 32)                 \
+    X(22, DUAL_1W, 64, 64)
 
 static const VariantInfo g_variants[] = {
 #define X(id, kind, nb, lpm) {id, VK_##kind, nb, lpm, 64 / lpm},
@@ -206,14 +208,14 @@ static hipError_t launch_fused(int n, int batch, const zd *A, zd *eig, int8_t *f
     return hipGetLastError();
 }
 
-template <int NB, int LPM, bool DUAL = false>
+template <int NB, int LPM, bool DUAL = false, bool ONEWAVE = false>
 static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *flag, zd *work, int itmax,
                                int bal, hipStream_t s, int *stage, hipEvent_t mid)
 {
     constexpr int GPW = 64 / LPM;
     const int blocks = (batch + GPW - 1) / GPW;
     *stage = 1;
-    if (NB >= 48)   // large matrices: one workgroup per matrix
+    if (NB >= 48 && !ONEWAVE)   // large matrices: one workgroup per matrix
         hipLaunchKernelGGL((k_hessenberg_packed_wg<NB, TKZ_HESS_NT>), dim3(batch), dim3(TKZ_HESS_NT), 0, s, n, batch,
                            A, work, flag, bal);
     else
@@ -239,7 +241,8 @@ hipError_t launch_plain(int variant, int n, int batch, const void *A, void *eig,
 #define X(id, kind, nb, lpm)                                                              \
     case id:                                                                              \
         if constexpr (VK_##kind == VK_FUSED) return launch_fused<nb, lpm>(n, batch, a, w, f, itmax, bal, s); \
-        else return launch_split<nb, lpm, VK_##kind == VK_DUAL>(n, batch, a, w, f, wk, itmax, bal, s, stage, mid);
+        else return launch_split<nb, lpm, VK_##kind == VK_DUAL || VK_##kind == VK_DUAL_1W, VK_##kind == VK_DUAL_1W>( \
+            n, batch, a, w, f, wk, itmax, bal, s, stage, mid);
         TKZ_VARIANTS(X)
 #undef X
     default:

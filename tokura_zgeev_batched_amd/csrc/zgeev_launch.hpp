@@ -4,8 +4,11 @@
 
 namespace tkz {
 
-// VK_DUAL: split path whose QR runs two bulges per sweep (one matrix per wave)
-enum VariantKind { VK_FUSED = 0, VK_SPLIT = 1, VK_DUAL = 2 };
+// VK_DUAL: split path whose QR runs two bulges per sweep (a 256-thread
+// workgroup Hessenberg for NB >= 48); VK_DUAL_1W: the same QR after the
+// one-wavefront Hessenberg (the arithmetic of the Schur-form kernels, so its
+// eigenvalues come in their order)
+enum VariantKind { VK_FUSED = 0, VK_SPLIT = 1, VK_DUAL = 2, VK_DUAL_1W = 3 };
 
 // ZGEBAL job bits of tokuraSetBalancing ('P' = permute, 'S' = scale, 'B' = both)
 enum BalanceBits { BAL_PERMUTE = 1, BAL_SCALE = 2 };

@@ -71,6 +71,46 @@ __device__ __forceinline__ void form_q_inplace(zd *A, const FullLayout lay, cons
     }
 }
 
+// The Schur-form QR: the two-bulge sweeps (zlahqr_dual with WANTT) for
+// n > 16, the single-shift ZLAHQR below, in every Schur path (same-size,
+// ragged, eigenvectors, the recompute backward), so that all of them give the
+// same T and U bits, and the eigenvalue order of the matching eigenvalue-only
+// kernels (the two-bulge QR after the one-wavefront Hessenberg).
+__host__ __device__ constexpr bool schur_dual(int n) { return n > 16; }
+
+// In-place conversion of the Hessenberg part of a full LDS layout (ld) to
+// the packed layout (column c: rows 0..c+1 from offset c(c+3)/2) and back.
+// Packed column c ends where full column c+1 starts or before, so columns
+// move in ascending order (descending to unpack); within a column every lane
+// reads its element before any lane writes (n <= LPM).
+template <int LPM>
+__device__ __forceinline__ void pack_hessenberg(zd *H, const FullLayout lay, int n, int t)
+{
+    const PackedLayout pk;
+    for (int c = 0; c < n; ++c) {
+        const int rows = c + 2 < n ? c + 2 : n;
+        zd v = mk(0.0, 0.0);
+        if (t < rows) v = H[lay(t, c)];
+        lds_order();
+        if (t < rows) H[pk(t, c)] = v;
+        lds_order();
+    }
+}
+
+template <int LPM>
+__device__ __forceinline__ void unpack_hessenberg(zd *H, const FullLayout lay, int n, int t)
+{
+    const PackedLayout pk;
+    for (int c = n - 1; c >= 0; --c) {
+        const int rows = c + 2 < n ? c + 2 : n;
+        zd v = mk(0.0, 0.0);
+        if (t < rows) v = H[pk(t, c)];
+        lds_order();
+        if (t < rows) H[lay(t, c)] = v;
+        lds_order();
+    }
+}
+
 // Schur form of the matrix at src: T into H (upper triangular), U into Z.
 // Returns -1 for a non-finite input, else the QR INFO.
 // GZ: Z is an n x n column-major global buffer (ld n), so that the LDS holds
@@ -107,7 +147,17 @@ __device__ __forceinline__ int schur_in_lds(zd *H, zd *Z, zd *taus, const FullLa
         form_q<LPM>(Z, zl, H, lay, taus, n, t);
     }
     int info = 0;
-    if (n > 1) info = zlahqr_core<LPM, S, true, ZH>(H, lay, n, t, itmax, Z, zl);
+    if constexpr (LPM >= 32) {
+        if (schur_dual(n)) {   // n <= NB <= LPM: one position and one row of Z per lane
+            pack_hessenberg<LPM>(H, lay, n, t);
+            info = zlahqr_dual<LPM, true>(H, PackedLayout{}, n, t, itmax, Z, zl.ld);
+            unpack_hessenberg<LPM>(H, lay, n, t);
+        } else if (n > 1) {
+            info = zlahqr_core<LPM, S, true, ZH>(H, lay, n, t, itmax, Z, zl);
+        }
+    } else {
+        if (n > 1) info = zlahqr_core<LPM, S, true, ZH>(H, lay, n, t, itmax, Z, zl);
+    }
     for (int c = 0; c < n; ++c)
         for (int r = c + 1 + t; r < n; r += LPM) H[lay(r, c)] = mk(0.0, 0.0);
     lds_order();
@@ -143,8 +193,9 @@ __device__ __forceinline__ bool schur_hq_one(zd *H, zd *taus, const zd *__restri
     return true;
 }
 
-// Second half (packed LDS layout, packed_size(n) complex; n <= LPM): ZLAHQR
-// with WANTT/WANTZ on the Hessenberg matrix from Th, Z = Ug updated in place
+// Second half (packed LDS layout, packed_size(n) complex; n <= LPM): the
+// Schur-form QR (two bulges per sweep for n > 16, else ZLAHQR) with
+// WANTT/WANTZ on the Hessenberg matrix from Th, Z = Ug updated in place
 // (lane p owns row p, as it wrote it), then the eigenvalues to out and T
 // (zeros below the diagonal) back to Th, row by row by the lane that read
 // it.  Returns the QR INFO.
@@ -157,7 +208,12 @@ __device__ __forceinline__ int schur_qr_one(zd *H, int n, int t, zd *Th, zd *Ug,
         for (int c = r > 0 ? r - 1 : 0; c < n; ++c) H[lay(r, c)] = Th[r + c * n];
     lds_order();
     int info = 0;
-    if (n > 1) info = zlahqr_core<LPM, 1, true, false>(H, lay, n, t, itmax, Ug, FullLayout{n});
+    if constexpr (LPM >= 32) {
+        if (schur_dual(n)) info = zlahqr_dual<LPM, true>(H, lay, n, t, itmax, Ug, n);
+        else if (n > 1) info = zlahqr_core<LPM, 1, true, false>(H, lay, n, t, itmax, Ug, FullLayout{n});
+    } else {
+        if (n > 1) info = zlahqr_core<LPM, 1, true, false>(H, lay, n, t, itmax, Ug, FullLayout{n});
+    }
     lds_order();
     if (r < n) {
         out[r] = H[lay(r, r)];
