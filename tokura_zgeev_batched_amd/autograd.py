@@ -23,10 +23,10 @@ from . import (Handle, TokuraError, _P, _check, _dev_check, _stream_ptr, default
 
 # fused same-size variants whose lanes-per-matrix equal the adjoint kernel's
 # (variant ids of the same-size table: 5 = 16x16, 8 = 32x32, 12 = 64x64)
-# eigenvalue variants whose arithmetic is the Schur-form kernels' (n <= 16:
-# fused single-shift QR; n > 16: the two-bulge QR after the one-wavefront
+# eigenvalue variants whose arithmetic is the Schur-form kernels' (n <= 32:
+# fused single-shift QR; n > 32: the two-bulge QR after the one-wavefront
 # Hessenberg), so the forward's order is the recompute backward's
-_MATCHED_VARIANT = {16: 5, 32: 21, 64: 22}
+_MATCHED_VARIANT = {16: 5, 32: 8, 64: 22}
 
 
 def _adj_nb(n: int) -> int:

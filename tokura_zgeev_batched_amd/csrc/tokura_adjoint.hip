@@ -178,7 +178,11 @@ static int adj_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 // (there is no second slot to park T in); everything else goes through
 // k_schur_hq / k_schur_qr with NB lanes per matrix.
 #define TKZ_SCHUR_CFG(X) X(16, 32, true, false) X(32, 64, true, false) X(64, 64, false, true)
-#define TKZ_SPLIT_NB(X) X(16) X(32) X(64)
+// (NB, lanes per matrix) of the two-launch Schur form: n 33..48 gets its own
+// LDS footprint (packed 19.6 KB: eight matrices per CU in the QR)
+#define TKZ_SPLIT_NB(X) X(16, 16) X(32, 32) X(48, 64) X(64, 64)
+
+static int split_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : (n <= 48 ? 48 : 64)); }
 
 // Tslot receives the Hessenberg matrix, then T; Uslot receives Q, then U.
 // With VEC, k_vectors_from_schur follows (eigenvectors into VLv / VRv).
@@ -186,16 +190,16 @@ static hipError_t launch_schur_split(int n, int batch, const zd *A, zd *eig, zd 
                                      zd *Uout, long long ustride, int8_t *flag, int itmax, hipStream_t s,
                                      bool vec = false, zd *VLv = nullptr, zd *VRv = nullptr)
 {
-    switch (adj_nb(n)) {
-#define X(nb)                                                                                                   \
+    switch (split_nb(n)) {
+#define X(nb, lpm)                                                                                              \
     case nb: {                                                                                                  \
-        const dim3 grid((batch + 64 / nb - 1) / (64 / nb));                                                     \
-        hipLaunchKernelGGL((k_schur_hq<nb, nb>), grid, dim3(64), 0, s, n, batch, A, eig, Tout, tstride, Uout,    \
+        const dim3 grid((batch + 64 / lpm - 1) / (64 / lpm));                                                   \
+        hipLaunchKernelGGL((k_schur_hq<nb, lpm>), grid, dim3(64), 0, s, n, batch, A, eig, Tout, tstride, Uout,   \
                            ustride, flag);                                                                      \
-        hipLaunchKernelGGL((k_schur_qr<nb, nb>), grid, dim3(64), 0, s, n, batch, eig, Tout, tstride, Uout,       \
+        hipLaunchKernelGGL((k_schur_qr<nb, lpm>), grid, dim3(64), 0, s, n, batch, eig, Tout, tstride, Uout,      \
                            ustride, flag, itmax);                                                               \
         if (vec)                                                                                                \
-            hipLaunchKernelGGL((k_vectors_from_schur<nb, nb>), grid, dim3(64), 0, s, n, batch, Tout, VLv, VRv,   \
+            hipLaunchKernelGGL((k_vectors_from_schur<nb, lpm>), grid, dim3(64), 0, s, n, batch, Tout, VLv, VRv,  \
                                flag);                                                                           \
         return hipGetLastError();                                                                               \
     }

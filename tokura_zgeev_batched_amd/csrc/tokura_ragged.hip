@@ -262,16 +262,23 @@ static hipError_t launch_ragged(int cfg, const zd *A, char *work, zd *eig, int8_
     }
 }
 // ---- differentiation of a ragged batch ------------------------------------
-// One launch per size class (n > 32, 17..32, <= 16), each with the LDS
-// footprint and lanes-per-matrix of the same-size Schur / adjoint kernels of
-// that class, so every matrix gets the same arithmetic as in a same-size
-// call.  order[] lists the class's matrices contiguously (n descending); the
-// Schur form of matrix b is at 2*moff[b] (T then U), dA at moff[b], the
-// eigenvalues and cotangents at eoff[b].
+// One launch per size class, each with the LDS footprint and lanes-per-matrix
+// of the same-size Schur / adjoint kernels of that class, so every matrix gets
+// the same arithmetic as in a same-size call: the Schur form in the classes
+// n > 48, 33..48, 17..32, <= 16 (as the eigenvalue path), the backward in
+// n > 32, 17..32, <= 16.  order[] lists each class's matrices contiguously
+// (n descending); the Schur form of matrix b is at 2*moff[b] (T then U), dA
+// at moff[b], the eigenvalues and cotangents at eoff[b].
 __device__ __forceinline__ void class_range(const RaggedView &V, int cls, int &lo, int &hi)
 {
     lo = cls == 0 ? 0 : (cls == 1 ? V.h->nbig : V.h->n16);
     hi = cls == 0 ? V.h->nbig : (cls == 1 ? V.h->n16 : V.h->batch);
+}
+
+__device__ __forceinline__ void schur_class_range(const RaggedView &V, int cls, int &lo, int &hi)
+{
+    lo = cls == 0 ? 0 : (cls == 1 ? V.h->n48 : (cls == 2 ? V.h->nbig : V.h->n16));
+    hi = cls == 0 ? V.h->n48 : (cls == 1 ? V.h->nbig : (cls == 2 ? V.h->n16 : V.h->batch));
 }
 
 // The Schur forward of size class CLS in two launches, as the same-size path:
@@ -286,7 +293,7 @@ __global__ __launch_bounds__(64) void k_ragged_schur_hq(char *work, const zd *__
     __shared__ zd lds[GPW * PER];
     const RaggedView V(work);
     int lo, hi;
-    class_range(V, CLS, lo, hi);
+    schur_class_range(V, CLS, lo, hi);
     const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
     zd *H = lds + g * PER;
     for (int base = lo + (int)blockIdx.x * GPW; base < hi; base += (int)gridDim.x * GPW) {
@@ -307,6 +314,9 @@ __global__ __launch_bounds__(64) void k_ragged_schur_hq(char *work, const zd *__
     }
 }
 
+// The QR of a Schur class as a persistent kernel: every wavefront pulls the
+// class's next GPW matrices from the class's queue (qctr[CLS], zeroed by
+// k_ragged_reset) until the class is done.
 template <int NB, int LPM, int CLS>
 __global__ __launch_bounds__(64) void k_ragged_schur_qr(char *work, zd *__restrict__ eig, zd *schur, int8_t *flag,
                                                         int itmax)
@@ -315,9 +325,13 @@ __global__ __launch_bounds__(64) void k_ragged_schur_qr(char *work, zd *__restri
     __shared__ zd lds[GPW * PER];
     const RaggedView V(work);
     int lo, hi;
-    class_range(V, CLS, lo, hi);
+    schur_class_range(V, CLS, lo, hi);
     const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
-    for (int base = lo + (int)blockIdx.x * GPW; base < hi; base += (int)gridDim.x * GPW) {
+    for (;;) {
+        int k = 0;
+        if (lane == 0) k = atomicAdd(&V.h->qctr[CLS], 1);
+        const int base = lo + __shfl(k, 0) * GPW;
+        if (base >= hi) break;
         const int j = base + g;
         if (j < hi) {
             const int b = V.order[j], n = V.nsz[b];
@@ -381,18 +395,20 @@ __global__ __launch_bounds__(NT) void k_ragged_adjoint_wg(char *work, const zd *
 static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schur, int8_t *flag, int itmax, int cus,
                                       hipStream_t s)
 {
-#define TKZ_RS(nb, cls)                                                                                       \
+    hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(1), 0, s, work);
+#define TKZ_RS(nb, lpm, cls)                                                                                  \
     {                                                                                                          \
-        static const int oh = resident_blocks(k_ragged_schur_hq<nb, nb, cls>);                                 \
-        static const int oq = resident_blocks(k_ragged_schur_qr<nb, nb, cls>);                                 \
-        hipLaunchKernelGGL((k_ragged_schur_hq<nb, nb, cls>), dim3(cus * oh), dim3(64), 0, s, work, A, eig, schur, \
-                           flag);                                                                              \
-        hipLaunchKernelGGL((k_ragged_schur_qr<nb, nb, cls>), dim3(cus * oq), dim3(64), 0, s, work, eig, schur,  \
+        static const int oh = resident_blocks(k_ragged_schur_hq<nb, lpm, cls>);                                \
+        static const int oq = resident_blocks(k_ragged_schur_qr<nb, lpm, cls>);                                \
+        hipLaunchKernelGGL((k_ragged_schur_hq<nb, lpm, cls>), dim3(cus * oh), dim3(64), 0, s, work, A, eig,     \
+                           schur, flag);                                                                       \
+        hipLaunchKernelGGL((k_ragged_schur_qr<nb, lpm, cls>), dim3(cus * oq), dim3(64), 0, s, work, eig, schur, \
                            flag, itmax);                                                                       \
     }
-    TKZ_RS(64, 0)
-    TKZ_RS(32, 1)
-    TKZ_RS(16, 2)
+    TKZ_RS(64, 64, 0)
+    TKZ_RS(48, 64, 1)
+    TKZ_RS(32, 32, 2)
+    TKZ_RS(16, 16, 3)
 #undef TKZ_RS
     return hipGetLastError();
 }

@@ -71,12 +71,14 @@ __device__ __forceinline__ void form_q_inplace(zd *A, const FullLayout lay, cons
     }
 }
 
-// The Schur-form QR: the two-bulge sweeps (zlahqr_dual with WANTT) for
-// n > 16, the single-shift ZLAHQR below, in every Schur path (same-size,
-// ragged, eigenvectors, the recompute backward), so that all of them give the
-// same T and U bits, and the eigenvalue order of the matching eigenvalue-only
-// kernels (the two-bulge QR after the one-wavefront Hessenberg).
-__host__ __device__ constexpr bool schur_dual(int n) { return n > 16; }
+// The Schur-form QR: the two-bulge sweeps (zlahqr_dual with WANTT, one
+// matrix per wavefront) for n > 32, the single-shift ZLAHQR below (two or
+// more matrices per wavefront, where the single chain serves them all), in
+// every Schur path (same-size, ragged, eigenvectors, the recompute backward),
+// so that all of them give the same T and U bits, and the eigenvalue order of
+// the matching eigenvalue-only kernels (for n > 32 the two-bulge QR after the
+// one-wavefront Hessenberg).
+__host__ __device__ constexpr bool schur_dual(int n) { return n > 32; }
 
 // In-place conversion of the Hessenberg part of a full LDS layout (ld) to
 // the packed layout (column c: rows 0..c+1 from offset c(c+3)/2) and back.
@@ -147,7 +149,7 @@ __device__ __forceinline__ int schur_in_lds(zd *H, zd *Z, zd *taus, const FullLa
         form_q<LPM>(Z, zl, H, lay, taus, n, t);
     }
     int info = 0;
-    if constexpr (LPM >= 32) {
+    if constexpr (LPM == 64) {
         if (schur_dual(n)) {   // n <= NB <= LPM: one position and one row of Z per lane
             pack_hessenberg<LPM>(H, lay, n, t);
             info = zlahqr_dual<LPM, true>(H, PackedLayout{}, n, t, itmax, Z, zl.ld);
@@ -194,7 +196,7 @@ __device__ __forceinline__ bool schur_hq_one(zd *H, zd *taus, const zd *__restri
 }
 
 // Second half (packed LDS layout, packed_size(n) complex; n <= LPM): the
-// Schur-form QR (two bulges per sweep for n > 16, else ZLAHQR) with
+// Schur-form QR (two bulges per sweep for n > 32, else ZLAHQR) with
 // WANTT/WANTZ on the Hessenberg matrix from Th, Z = Ug updated in place
 // (lane p owns row p, as it wrote it), then the eigenvalues to out and T
 // (zeros below the diagonal) back to Th, row by row by the lane that read
@@ -208,7 +210,7 @@ __device__ __forceinline__ int schur_qr_one(zd *H, int n, int t, zd *Th, zd *Ug,
         for (int c = r > 0 ? r - 1 : 0; c < n; ++c) H[lay(r, c)] = Th[r + c * n];
     lds_order();
     int info = 0;
-    if constexpr (LPM >= 32) {
+    if constexpr (LPM == 64) {
         if (schur_dual(n)) info = zlahqr_dual<LPM, true>(H, lay, n, t, itmax, Ug, n);
         else if (n > 1) info = zlahqr_core<LPM, 1, true, false>(H, lay, n, t, itmax, Ug, FullLayout{n});
     } else {
