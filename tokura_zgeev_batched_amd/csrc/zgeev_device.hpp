@@ -812,8 +812,11 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
             // State between steps: reflector input (x1, x2), window
             // a = H(k,k), b = H(k,k+1), c = H(k+1,k), d = H(k+1,k+1),
             // e = H(k+2,k+1); per position the carried element and the
-            // prefetched partner (ypre), and for Z the carried Z(p,k) and Z(p,k+1).
-            zd carry[S], ypre[S], zc[S], zy[S];
+            // prefetched partner (ypre), and for Z the carried Z(p,k) and
+            // Z(p,k+1), the latter read two steps ahead into zyE (steps of
+            // even k - m) or zyO (odd) so that no register move waits for a
+            // load in flight.
+            zd carry[S], ypre[S], zc[S], zyE[S], zyO[S];
             int cbp[S];   // column base of the position's own column
 #pragma unroll
             for (int s = 0; s < S; ++s) {
@@ -828,15 +831,21 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                 }
                 if constexpr (WANTT && !ZH) {
                     zc[s] = mk(0.0, 0.0);
-                    zy[s] = mk(0.0, 0.0);
-                    if (wantz && p < n) { zc[s] = Z[zlay(p, m)]; zy[s] = Z[zlay(p, m + 1)]; }
+                    zyE[s] = mk(0.0, 0.0);
+                    zyO[s] = mk(0.0, 0.0);
+                    if (wantz && p < n) {
+                        zc[s] = Z[zlay(p, m)];
+                        zyE[s] = Z[zlay(p, m + 1)];
+                        if (m + 2 <= i) zyO[s] = Z[zlay(p, m + 2)];
+                    }
                 }
             }
             zd a = H[lay(m, m)], b = H[lay(m, m + 1)], c = H[lay(m + 1, m)], d = H[lay(m + 1, m + 1)];
             zd e = (m + 2 <= i) ? H[lay(m + 2, m + 1)] : mk(0.0, 0.0);
 
             // one chase step kk; returns the new bulge H(kk+2, kk)
-            auto step = [&](const int kk, const bool first, const bool has_e) -> double {
+            auto step = [&](auto PAR, const int kk, const bool first, const bool has_e) -> double {
+                constexpr int par = decltype(PAR)::value;   // (kk - m) & 1: the Z read-ahead register
                 zd t1, v2;
                 double t2, beta;
                 bool ident;
@@ -865,10 +874,12 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                     }
                     if constexpr (WANTT && !ZH) {
                         if (wantz && p < n) {
-                            zd x = zc[s], y = zy[s];
+                            zd &zy = par ? zyO[s] : zyE[s];
+                            zd x = zc[s], y = zy;
                             pair_op(x, y, t1, t2, zconj(v2));
                             Z[zlay(p, kk)] = x;
                             zc[s] = y;
+                            if (kk + 3 <= i) zy = Z[zlay(p, kk + 3)];   // for step kk + 2
                         }
                     }
                 }
@@ -916,13 +927,13 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                     } else {
                         ypre[s] = H[rowop ? cbp[s] + k1 + 1 : cb1 + pc];
                     }
-                    if constexpr (WANTT && !ZH) {
-                        if (wantz) zy[s] = Z[zlay(p < n ? p : n - 1, k1 + 1)];
-                    }
+
                 }
             };
 
-            double f = step(m, true, m + 2 <= i);
+            using P0 = std::integral_constant<int, 0>;
+            using P1 = std::integral_constant<int, 1>;
+            double f = step(P0(), m, true, m + 2 <= i);
             if (m > l) {
                 // ZLAHQR keeps H(m, m-1) real after a start at m > l:
                 // flush the step's state, scale in LDS, reload.
@@ -987,8 +998,12 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                         if constexpr (ZH) {
                             if (zact) { carry[s] = Z[zlay(zrow, k1)]; ypre[s] = Z[zlay(zrow, k1 + 1)]; }
                         }
-                        if constexpr (WANTT && !ZH) {
-                            if (wantz && p < n) { zc[s] = Z[zlay(p, k1)]; zy[s] = Z[zlay(p, k1 + 1)]; }
+                        if constexpr (WANTT && !ZH) {   // k1 - m = 1: step k1 reads zyO
+                            if (wantz && p < n) {
+                                zc[s] = Z[zlay(p, k1)];
+                                zyO[s] = Z[zlay(p, k1 + 1)];
+                                if (k1 + 2 <= i) zyE[s] = Z[zlay(p, k1 + 2)];
+                            }
                         }
                     }
                 } else {
@@ -1011,11 +1026,29 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                 advance(m, f);
             }
             // main chase: every step has a row kk+2 and a successor
-            for (int kk = m + 1; kk < i - 1; ++kk) {
-                f = step(kk, false, true);
-                advance(kk, f);
+            if constexpr (WANTT && !ZH) {
+                int kk = m + 1;
+                for (; kk + 1 < i - 1; kk += 2) {
+                    f = step(P1(), kk, false, true);
+                    advance(kk, f);
+                    f = step(P0(), kk + 1, false, true);
+                    advance(kk + 1, f);
+                }
+                if (kk < i - 1) {
+                    f = step(P1(), kk, false, true);
+                    advance(kk, f);
+                }
+                if (i - 1 > m) {
+                    if ((i - 1 - m) & 1) f = step(P1(), i - 1, false, false);
+                    else f = step(P0(), i - 1, false, false);
+                }
+            } else {
+                for (int kk = m + 1; kk < i - 1; ++kk) {
+                    f = step(P0(), kk, false, true);
+                    advance(kk, f);
+                }
+                if (i - 1 > m) f = step(P0(), i - 1, false, false);
             }
-            if (i - 1 > m) f = step(i - 1, false, false);
             // ---- end of sweep: c = H(i,i-1), d = H(i,i); make H(i,i-1) real
             zd hsub = c;
             bool rot = (c.im != 0.0);
@@ -1221,14 +1254,16 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
             double x2 = 0.0;
             int kk = l, cbk = lay.cb(l);
             // Z state of this lane's row (WANTT with Z): A's carried column
-            // kA and the next two (read ahead), the two columns A finished
-            // last (zA0 newest) and the first one, B's carried and next column
-            zd zcA = mk(0.0, 0.0), zyA = zcA, zyA2 = zcA, zA0 = zcA, zA1 = zcA, zF = zcA, zcB = zcA, zyB = zcA;
+            // kA; the column A's step tau needs next, read two steps ahead
+            // into zyE (even tau) or zyO (odd tau) so that no register move
+            // waits for a load in flight; the two columns A finished last
+            // (zA0 newest) and the first one; B's carried and next column
+            zd zcA = mk(0.0, 0.0), zyE = zcA, zyO = zcA, zA0 = zcA, zA1 = zcA, zF = zcA, zcB = zcA, zyB = zcA;
             if constexpr (WANTT) {
                 if (zp) {
                     zcA = zr[l * zld];
-                    zyA = zr[(l + 1) * zld];
-                    zyA2 = zr[(l + 2) * zld];
+                    zyE = zr[(l + 1) * zld];
+                    zyO = zr[(l + 2) * zld];
                 }
             }
             auto start = [&]() {
@@ -1274,9 +1309,10 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
             // one time step: bulge A at kA = l + tau, bulge B at kA - 3.
             // HA/HB: bulge A/B active; ADV bit 0: A advances, bit 1: B
             // advances (compile-time: the steady state has no per-half branch)
-            auto dstep = [&](auto HA, auto HB, auto ADV, const int tau) {
+            auto dstep = [&](auto HA, auto HB, auto ADV, auto PAR, const int tau) {
                 constexpr bool hasA = decltype(HA)::value, hasB = decltype(HB)::value;
                 constexpr int adv = decltype(ADV)::value;
+                constexpr int par = decltype(PAR)::value;   // tau & 1 (Z read-ahead register)
                 const int kA = l + tau, kB = kA - 3;
                 // phase-1 operands (bulge B): independent of this step's chain
                 bool rowB = false, colB = false;
@@ -1351,7 +1387,8 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                     }
                     phase_apply(rowA, colA, axA, ayA, xA, yA, tA, vA, t2A);
                     if constexpr (WANTT) {       // Z(p, kA), Z(p, kA+1): column kA goes to B
-                        zd x = zcA, y = zyA;
+                        zd &zy = par ? zyO : zyE;
+                        zd x = zcA, y = zy;
                         pair_op(x, y, tA, t2A, zconj(vA));
                         if constexpr (!hasB) {
                             if (tau == 0) zF = x;
@@ -1359,6 +1396,7 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                         zA1 = zA0;
                         zA0 = x;
                         zcA = y;
+                        if (zp && kA + 3 <= i) zy = zr[(kA + 3) * zld];   // for step tau + 2
                     }
                 }
                 lds_order();
@@ -1379,11 +1417,6 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                     }
                 }
                 if constexpr (WANTT) {
-                    // A's next columns: kA+2 (read last step), kA+3 (if <= i)
-                    if constexpr ((adv & 1) != 0) {
-                        zyA = zyA2;
-                        if (zp && kA + 3 <= i) zyA2 = zr[(kA + 3) * zld];
-                    }
                     // B's next column kB+2 = kA-1: the column A finished last step
                     if constexpr ((adv & 2) != 0) zyB = zA1;
                     if constexpr (!hasA) zA1 = zA0;   // after A's end: column i next
@@ -1433,20 +1466,40 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
             };
             using T_ = std::true_type;
             using F_ = std::false_type;
+            using P0 = std::integral_constant<int, 0>;
+            using P1 = std::integral_constant<int, 1>;
+            using A1 = std::integral_constant<int, 1>;
+            using A3 = std::integral_constant<int, 3>;
+            using A2 = std::integral_constant<int, 2>;
             // bulge A alone for three steps, then B starts three positions behind
             if (half == 0) start();
-            for (int tau = 0; tau < 3; ++tau) dstep(T_(), F_(), std::integral_constant<int, 1>(), tau);
+            if constexpr (WANTT) {            // the Z read-ahead register follows tau's parity
+                dstep(T_(), F_(), A1(), P0(), 0);
+                dstep(T_(), F_(), A1(), P1(), 1);
+                dstep(T_(), F_(), A1(), P0(), 2);
+            } else {
+                for (int tau = 0; tau < 3; ++tau) dstep(T_(), F_(), A1(), P0(), tau);
+            }
             if (half == 1) start();
             if constexpr (WANTT) {            // B's columns l, l+1: A's first two
                 zcB = zF;
                 zyB = zA1;
+                int tau = 3;
+                for (; tau + 1 < nsteps - 1; tau += 2) {
+                    dstep(T_(), T_(), A3(), P1(), tau);
+                    dstep(T_(), T_(), A3(), P0(), tau + 1);
+                }
+                if (tau < nsteps - 1) dstep(T_(), T_(), A3(), P1(), tau);
+                if ((nsteps - 1) & 1) dstep(T_(), T_(), A2(), P1(), nsteps - 1);   // A's last step
+                else dstep(T_(), T_(), A2(), P0(), nsteps - 1);
+            } else {
+                for (int tau = 3; tau < nsteps - 1; ++tau) dstep(T_(), T_(), A3(), P0(), tau);
+                dstep(T_(), T_(), A2(), P0(), nsteps - 1);   // A's last step
             }
-            for (int tau = 3; tau < nsteps - 1; ++tau) dstep(T_(), T_(), std::integral_constant<int, 3>(), tau);
-            dstep(T_(), T_(), std::integral_constant<int, 2>(), nsteps - 1);   // A's last step
             finish(0);
-            dstep(F_(), T_(), std::integral_constant<int, 2>(), nsteps);
-            dstep(F_(), T_(), std::integral_constant<int, 2>(), nsteps + 1);
-            dstep(F_(), T_(), std::integral_constant<int, 0>(), nsteps + 2);   // B's last step
+            dstep(F_(), T_(), A2(), P0(), nsteps);
+            dstep(F_(), T_(), A2(), P0(), nsteps + 1);
+            dstep(F_(), T_(), std::integral_constant<int, 0>(), P0(), nsteps + 2);   // B's last step
             finish(1);
         }
         if (!conv) return i + 1;
