@@ -1,8 +1,9 @@
 """GPU checks of the differentiation path through the C ABI.
 
 * forward with Schur vectors: A = U T U^H, U unitary, T upper triangular,
-  diag(T) = eigenvalues (<= 1e-10 vs LAPACK zgeev), same order as the
-  eigenvalue-only kernel with matching lanes per matrix;
+  diag(T) = eigenvalues (<= 1e-10 vs LAPACK zgeev), same values in the same
+  order as the matching eigenvalue-only variant (n <= 32: single-bulge QR,
+  n > 32: two-bulge QR);
 * adjoint: vs the oracle's dL/dA (same method on the CPU), vs CPU
   torch.linalg.eigvals autograd in complex128, and vs central finite
   differences (h = 1e-6) with gradient relative error <= 1e-7.

@@ -10,9 +10,11 @@ Two modes:
 * ``mode="store"`` (default): the forward runs the Schur-vector kernel and
   keeps T and U (2 n^2 complex per matrix) for the backward.
 * ``mode="recompute"``: the forward runs the eigenvalue-only kernel and keeps
-  only A; the backward recomputes the Schur form.  The forward uses the fused
-  variant with the adjoint kernel's lanes-per-matrix so that both passes run
-  the same arithmetic and return the eigenvalues in the same order.
+  only A; the backward recomputes the Schur form.  The forward uses the
+  eigenvalue variant whose Hessenberg and QR sweeps are the backward's (for
+  n <= 32 the fused single-bulge variant with the adjoint kernel's lanes per
+  matrix, for n > 32 the two-bulge QR after the one-wavefront Hessenberg), so
+  both passes return the eigenvalues in the same order.
 """
 from __future__ import annotations
 
@@ -21,11 +23,10 @@ import torch
 from . import (Handle, TokuraError, _P, _check, _dev_check, _stream_ptr, default_handle, lib, workspace_size,
                MAX_N)
 
-# fused same-size variants whose lanes-per-matrix equal the adjoint kernel's
-# (variant ids of the same-size table: 5 = 16x16, 8 = 32x32, 12 = 64x64)
-# eigenvalue variants whose arithmetic is the Schur-form kernels' (n <= 32:
-# fused single-shift QR; n > 32: the two-bulge QR after the one-wavefront
-# Hessenberg), so the forward's order is the recompute backward's
+# eigenvalue variants whose arithmetic is the Schur-form kernels' (ids of the
+# same-size table; n <= 32: fused single-shift QR, 5 = 16x16, 8 = 32x32;
+# n > 32: 22, the two-bulge QR after the one-wavefront Hessenberg), so the
+# forward's order is the recompute backward's
 _MATCHED_VARIANT = {16: 5, 32: 8, 64: 22}
 
 

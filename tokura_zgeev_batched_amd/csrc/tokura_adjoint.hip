@@ -178,11 +178,16 @@ static int adj_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 // (there is no second slot to park T in); everything else goes through
 // k_schur_hq / k_schur_qr with NB lanes per matrix.
 #define TKZ_SCHUR_CFG(X) X(16, 32, true, false) X(32, 64, true, false) X(64, 64, false, true)
-// (NB, lanes per matrix) of the two-launch Schur form: n 33..48 gets its own
-// LDS footprint (packed 19.6 KB: eight matrices per CU in the QR)
-#define TKZ_SPLIT_NB(X) X(16, 16) X(32, 32) X(48, 64) X(64, 64)
+// (NB, lanes per matrix) of the two-launch Schur form: for n > 32 the LDS is
+// sized in steps (the size classes of the ragged paths), so that smaller
+// matrices keep more wavefronts resident in the QR (packed n = 64: 34.3 KB,
+// four per CU; n = 56: six; n = 48: eight)
+#define TKZ_SPLIT_NB(X) X(16, 16) X(32, 32) X(40, 64) X(48, 64) X(56, 64) X(60, 64) X(64, 64)
 
-static int split_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : (n <= 48 ? 48 : 64)); }
+static int split_nb(int n)
+{
+    return n <= 16 ? 16 : (n <= 32 ? 32 : (n <= 40 ? 40 : (n <= 48 ? 48 : (n <= 56 ? 56 : (n <= 60 ? 60 : 64)))));
+}
 
 // Tslot receives the Hessenberg matrix, then T; Uslot receives Q, then U.
 // With VEC, k_vectors_from_schur follows (eigenvectors into VLv / VRv).

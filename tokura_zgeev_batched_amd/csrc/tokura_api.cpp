@@ -36,8 +36,11 @@ int default_variant(int n)
     if (n <= 8) return 1;    // fused 8 x 4
     if (n <= 16) return 5;   // fused 16 x 16
     if (n <= 32) return 14;  // split 32 x 32
-    if (n <= 48) return 19;  // split 48 x 64, two bulges per sweep
-    return 20;               // split 64 x 64, two bulges per sweep
+    if (n <= 40) return 24;  // split, one matrix per wavefront, two bulges per sweep, LDS for n = 40
+    if (n <= 48) return 19;  // the same with LDS for n = 48
+    if (n <= 56) return 23;  // n = 56
+    if (n <= 60) return 25;  // n = 60
+    return 20;               // n = 64
 }
 
 bool variant_fits(int v, int n)

@@ -12,13 +12,14 @@
 //                       the queue is empty, so matrices that need more QR
 //                       sweeps never hold up a whole launch; tiles are queued
 //                       in decreasing size (longest-processing-time first).
-// A tile is the work of one wavefront: 64/LPM matrices of one size class,
-// A1 (n 49..64), A2 (33..48), B (17..32) or C (1..16); the lanes-per-matrix of
-// each class form a "class configuration" that the autotuner picks.  Smaller
+// A tile is the work of one wavefront: 64/LPM matrices of one size class.
+// The classes are n 61..64, 57..60, 49..56, 41..48, 33..40 (one matrix per
+// wavefront, the two-bulge QR: two shifts per sweep, one chain per half-wave,
+// each class with the LDS of its largest n, so that smaller matrices keep more
+// wavefronts resident), B (17..32) and C (1..16).  The lanes-per-matrix of B
+// and C form a "class configuration" that the autotuner picks: smaller
 // classes put more matrices on a wavefront, so the serial reflector chain of
-// every chase step serves several matrices at once; a class with one matrix
-// per wavefront runs the two-bulge QR (two shifts per sweep, one chain per
-// half-wave).
+// every chase step serves several matrices at once.
 //
 // `work` layout (built on the host by tokura_zgeev_ragged_prepare):
 //   RaggedHeader | tiles int4[ntiles] | order int[B] | n int[B] |
@@ -34,27 +35,37 @@
 
 namespace tkz {
 
+// Size classes, largest n first (the order of order[], tiles and launches).
+// Classes 0..4 (n > 32) run one matrix per wavefront with LDS for cls_hi(c).
+constexpr int kNumCls = 7, kClsB = 5, kClsC = 6;
+__host__ __device__ constexpr int cls_hi(int c)
+{
+    return c == 0 ? 64 : (c == 1 ? 60 : (c == 2 ? 56 : (c == 3 ? 48 : (c == 4 ? 40 : (c == 5 ? 32 : 16)))));
+}
+__host__ __device__ constexpr int cls_of(int n)
+{
+    return n > 60 ? 0 : (n > 56 ? 1 : (n > 48 ? 2 : (n > 40 ? 3 : (n > 32 ? 4 : (n > 16 ? 5 : 6)))));
+}
+
 struct RaggedHeader {
-    int qctr[4];       // QR queue heads, one per size class
+    int qctr[8];       // QR queue heads, one per size class
+    int cls_end[8];    // class c's matrices: order[cls_end[c-1] .. cls_end[c]) (order[0 ..) for c = 0)
+    int tile_cls[8];   // class c's tiles: [tile_cls[c], tile_cls[c+1]); tile_cls[kNumCls] = ntiles
     int ntiles;
     int batch;
     int cfg;
-    int nbig;          // matrices of classes A1/A2 (n > 32): order[0 .. nbig)
+    int tile_small0;   // first tile of classes B/C
     long long off_tiles, off_order, off_n, off_moff, off_eoff, off_hoff, off_H;
     long long total_bytes;
-    int tile_small0;   // first tile of classes B/C
-    int n16;           // matrices with n > 16: order[0 .. n16)
-    int tile_cls[5];   // tiles of class c: [tile_cls[c], tile_cls[c+1])
-    int n48;           // matrices with n > 48: order[0 .. n48)
 };
-static_assert(sizeof(RaggedHeader) == 128, "header size");
+static_assert(sizeof(RaggedHeader) == 176, "header size");
 
-// lanes per matrix of classes A1 (49..64), A2 (33..48), B (17..32), C (1..16)
+// lanes per matrix of the classes n > 48, 33..48, B (17..32), C (1..16)
 #define TKZ_RAGGED_CFGS(X) \
     X(0, 64, 64, 16, 8)    \
     X(1, 64, 64, 32, 8)    \
-    X(2, 64, 32, 16, 8)    \
-    X(3, 64, 64, 32, 16)
+    X(2, 64, 64, 32, 16)   \
+    X(3, 64, 64, 16, 16)
 static const int g_cfg_lpm[][4] = {
 #define X(id, a1, a2, b, c) {a1, a2, b, c},
     TKZ_RAGGED_CFGS(X)
@@ -62,7 +73,7 @@ static const int g_cfg_lpm[][4] = {
 };
 static constexpr int kNumRaggedCfg = 4;
 
-__host__ __device__ constexpr int cls_of(int n) { return n > 48 ? 0 : (n > 32 ? 1 : (n > 16 ? 2 : 3)); }
+static int cfg_lpm(int cfg, int cls) { return cls < kClsB ? 64 : g_cfg_lpm[cfg][cls == kClsB ? 2 : 3]; }
 
 struct RaggedView {
     RaggedHeader *h;
@@ -86,7 +97,14 @@ struct RaggedView {
 __global__ void k_ragged_reset(char *work)
 {
     RaggedHeader *h = (RaggedHeader *)work;
-    for (int c = 0; c < 4; ++c) h->qctr[c] = 0;
+    for (int c = 0; c < 8; ++c) h->qctr[c] = 0;
+}
+
+// matrices of size class c: order[lo .. hi)
+__device__ __forceinline__ void cls_range(const RaggedView &V, int c, int &lo, int &hi)
+{
+    lo = c == 0 ? 0 : V.h->cls_end[c - 1];
+    hi = V.h->cls_end[c];
 }
 
 template <int NB, int LPM>
@@ -152,23 +170,23 @@ __global__ __launch_bounds__(64) void k_ragged_hess(const zd *__restrict__ A, ch
     const int nt = V.h->ntiles;
     for (int tile = V.h->tile_small0 + blockIdx.x; tile < nt; tile += gridDim.x) {
         const int4 td = V.tiles[tile];
-        if (td.x == 2) hess_tile<32, LB>(lds, V, td, A, flag, bal);
+        if (td.x == kClsB) hess_tile<32, LB>(lds, V, td, A, flag, bal);
         else hess_tile<16, LC>(lds, V, td, A, flag, bal);
         lds_order();
     }
 }
 
-// Hessenberg of a large class (NB 64: A1, n 49..64; NB 48: A2, n 33..48):
-// one 256-thread workgroup per matrix, quads of threads sharing each
-// row/column, LDS sized for the class.
-template <int NB>
+// Hessenberg of a large class (n > 32): one 256-thread workgroup per
+// matrix, quads of threads sharing each row/column, LDS sized for the class.
+template <int NB, int CLS>
 __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A, char *work,
                                                         int8_t *__restrict__ flag, int bal)
 {
     __shared__ zd lds[NB * (NB | 1)];
     __shared__ double red[8];
     const RaggedView V(work);
-    const int lo = NB == 64 ? 0 : V.h->n48, hi = NB == 64 ? V.h->n48 : V.h->nbig;
+    int lo, hi;
+    cls_range(V, CLS, lo, hi);
     const int tid = threadIdx.x;
     for (int j = lo + (int)blockIdx.x; j < hi; j += gridDim.x) {
         const int b = V.order[j];
@@ -218,35 +236,45 @@ __global__ __launch_bounds__(64) void k_ragged_qr(char *work, zd *__restrict__ e
     }
 }
 
+// The classes n > 32: (class, NB)
+#define TKZ_BIG_CLS(X) X(0, 64) X(1, 60) X(2, 56) X(3, 48) X(4, 40)
+
 template <int LA1, int LA2, int LB, int LC>
 static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int bal, int cus,
                                     hipStream_t s)
 {
+    static_assert(LA1 == 64 && LA2 == 64, "classes n > 32: one matrix per wavefront");
     hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(1), 0, s, work);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // resident workgroups per CU, queried once (outside any graph capture the
     // first call is made from tokura_zgeev_ragged_graph's capture stream, where
     // occupancy queries are not stream operations and are allowed)
-    static const int occ_h = resident_blocks(k_ragged_hess<LB, LC>);
-    static const int occ_w1 = resident_blocks(k_ragged_hess_wg<64>, 256);
-    static const int occ_w2 = resident_blocks(k_ragged_hess_wg<48>, 256);
-    static const int occ_q0 = resident_blocks(k_ragged_qr<0, 64, LA1>);
-    static const int occ_q1 = resident_blocks(k_ragged_qr<1, 48, LA2>);
-    static const int occ_q2 = resident_blocks(k_ragged_qr<2, 32, LB>);
-    static const int occ_q3 = resident_blocks(k_ragged_qr<3, 16, LC>);
-    hipLaunchKernelGGL((k_ragged_hess_wg<64>), dim3(cus * occ_w1 * 4), dim3(256), 0, s, A, work, flag, bal);
-    hipLaunchKernelGGL((k_ragged_hess_wg<48>), dim3(cus * occ_w2 * 4), dim3(256), 0, s, A, work, flag, bal);
+#define X(c, nb)                                                                                          \
+    {                                                                                                      \
+        static const int occ = resident_blocks(k_ragged_hess_wg<nb, c>, 256);                              \
+        hipLaunchKernelGGL((k_ragged_hess_wg<nb, c>), dim3(cus * occ * 4), dim3(256), 0, s, A, work, flag, bal); \
+    }
+    TKZ_BIG_CLS(X)
+#undef X
     e = hipGetLastError();
     if (e != hipSuccess) return e;
+    static const int occ_h = resident_blocks(k_ragged_hess<LB, LC>);
     hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag, bal);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     // largest class first: the tail of one launch is at most one tile of it
-    hipLaunchKernelGGL((k_ragged_qr<0, 64, LA1>), dim3(cus * occ_q0), dim3(64), 0, s, work, eig, flag, itmax);
-    hipLaunchKernelGGL((k_ragged_qr<1, 48, LA2>), dim3(cus * occ_q1), dim3(64), 0, s, work, eig, flag, itmax);
-    hipLaunchKernelGGL((k_ragged_qr<2, 32, LB>), dim3(cus * occ_q2), dim3(64), 0, s, work, eig, flag, itmax);
-    hipLaunchKernelGGL((k_ragged_qr<3, 16, LC>), dim3(cus * occ_q3), dim3(64), 0, s, work, eig, flag, itmax);
+#define X(c, nb)                                                                                          \
+    {                                                                                                      \
+        static const int occ = resident_blocks(k_ragged_qr<c, nb, 64>);                                    \
+        hipLaunchKernelGGL((k_ragged_qr<c, nb, 64>), dim3(cus * occ), dim3(64), 0, s, work, eig, flag, itmax); \
+    }
+    TKZ_BIG_CLS(X)
+#undef X
+    static const int occ_q5 = resident_blocks(k_ragged_qr<kClsB, 32, LB>);
+    static const int occ_q6 = resident_blocks(k_ragged_qr<kClsC, 16, LC>);
+    hipLaunchKernelGGL((k_ragged_qr<kClsB, 32, LB>), dim3(cus * occ_q5), dim3(64), 0, s, work, eig, flag, itmax);
+    hipLaunchKernelGGL((k_ragged_qr<kClsC, 16, LC>), dim3(cus * occ_q6), dim3(64), 0, s, work, eig, flag, itmax);
     return hipGetLastError();
 }
 
@@ -264,21 +292,15 @@ static hipError_t launch_ragged(int cfg, const zd *A, char *work, zd *eig, int8_
 // ---- differentiation of a ragged batch ------------------------------------
 // One launch per size class, each with the LDS footprint and lanes-per-matrix
 // of the same-size Schur / adjoint kernels of that class, so every matrix gets
-// the same arithmetic as in a same-size call: the Schur form in the classes
-// n > 48, 33..48, 17..32, <= 16 (as the eigenvalue path), the backward in
-// n > 32, 17..32, <= 16.  order[] lists each class's matrices contiguously
-// (n descending); the Schur form of matrix b is at 2*moff[b] (T then U), dA
-// at moff[b], the eigenvalues and cotangents at eoff[b].
+// the same arithmetic as in a same-size call: the Schur form in the size
+// classes of the eigenvalue path, the backward in n > 32, 17..32, <= 16.
+// order[] lists each class's matrices contiguously (n descending); the Schur
+// form of matrix b is at 2*moff[b] (T then U), dA at moff[b], the eigenvalues
+// and cotangents at eoff[b].
 __device__ __forceinline__ void class_range(const RaggedView &V, int cls, int &lo, int &hi)
 {
-    lo = cls == 0 ? 0 : (cls == 1 ? V.h->nbig : V.h->n16);
-    hi = cls == 0 ? V.h->nbig : (cls == 1 ? V.h->n16 : V.h->batch);
-}
-
-__device__ __forceinline__ void schur_class_range(const RaggedView &V, int cls, int &lo, int &hi)
-{
-    lo = cls == 0 ? 0 : (cls == 1 ? V.h->n48 : (cls == 2 ? V.h->nbig : V.h->n16));
-    hi = cls == 0 ? V.h->n48 : (cls == 1 ? V.h->nbig : (cls == 2 ? V.h->n16 : V.h->batch));
+    lo = cls == 0 ? 0 : V.h->cls_end[kClsB - 2 + cls];
+    hi = V.h->cls_end[kClsB - 1 + cls];
 }
 
 // The Schur forward of size class CLS in two launches, as the same-size path:
@@ -293,7 +315,7 @@ __global__ __launch_bounds__(64) void k_ragged_schur_hq(char *work, const zd *__
     __shared__ zd lds[GPW * PER];
     const RaggedView V(work);
     int lo, hi;
-    schur_class_range(V, CLS, lo, hi);
+    cls_range(V, CLS, lo, hi);
     const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
     zd *H = lds + g * PER;
     for (int base = lo + (int)blockIdx.x * GPW; base < hi; base += (int)gridDim.x * GPW) {
@@ -325,7 +347,7 @@ __global__ __launch_bounds__(64) void k_ragged_schur_qr(char *work, zd *__restri
     __shared__ zd lds[GPW * PER];
     const RaggedView V(work);
     int lo, hi;
-    schur_class_range(V, CLS, lo, hi);
+    cls_range(V, CLS, lo, hi);
     const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
     for (;;) {
         int k = 0;
@@ -406,9 +428,12 @@ static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schu
                            flag, itmax);                                                                       \
     }
     TKZ_RS(64, 64, 0)
-    TKZ_RS(48, 64, 1)
-    TKZ_RS(32, 32, 2)
-    TKZ_RS(16, 16, 3)
+    TKZ_RS(60, 64, 1)
+    TKZ_RS(56, 64, 2)
+    TKZ_RS(48, 64, 3)
+    TKZ_RS(40, 64, 4)
+    TKZ_RS(32, 32, kClsB)
+    TKZ_RS(16, 16, kClsC)
 #undef TKZ_RS
     return hipGetLastError();
 }
@@ -469,7 +494,7 @@ static int build_schedule(int batch, const int64_t *offsets, int cfg, Schedule &
     int i = 0;
     while (i < batch) {
         const int c = cls_of(S.nsz[S.order[i]]);
-        const int per = 64 / g_cfg_lpm[cfg][c];
+        const int per = 64 / cfg_lpm(cfg, c);
         int j = i;
         while (j < batch && j - i < per && cls_of(S.nsz[S.order[j]]) == c) ++j;
         S.tiles.insert(S.tiles.end(), {c, j - i, i, S.nsz[S.order[i]]});
@@ -477,17 +502,15 @@ static int build_schedule(int batch, const int64_t *offsets, int cfg, Schedule &
     }
     const int ntiles = (int)(S.tiles.size() / 4);
     std::memset(&S.hdr, 0, sizeof S.hdr);
-    S.hdr.n48 = 0;
-    while (S.hdr.n48 < batch && S.nsz[S.order[S.hdr.n48]] > 48) ++S.hdr.n48;
-    S.hdr.nbig = S.hdr.n48;
-    while (S.hdr.nbig < batch && S.nsz[S.order[S.hdr.nbig]] > 32) ++S.hdr.nbig;
-    S.hdr.n16 = S.hdr.nbig;
-    while (S.hdr.n16 < batch && S.nsz[S.order[S.hdr.n16]] > 16) ++S.hdr.n16;
+    for (int c = 0, k = 0; c < kNumCls; ++c) {   // order[] is in class order
+        while (k < batch && cls_of(S.nsz[S.order[k]]) <= c) ++k;
+        S.hdr.cls_end[c] = k;
+    }
     S.hdr.tile_small0 = 0;
-    while (S.hdr.tile_small0 < ntiles && S.tiles[4 * S.hdr.tile_small0] < 2) ++S.hdr.tile_small0;
-    for (int c = 0, k = 0; c <= 4; ++c) {   // tiles are in class order
+    while (S.hdr.tile_small0 < ntiles && S.tiles[4 * S.hdr.tile_small0] < kClsB) ++S.hdr.tile_small0;
+    for (int c = 0, k = 0; c <= kNumCls; ++c) {   // tiles are in class order
         while (k < ntiles && S.tiles[4 * k] < c) ++k;
-        S.hdr.tile_cls[c] = c == 4 ? ntiles : k;
+        S.hdr.tile_cls[c] = c == kNumCls ? ntiles : k;
     }
     S.hdr.ntiles = ntiles;
     S.hdr.batch = batch;
