@@ -133,12 +133,12 @@ def test_adjoint_vs_finite_differences():
     assert err4.max() <= 1e-6
 
 
-@pytest.mark.parametrize("n", [33, 50, 64])
+@pytest.mark.parametrize("n", [17, 32, 33, 40, 47, 50, 64])
 def test_store_and_recompute_gradients_bitwise(n):
-    """The stored-Schur backward (for n > 32 a 256-thread workgroup per matrix,
-    after a forward that accumulates U in global memory) and the recompute
-    backward (one wavefront, everything in LDS) share the per-element
-    arithmetic: identical dL/dA bits."""
+    """The stored-Schur backward (for n > 16 a 256-thread workgroup per matrix,
+    LDS sized for n <= 32, 40, 48 or 64, after a forward that accumulates U in
+    global memory) and the recompute backward (one wavefront, everything in
+    LDS) share the per-element arithmetic: identical dL/dA bits."""
     rng = np.random.default_rng(900 + n)
     M = rand(rng, 40, n)
     g = torch.tensor(rng.standard_normal((40, n)) + 1j * rng.standard_normal((40, n)), device="cuda")

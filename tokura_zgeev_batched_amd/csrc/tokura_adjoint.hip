@@ -154,15 +154,16 @@ __global__ __launch_bounds__(64) void k_zgeev_adjoint(int n, int batch, const zd
                                     flag + mat, itmax);
 }
 
-// Stored-Schur backward for n > 32: one matrix per workgroup of
-// TKZ_ADJ_WG_NT threads (the two LDS buffers allow one matrix per CU, so the
-// extra wavefronts share its work instead of idling the CU).
-template <int NT>
+// Stored-Schur backward for n > 16: one matrix per workgroup of
+// TKZ_ADJ_WG_NT threads, LDS for NB (at NB = 64 the two buffers allow one
+// matrix per CU, so the extra wavefronts share its work instead of idling the
+// CU; NB = 32 fits four workgroups).
+template <int NT, int NB>
 __global__ __launch_bounds__(NT) void k_zgeev_adjoint_wg(int n, const zd *__restrict__ schur,
                                                          const zd *__restrict__ gbar, zd *__restrict__ dA,
                                                          const int8_t *__restrict__ flag)
 {
-    using G = AdjGeom<64>;
+    using G = AdjGeom<NB>;
     __shared__ zd lds[G::PER];
     const long long mat = blockIdx.x, nn = (long long)n * n;
     zd *B1 = lds, *B2 = lds + G::MAT, *gv = B2 + G::MAT;
@@ -244,9 +245,18 @@ static hipError_t launch_vectors(int n, int batch, const zd *A, zd *eig, zd *VL,
 static hipError_t launch_adjoint(int n, int batch, const zd *A, const zd *schur, const zd *gbar, zd *dA,
                                  int8_t *flag, int itmax, hipStream_t s)
 {
-    if (schur && adj_nb(n) == 64) {
-        hipLaunchKernelGGL((k_zgeev_adjoint_wg<TKZ_ADJ_WG_NT>), dim3(batch), dim3(TKZ_ADJ_WG_NT), 0, s, n, schur,
-                           gbar, dA, flag);
+    if (schur && n > 16) {   // one 256-thread workgroup per matrix, LDS sized in steps (two per CU for n <= 48,
+                             // three for n <= 40, four for n <= 32)
+        const int nb = n <= 32 ? 32 : (n <= 40 ? 40 : (n <= 48 ? 48 : 64));
+        switch (nb) {
+#define X(b)                                                                                                   \
+    case b:                                                                                                    \
+        hipLaunchKernelGGL((k_zgeev_adjoint_wg<TKZ_ADJ_WG_NT, b>), dim3(batch), dim3(TKZ_ADJ_WG_NT), 0, s, n, schur, \
+                           gbar, dA, flag);                                                                    \
+        break;
+            X(32) X(40) X(48) X(64)
+#undef X
+        }
         return hipGetLastError();
     }
     switch (adj_nb(n)) {

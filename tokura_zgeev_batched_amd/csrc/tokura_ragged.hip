@@ -392,18 +392,17 @@ __global__ __launch_bounds__(64) void k_ragged_adjoint(char *work, const zd *__r
     }
 }
 
-// class 0 (n > 32) of the backward: one matrix at a time per workgroup of
-// TKZ_ADJ_WG_NT threads (adjoint_wg)
-template <int NT>
+// The backward of size classes C0..C1 (n > 16): one matrix at a time per
+// workgroup of NT threads (adjoint_wg), LDS sized for NB
+template <int NT, int NB, int C0, int C1>
 __global__ __launch_bounds__(NT) void k_ragged_adjoint_wg(char *work, const zd *__restrict__ schur,
                                                           const zd *__restrict__ gbar, zd *dA,
                                                           const int8_t *__restrict__ flag)
 {
-    using G = AdjGeom<64>;
+    using G = AdjGeom<NB>;
     __shared__ zd lds[G::PER];
     const RaggedView V(work);
-    int lo, hi;
-    class_range(V, 0, lo, hi);
+    const int lo = C0 == 0 ? 0 : V.h->cls_end[C0 - 1], hi = V.h->cls_end[C1];
     zd *B1 = lds, *B2 = lds + G::MAT, *gv = B2 + G::MAT;
     for (int j = lo + (int)blockIdx.x; j < hi; j += (int)gridDim.x) {
         const int b = V.order[j];
@@ -441,12 +440,21 @@ static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schu
 static hipError_t launch_ragged_adjoint(const zd *schur, const zd *gbar, zd *dA, char *work, int8_t *flag, int itmax,
                                         int cus, hipStream_t s)
 {
-    static const int o0 = resident_blocks(k_ragged_adjoint_wg<TKZ_ADJ_WG_NT>, TKZ_ADJ_WG_NT);
-    static const int o1 = resident_blocks(k_ragged_adjoint<32, 32, 1>);
+    // n > 16 by 256-thread workgroups; the LDS is sized in steps where a
+    // smaller footprint keeps more workgroups resident (n <= 48: two per CU,
+    // n <= 40: three, n <= 32: four; the classes above 48 share the n = 64 size)
+#define X(nb, c0, c1)                                                                                      \
+    {                                                                                                       \
+        static const int o = resident_blocks(k_ragged_adjoint_wg<TKZ_ADJ_WG_NT, nb, c0, c1>, TKZ_ADJ_WG_NT); \
+        hipLaunchKernelGGL((k_ragged_adjoint_wg<TKZ_ADJ_WG_NT, nb, c0, c1>), dim3(cus * o), dim3(TKZ_ADJ_WG_NT), 0, s, \
+                           work, schur, gbar, dA, flag);                                                   \
+    }
+    X(64, 0, 2)
+    X(48, 3, 3)
+    X(40, 4, 4)
+    X(32, kClsB, kClsB)
+#undef X
     static const int o2 = resident_blocks(k_ragged_adjoint<16, 16, 2>);
-    hipLaunchKernelGGL((k_ragged_adjoint_wg<TKZ_ADJ_WG_NT>), dim3(cus * o0), dim3(TKZ_ADJ_WG_NT), 0, s, work, schur,
-                       gbar, dA, flag);
-    hipLaunchKernelGGL((k_ragged_adjoint<32, 32, 1>), dim3(cus * o1), dim3(64), 0, s, work, schur, gbar, dA, flag, itmax);
     hipLaunchKernelGGL((k_ragged_adjoint<16, 16, 2>), dim3(cus * o2), dim3(64), 0, s, work, schur, gbar, dA, flag, itmax);
     return hipGetLastError();
 }
