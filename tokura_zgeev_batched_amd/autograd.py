@@ -13,8 +13,8 @@ Two modes:
   only A; the backward recomputes the Schur form.  The forward uses the
   eigenvalue variant whose Hessenberg and QR sweeps are the backward's (for
   n <= 32 the fused single-bulge variant with the adjoint kernel's lanes per
-  matrix, for n > 32 the two-bulge QR after the one-wavefront Hessenberg), so
-  both passes return the eigenvalues in the same order.
+  matrix, for n > 32 the two-bulge QR after the 256-thread workgroup
+  Hessenberg), so both passes return the eigenvalues in the same order.
 """
 from __future__ import annotations
 
@@ -25,9 +25,9 @@ from . import (Handle, TokuraError, _P, _check, _dev_check, _stream_ptr, default
 
 # eigenvalue variants whose arithmetic is the Schur-form kernels' (ids of the
 # same-size table; n <= 32: fused single-shift QR, 5 = 16x16, 8 = 32x32;
-# n > 32: 22, the two-bulge QR after the one-wavefront Hessenberg), so the
-# forward's order is the recompute backward's
-_MATCHED_VARIANT = {16: 5, 32: 8, 64: 22}
+# n > 32: 20, the two-bulge QR after the 256-thread workgroup Hessenberg), so
+# the forward's order is the recompute backward's
+_MATCHED_VARIANT = {16: 5, 32: 8, 64: 20}
 
 
 def _adj_nb(n: int) -> int:

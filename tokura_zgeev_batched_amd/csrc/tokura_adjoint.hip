@@ -64,6 +64,69 @@ __global__ __launch_bounds__(64) void k_zgeev_vectors(int n, int batch, const zd
     vectors_from_schur<LPM, GZ>(B1, lay, U, ul, n, t, VL ? VL + mat * nn : nullptr, VR ? VR + mat * nn : nullptr);
 }
 
+// Eigenvectors for n > 32 without a workspace, one matrix per workgroup of NT
+// threads: schur_wg with U accumulated in the home of VR (VL when only left
+// vectors are asked for) and T in LDS, then vectors_from_schur by the first
+// wavefront (the arithmetic of the two-launch path and k_vectors_from_schur).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_zgeev_vectors_wg(int n, int batch, const zd *__restrict__ A,
+                                                        zd *__restrict__ eig, zd *VL, zd *VR,
+                                                        int8_t *__restrict__ flag, int itmax)
+{
+    __shared__ zd lds[64 * 65 + 64];
+    __shared__ double red[8];
+    __shared__ int ish;
+    const long long mat = blockIdx.x, nn = (long long)n * n;
+    const int tid = (int)threadIdx.x;
+    zd *B1 = lds, *taus = lds + 64 * 65;
+    zd *U = (VR ? VR : VL) + mat * nn;
+    const FullLayout lay{n | 1};
+    const int info = schur_wg<NT>(B1, U, n, taus, red, &ish, A + mat * nn, n, itmax);
+    zd *out = eig + mat * n;
+    if (info < 0) fill_nan<NT>(out, n, tid);
+    else
+        for (int r = tid; r < n; r += NT) out[r] = B1[lay(r, r)];
+    if (tid == 0) flag[mat] = info < 0 ? 2 : (info != 0 ? 1 : 0);
+    if (info != 0) {
+        if (VR) fill_nan<NT>(VR + mat * nn, (int)nn, tid);
+        if (VL) fill_nan<NT>(VL + mat * nn, (int)nn, tid);
+        return;
+    }
+    if (tid < 64) vectors_from_schur<64, true>(B1, lay, U, FullLayout{n}, n, tid, VL ? VL + mat * nn : nullptr,
+                                               VR ? VR + mat * nn : nullptr);
+}
+
+// Recompute backward for n > 32, one matrix per workgroup of NT threads:
+// schur_wg with U in LDS (the arithmetic of the two-launch forward), U
+// parked in dA, then adjoint_wg's phases.  Writes the flag (0/1/2); flagged
+// matrices get dA = NaN.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_zgeev_adjoint_rc_wg(int n, const zd *__restrict__ A,
+                                                           const zd *__restrict__ gbar, zd *dA,
+                                                           int8_t *__restrict__ flag, int itmax)
+{
+    using G = AdjGeom<64>;
+    __shared__ zd lds[G::PER];
+    __shared__ double red[8];
+    __shared__ int ish;
+    const long long mat = blockIdx.x, nn = (long long)n * n;
+    const int tid = (int)threadIdx.x;
+    zd *B1 = lds, *B2 = lds + G::MAT, *taus = B2 + G::MAT, *gv = taus + 64;
+    const FullLayout lay{n | 1};
+    zd *dst = dA + mat * nn;
+    const int info = schur_wg<NT>(B1, B2, lay.ld, taus, red, &ish, A + mat * nn, n, itmax);
+    const int f = info < 0 ? 2 : (info != 0 ? 1 : 0);
+    if (tid == 0) flag[mat] = (int8_t)f;
+    if (f != 0) {
+        fill_nan<NT>(dst, (int)nn, tid);
+        return;
+    }
+    for (int e = tid; e < (int)nn; e += NT) dst[e] = B2[lay(e % n, e / n)];   // park U in dA
+    for (int i = tid; i < n; i += NT) gv[i] = gbar[mat * n + i];
+    __syncthreads();
+    adjoint_wg_phases<NT>(B1, B2, gv, n, dst, dst);
+}
+
 // Schur form in two launches (schur_hq_one, schur_qr_one), LPM lanes per
 // matrix.  The matrix's T and U slots are Tout + mat*tstride and
 // Uout + mat*ustride: the Schur-form output (T then U per matrix) or, for
@@ -88,6 +151,29 @@ __global__ __launch_bounds__(64) void k_schur_hq(int n, int batch, const zd *__r
         fill_nan<LPM>(Tm, (int)nn, t);
         fill_nan<LPM>(Um, (int)nn, t);
         if (t == 0) flag[mat] = 2;
+    }
+}
+
+// First half for n > 32 by a workgroup of NT threads per matrix
+// (schur_hq_wg): the 256-thread Hessenberg of the eigenvalue path with kept
+// reflectors, then Q, LDS sized for NB.
+template <int NB, int NT>
+__global__ __launch_bounds__(NT) void k_schur_hq_wg(int n, int batch, const zd *__restrict__ A, zd *__restrict__ eig,
+                                                    zd *Tout, long long tstride, zd *Uout, long long ustride,
+                                                    int8_t *__restrict__ flag)
+{
+    __shared__ zd lds[NB * (NB | 1) + NB];
+    __shared__ double red[8];
+    const long long mat = blockIdx.x, nn = (long long)n * n;
+    const int tid = (int)threadIdx.x;
+    zd *Tm = Tout + mat * tstride, *Um = Uout + mat * ustride;
+    if (schur_hq_wg<NT, NT / 64>(lds, lds + NB * (NB | 1), red, A + mat * nn, n, Tm, Um)) {
+        if (tid == 0) flag[mat] = 0;
+    } else {
+        fill_nan<NT>(eig + mat * n, n, tid);
+        fill_nan<NT>(Tm, (int)nn, tid);
+        fill_nan<NT>(Um, (int)nn, tid);
+        if (tid == 0) flag[mat] = 2;
     }
 }
 
@@ -170,7 +256,7 @@ __global__ __launch_bounds__(NT) void k_zgeev_adjoint_wg(int n, const zd *__rest
     adjoint_wg<NT>(B1, B2, gv, n, schur + mat * 2 * nn, gbar + mat * n, dA + mat * nn, flag[mat]);
 }
 
-#define TKZ_ADJ_NB(X) X(16, 16) X(32, 32) X(64, 64)
+#define TKZ_ADJ_NB(X) X(16, 16) X(32, 32)   // n > 32: the workgroup kernels
 
 static int adj_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 
@@ -178,7 +264,7 @@ static int adj_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 // half-wave, Z in global memory), used by one-sided eigenvector requests
 // (there is no second slot to park T in); everything else goes through
 // k_schur_hq / k_schur_qr with NB lanes per matrix.
-#define TKZ_SCHUR_CFG(X) X(16, 32, true, false) X(32, 64, true, false) X(64, 64, false, true)
+#define TKZ_SCHUR_CFG(X) X(16, 32, true, false) X(32, 64, true, false)   // n > 32: k_zgeev_vectors_wg
 // (NB, lanes per matrix) of the two-launch Schur form: for n > 32 the LDS is
 // sized in steps (the size classes of the ragged paths), so that smaller
 // matrices keep more wavefronts resident in the QR (packed n = 64: 34.3 KB,
@@ -200,8 +286,12 @@ static hipError_t launch_schur_split(int n, int batch, const zd *A, zd *eig, zd 
 #define X(nb, lpm)                                                                                              \
     case nb: {                                                                                                  \
         const dim3 grid((batch + 64 / lpm - 1) / (64 / lpm));                                                   \
-        hipLaunchKernelGGL((k_schur_hq<nb, lpm>), grid, dim3(64), 0, s, n, batch, A, eig, Tout, tstride, Uout,   \
-                           ustride, flag);                                                                      \
+        if (nb > 32)                                                                                            \
+            hipLaunchKernelGGL((k_schur_hq_wg<nb, TKZ_ADJ_WG_NT>), dim3(batch), dim3(TKZ_ADJ_WG_NT), 0, s, n, batch, \
+                               A, eig, Tout, tstride, Uout, ustride, flag);                                     \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_schur_hq<nb, lpm>), grid, dim3(64), 0, s, n, batch, A, eig, Tout, tstride, Uout, \
+                               ustride, flag);                                                                  \
         hipLaunchKernelGGL((k_schur_qr<nb, lpm>), grid, dim3(64), 0, s, n, batch, eig, Tout, tstride, Uout,      \
                            ustride, flag, itmax);                                                               \
         if (vec)                                                                                                \
@@ -230,6 +320,11 @@ static hipError_t launch_vectors(int n, int batch, const zd *A, zd *eig, zd *VL,
     const long long nn = (long long)n * n;
     zd *Tslot = (VL && VR) ? VL : work;
     if (Tslot) return launch_schur_split(n, batch, A, eig, Tslot, nn, VR ? VR : VL, nn, flag, itmax, s, true, VL, VR);
+    if (n > 32) {
+        hipLaunchKernelGGL((k_zgeev_vectors_wg<TKZ_ADJ_WG_NT>), dim3(batch), dim3(TKZ_ADJ_WG_NT), 0, s, n, batch, A, eig,
+                           VL, VR, flag, itmax);
+        return hipGetLastError();
+    }
     switch (adj_nb(n)) {
 #define X(nb, lpm, zh, gz)                                                                                        \
     case nb:                                                                                                      \
@@ -257,6 +352,11 @@ static hipError_t launch_adjoint(int n, int batch, const zd *A, const zd *schur,
             X(32) X(40) X(48) X(64)
 #undef X
         }
+        return hipGetLastError();
+    }
+    if (!schur && n > 32) {   // recompute: one 256-thread workgroup per matrix
+        hipLaunchKernelGGL((k_zgeev_adjoint_rc_wg<TKZ_ADJ_WG_NT>), dim3(batch), dim3(TKZ_ADJ_WG_NT), 0, s, n, A, gbar,
+                           dA, flag, itmax);
         return hipGetLastError();
     }
     switch (adj_nb(n)) {

@@ -336,6 +336,35 @@ __global__ __launch_bounds__(64) void k_ragged_schur_hq(char *work, const zd *__
     }
 }
 
+// The first half for the classes n > 32: one matrix at a time per workgroup
+// of TKZ_ADJ_WG_NT threads (schur_hq_wg, the arithmetic of k_schur_hq_wg).
+template <int NB, int CLS>
+__global__ __launch_bounds__(TKZ_ADJ_WG_NT) void k_ragged_schur_hq_wg(char *work, const zd *__restrict__ A,
+                                                                     zd *__restrict__ eig, zd *schur,
+                                                                     int8_t *__restrict__ flag)
+{
+    constexpr int NT = TKZ_ADJ_WG_NT;
+    __shared__ zd lds[NB * (NB | 1) + NB];
+    __shared__ double red[8];
+    const RaggedView V(work);
+    int lo, hi;
+    cls_range(V, CLS, lo, hi);
+    const int tid = (int)threadIdx.x;
+    for (int j = lo + (int)blockIdx.x; j < hi; j += (int)gridDim.x) {
+        const int b = V.order[j], n = V.nsz[b];
+        const long long nn = (long long)n * n;
+        zd *Tm = schur + 2 * V.moff[b], *Um = Tm + nn;
+        if (schur_hq_wg<NT, NT / 64>(lds, lds + NB * (NB | 1), red, A + V.moff[b], n, Tm, Um)) {
+            if (tid == 0) flag[b] = 0;
+        } else {
+            fill_nan<NT>(eig + V.eoff[b], n, tid);
+            fill_nan<NT>(Tm, (int)(2 * nn), tid);
+            if (tid == 0) flag[b] = 2;
+        }
+        __syncthreads();
+    }
+}
+
 // The QR of a Schur class as a persistent kernel: every wavefront pulls the
 // class's next GPW matrices from the class's queue (qctr[CLS], zeroed by
 // k_ragged_reset) until the class is done.
@@ -419,10 +448,16 @@ static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schu
     hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(1), 0, s, work);
 #define TKZ_RS(nb, lpm, cls)                                                                                  \
     {                                                                                                          \
-        static const int oh = resident_blocks(k_ragged_schur_hq<nb, lpm, cls>);                                \
+        if constexpr (nb > 32) {                                                                               \
+            static const int oh = resident_blocks(k_ragged_schur_hq_wg<nb, cls>, TKZ_ADJ_WG_NT);               \
+            hipLaunchKernelGGL((k_ragged_schur_hq_wg<nb, cls>), dim3(cus * oh * 4), dim3(TKZ_ADJ_WG_NT), 0, s,   \
+                               work, A, eig, schur, flag);                                                     \
+        } else {                                                                                               \
+            static const int oh = resident_blocks(k_ragged_schur_hq<nb, lpm, cls>);                            \
+            hipLaunchKernelGGL((k_ragged_schur_hq<nb, lpm, cls>), dim3(cus * oh), dim3(64), 0, s, work, A, eig, \
+                               schur, flag);                                                                   \
+        }                                                                                                      \
         static const int oq = resident_blocks(k_ragged_schur_qr<nb, lpm, cls>);                                \
-        hipLaunchKernelGGL((k_ragged_schur_hq<nb, lpm, cls>), dim3(cus * oh), dim3(64), 0, s, work, A, eig,     \
-                           schur, flag);                                                                       \
         hipLaunchKernelGGL((k_ragged_schur_qr<nb, lpm, cls>), dim3(cus * oq), dim3(64), 0, s, work, eig, schur, \
                            flag, itmax);                                                                       \
     }

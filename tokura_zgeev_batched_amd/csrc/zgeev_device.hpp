@@ -306,9 +306,11 @@ __device__ void hessenberg(zd *A, const FullLayout lay, int n, int t, zd *taus =
 // i+1 in the left update also form the sum of squares and max of its tail
 // (the next reflector's x), so no separate norm pass is needed; entries
 // below the sub-diagonal are zeroed once at the end.  `red` is LDS scratch
-// of at least 2 doubles.
-template <int NT, int TPR>
-__device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red)
+// of at least 2 doubles.  With KEEPV the reflector vectors stay below the
+// sub-diagonal (v_0 = 1 implied) and taus[i] (LDS) receives tau of step i,
+// as hessenberg<LPM, true>, for form_q_wg.
+template <int NT, int TPR, bool KEEPV = false>
+__device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red, zd *taus = nullptr)
 {
     const int tid = threadIdx.x;
     const int q = tid % TPR, row = tid / TPR;     // lane within the row team, row/column slot
@@ -354,6 +356,7 @@ __device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red)
             }
         }
         if (xnorm == 0.0 && alpha.im == 0.0) {   // tau = 0, H(i) = I; next tail norm
+            if (KEEPV && tid == 0) taus[i] = mk(0.0, 0.0);
             __syncthreads();                      // everyone has read red[] and alpha
             if (row == 0) {
                 double s = 0.0, mx = 0.0;
@@ -378,6 +381,7 @@ __device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red)
         const double beta = -copysign(w * sqrt(ar * ar + ai * ai + xn * xn), alpha.re);
         const zd tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
         const zd scal = zdiv(mk(1.0, 0.0), mk(alpha.re - beta, alpha.im));
+        if (KEEPV && tid == 0) taus[i] = tau;
         for (int r = i + 2 + tid; r < n; r += NT) A[lay(r, i)] = zmul(A[lay(r, i)], scal);
         __syncthreads();                 // v is scaled; red[] and alpha have been read
         const zd *vcol = A + lay(i + 1, i);   // v(c) = vcol[c], c >= 1, v(0) = 1
@@ -431,10 +435,11 @@ __device__ void hessenberg_wg(zd *A, const FullLayout lay, int n, double *red)
         if (tid == 0) A[lay(i + 1, i)] = mk(beta, 0.0);
         __syncthreads();
     }
-    // the reflector vectors below the sub-diagonal are not kept
-    for (int c = 0; c + 2 < n; ++c)
-        for (int r = c + 2 + tid; r < n; r += NT) A[lay(r, c)] = mk(0.0, 0.0);
-    __syncthreads();
+    if constexpr (!KEEPV) {   // the reflector vectors below the sub-diagonal are not kept
+        for (int c = 0; c + 2 < n; ++c)
+            for (int r = c + 2 + tid; r < n; r += NT) A[lay(r, c)] = mk(0.0, 0.0);
+        __syncthreads();
+    }
 }
 
 // ---- balancing (LAPACK ZGEBAL, the 3.11+ algorithm) ----

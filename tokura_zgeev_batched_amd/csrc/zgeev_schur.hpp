@@ -113,6 +113,73 @@ __device__ __forceinline__ void unpack_hessenberg(zd *H, const FullLayout lay, i
     }
 }
 
+// form_q_inplace by a workgroup of NT threads (teams of TPR threads share a
+// column; each column's dot product is split over the team's lanes and
+// combined with shuffles, as hessenberg_wg's left update), for the
+// reflectors and taus of hessenberg_wg<NT, TPR, true>.
+template <int NT, int TPR>
+__device__ __forceinline__ void form_q_wg(zd *A, const FullLayout lay, const zd *taus, int n)
+{
+    const int tid = (int)threadIdx.x, q = tid % TPR, team = tid / TPR;
+    for (int i = n - 2; i >= -1; --i) {
+        const int j = i + 1;
+        for (int r = j + tid; r < n; r += NT) {
+            A[lay(r, j)] = mk(r == j ? 1.0 : 0.0, 0.0);
+            if (r != j) A[lay(j, r)] = mk(0.0, 0.0);
+        }
+        __syncthreads();
+        if (i < 0) break;
+        const zd tau = taus[i];
+        if (tau.re == 0.0 && tau.im == 0.0) continue;
+        const int m = n - 1 - i;
+        const zd *v = A + lay(i + 1, i);   // v(r) = v[r], r >= 1, v(0) = 1
+        for (int c = i + 1 + team; c < n; c += NT / TPR) {
+            zd *col = A + lay(i + 1, c);
+            zd sv = q == 0 ? col[0] : mk(0.0, 0.0);
+            for (int r = (q == 0 ? TPR : q); r < m; r += TPR) sv = zadd(sv, zmulc(v[r], col[r]));
+#pragma unroll
+            for (int o = 1; o < TPR; o <<= 1) {
+                sv.re += __shfl_xor(sv.re, o, TPR);
+                sv.im += __shfl_xor(sv.im, o, TPR);
+            }
+            sv = zmul(tau, sv);
+            if (q == 0) col[0] = zsub(col[0], sv);
+            for (int r = (q == 0 ? TPR : q); r < m; r += TPR) col[r] = zsub(col[r], zmul(v[r], sv));
+        }
+        __syncthreads();
+    }
+}
+
+// First half of the Schur form for n > 32 by a workgroup of NT threads
+// (hessenberg_wg with kept reflectors, form_q_wg): the Hessenberg matrix
+// (zeros below the sub-diagonal) to Th and Q to Ug, n x n column-major.
+// Returns false for a non-finite input (nothing written).  H holds n x (n|1)
+// complex, taus n, red 8 doubles (LDS).
+template <int NT, int TPR>
+__device__ __forceinline__ bool schur_hq_wg(zd *H, zd *taus, double *red, const zd *__restrict__ src, int n, zd *Th,
+                                            zd *Ug)
+{
+    const int tid = (int)threadIdx.x;
+    const FullLayout lay{n | 1};
+    const int nn = n * n;
+    bool bad = false;
+    for (int e = tid; e < nn; e += NT) {
+        const zd v = src[e];
+        bad |= !zfinite(v);
+        H[lay(e % n, e / n)] = v;
+    }
+    if (__syncthreads_or(bad)) return false;
+    hessenberg_wg<NT, TPR, true>(H, lay, n, red, taus);
+    for (int e = tid; e < nn; e += NT) {
+        const int r = e % n, c = e / n;
+        Th[e] = r > c + 1 ? mk(0.0, 0.0) : H[lay(r, c)];
+    }
+    __syncthreads();
+    form_q_wg<NT, TPR>(H, lay, taus, n);
+    for (int e = tid; e < nn; e += NT) Ug[e] = H[lay(e % n, e / n)];
+    return true;
+}
+
 // Schur form of the matrix at src: T into H (upper triangular), U into Z.
 // Returns -1 for a non-finite input, else the QR INFO.
 // GZ: Z is an n x n column-major global buffer (ld n), so that the LDS holds
@@ -492,31 +559,18 @@ __device__ __forceinline__ void adjoint_one(zd *B1, zd *B2, zd *taus, zd *gv, in
     }
 }
 
-// dL/dA of one matrix (n <= 64) from its stored Schur form by a workgroup of
-// NT threads (NT/64 wavefronts): adjoint_one's quantities with the same
-// per-element arithmetic, so bitwise its results, spread over the
-// wavefronts.  X (wavefront 0) and Y (wavefront 1) are solved side by side;
-// then wavefront w forms rows a = w (mod NT/64) of W, rows of P = W U^H over
-// W's rows (a row of P needs only its own row of W, so no other wavefront
-// reads what is overwritten), and columns c = w (mod NT/64) of dA = U P.
-// Buffers (AdjGeom<64>): B1 = T/Y -> W -> P, B2 = X -> U, gv = the cotangent.
-// `f` is the matrix's forward flag (uniform over the workgroup).
+// adjoint_wg's phases after T (upper triangular, full layout) is in B1 and
+// the cotangent in gv: U is read once from Ug (global memory), dA written to
+// dst (which may be where U was parked: every read of Ug precedes a
+// workgroup barrier before the first write of dst).
 template <int NT>
-__device__ __forceinline__ void adjoint_wg(zd *B1, zd *B2, zd *gv, int n, const zd *__restrict__ schur_m,
-                                           const zd *__restrict__ gbar_m, zd *__restrict__ dst, int f)
+__device__ __forceinline__ void adjoint_wg_phases(zd *B1, zd *B2, const zd *gv, int n, const zd *Ug, zd *dst)
 {
     static_assert(NT % 64 == 0 && NT >= 128, "X and Y need one wavefront each");
     constexpr int NW = NT / 64;
     const int tid = (int)threadIdx.x, w = tid >> 6, t = tid & 63;
     const FullLayout lay{n | 1};
     const int nn = n * n;
-    if (f != 0) {
-        fill_nan<NT>(dst, nn, tid);
-        return;
-    }
-    for (int e = tid; e < nn; e += NT) B1[lay(e % n, e / n)] = schur_m[e];
-    for (int i = tid; i < n; i += NT) gv[i] = gbar_m[i];
-    __syncthreads();
     if (w == 0) trevc_right_upper<64, 1>(B1, B2, lay, n, t);
     else if (w == 1) trevc_left_lower<64, 1>(B1, lay, n, t);
     __syncthreads();
@@ -535,7 +589,6 @@ __device__ __forceinline__ void adjoint_wg(zd *B1, zd *B2, zd *gv, int n, const 
         if (b <= a) B1[lay(a, b)] = acc;
     }
     __syncthreads();
-    const zd *Ug = schur_m + nn;
     for (int e = tid; e < nn; e += NT) B2[lay(e % n, e / n)] = Ug[e];
     __syncthreads();
     // P = W U^H, row a over W's row a: lane c
@@ -557,6 +610,82 @@ __device__ __forceinline__ void adjoint_wg(zd *B1, zd *B2, zd *gv, int n, const 
             dst[r + c * n] = acc;
         }
     }
+}
+
+// dL/dA of one matrix (n <= 64) from its stored Schur form by a workgroup of
+// NT threads (NT/64 wavefronts): adjoint_one's quantities with the same
+// per-element arithmetic, so bitwise its results, spread over the
+// wavefronts.  X (wavefront 0) and Y (wavefront 1) are solved side by side;
+// then wavefront w forms rows a = w (mod NT/64) of W, rows of P = W U^H over
+// W's rows (a row of P needs only its own row of W, so no other wavefront
+// reads what is overwritten), and columns c = w (mod NT/64) of dA = U P.
+// Buffers (AdjGeom<NB>): B1 = T/Y -> W -> P, B2 = X -> U, gv = the cotangent.
+// `f` is the matrix's forward flag (uniform over the workgroup).
+template <int NT>
+__device__ __forceinline__ void adjoint_wg(zd *B1, zd *B2, zd *gv, int n, const zd *__restrict__ schur_m,
+                                           const zd *__restrict__ gbar_m, zd *__restrict__ dst, int f)
+{
+    const int tid = (int)threadIdx.x;
+    const FullLayout lay{n | 1};
+    const int nn = n * n;
+    if (f != 0) {
+        fill_nan<NT>(dst, nn, tid);
+        return;
+    }
+    for (int e = tid; e < nn; e += NT) B1[lay(e % n, e / n)] = schur_m[e];
+    for (int i = tid; i < n; i += NT) gv[i] = gbar_m[i];
+    __syncthreads();
+    adjoint_wg_phases<NT>(B1, B2, gv, n, schur_m + nn, dst);
+}
+
+// The Schur form of one matrix for n > 32 in LDS by a workgroup of NT
+// threads, with the arithmetic of the two-launch forward (schur_hq_wg's
+// Hessenberg and Q, the two-bulge QR by the first wavefront): T into H (upper
+// triangular, full layout), U into Z, an n x n column-major buffer with
+// leading dimension zld in LDS or in global memory (the Hessenberg matrix is
+// parked there while Q is formed in H).  taus: n complex, red: 8 doubles,
+// ishared: one int, all LDS.  Returns -1 for a non-finite input, else the QR
+// INFO (uniform over the workgroup).
+template <int NT>
+__device__ __forceinline__ int schur_wg(zd *H, zd *Z, int zld, zd *taus, double *red, int *ishared,
+                                        const zd *__restrict__ src, int n, int itmax)
+{
+    const int tid = (int)threadIdx.x, w = tid >> 6, t = tid & 63;
+    const FullLayout lay{n | 1};
+    const int nn = n * n;
+    bool bad = false;
+    for (int e = tid; e < nn; e += NT) {
+        const zd v = src[e];
+        bad |= !zfinite(v);
+        H[lay(e % n, e / n)] = v;
+    }
+    if (__syncthreads_or(bad)) return -1;
+    hessenberg_wg<NT, NT / 64, true>(H, lay, n, red, taus);
+    // park the Hessenberg matrix in Z's home, form Q in place, swap (each
+    // element by the thread that parked it)
+    for (int e = tid; e < nn; e += NT) {
+        const int r = e % n, c = e / n;
+        Z[r + c * zld] = r > c + 1 ? mk(0.0, 0.0) : H[lay(r, c)];
+    }
+    __syncthreads();
+    form_q_wg<NT, NT / 64>(H, lay, taus, n);
+    for (int e = tid; e < nn; e += NT) {
+        const int r = e % n, c = e / n;
+        const zd h = Z[r + c * zld];
+        Z[r + c * zld] = H[lay(r, c)];
+        H[lay(r, c)] = h;
+    }
+    __syncthreads();
+    if (w == 0) {
+        pack_hessenberg<64>(H, lay, n, t);
+        const int info = zlahqr_dual<64, true>(H, PackedLayout{}, n, t, itmax, Z, zld);
+        unpack_hessenberg<64>(H, lay, n, t);
+        for (int c = 0; c < n; ++c)
+            for (int r = c + 1 + t; r < n; r += 64) H[lay(r, c)] = mk(0.0, 0.0);
+        if (t == 0) *ishared = info;
+    }
+    __syncthreads();
+    return *ishared;
 }
 
 }  // namespace tkz
