@@ -46,7 +46,7 @@ def test_schur_forward(n):
 
 def test_recompute_forward_order_matches_schur():
     rng = np.random.default_rng(5)
-    for n in (3, 16, 24, 32, 48, 64):
+    for n in (3, 16, 17, 24, 32, 33, 40, 48, 57, 64):
         M = torch.tensor(rand(rng, 30, n), device="cuda")
         e1, f1 = eigvals_autograd(M, mode="store")
         e2, f2 = eigvals_autograd(M, mode="recompute")

@@ -61,7 +61,7 @@ def test_vectors_vs_oracle_and_lapack(gpu, n):
         assert np.abs(VL[b][:, pl] - VLl[b]).max() <= 1e-8
 
 
-@pytest.mark.parametrize("n", [5, 32, 64])
+@pytest.mark.parametrize("n", [5, 20, 32, 40, 57, 64])
 def test_vectors_one_side_and_schur_order(gpu, n):
     rng = np.random.default_rng(800 + n)
     M = torch.tensor(rand(rng, 16, n), device=gpu)
