@@ -40,6 +40,12 @@ PEAK_HBM = 8.0e12
 # two-bulge QR kernel (each counter scaled by its reading for a 1 GiB device
 # copy: FETCH x2.0, WRITE x1.0).
 QR64_TRAFFIC_BYTES_PER_MATRIX = 35_480.0
+# FP64 lane operations executed per matrix by the two-bulge QR k_qr_packed<64,64>
+# (rocprofv3 SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 over one 20,000-matrix call,
+# 64 lanes per wave instruction, an FMA counted as 2): 64 * (2 * 311,000 +
+# 140,614 + 69,864 + 13,011) per matrix.  It counts the serial reflector chain
+# that every lane computes redundantly, so it is ~7.7x the 80/3 n^3 model.
+QR64_EXECUTED_FP64_PER_MATRIX = 5.411e7
 
 
 def flops_per_solve(n: int) -> float:
@@ -482,7 +488,12 @@ def main():
                      "compute_unit": "fp64 VALU; MI355X fp64 vector and matrix peaks coincide (78.6 TFLOP/s)",
                      "kernel_avg_ms": avg_qr_s * 1e3, "hessenberg_avg_ms": avg_hess_s * 1e3,
                      "path_achieved_tflops": path_achieved / 1e12,
-                     "path_frac": path_achieved / PEAK_FP64},
+                     "path_frac": path_achieved / PEAK_FP64,
+                     # issued FP64 lane operations (rocprof counters per matrix, live kernel time)
+                     "executed_fp64_tflops": (QR64_EXECUTED_FP64_PER_MATRIX * B / avg_qr_s / 1e12
+                                              if n == 64 and variant == 20 else None),
+                     "executed_fp64_frac": (QR64_EXECUTED_FP64_PER_MATRIX * B / avg_qr_s / PEAK_FP64
+                                            if n == 64 and variant == 20 else None)},
         "flags_nonzero": nflag,
     }
 
