@@ -278,27 +278,34 @@ static int split_nb(int n)
 
 // Tslot receives the Hessenberg matrix, then T; Uslot receives Q, then U.
 // With VEC, k_vectors_from_schur follows (eigenvectors into VLv / VRv).
+template <int NB, int LPM>
+static hipError_t launch_schur_split_nb(int n, int batch, const zd *A, zd *eig, zd *Tout, long long tstride,
+                                        zd *Uout, long long ustride, int8_t *flag, int itmax, hipStream_t s, bool vec,
+                                        zd *VLv, zd *VRv)
+{
+    const dim3 grid((batch + 64 / LPM - 1) / (64 / LPM));
+    if constexpr (NB > 32)
+        hipLaunchKernelGGL((k_schur_hq_wg<NB, TKZ_ADJ_WG_NT>), dim3(batch), dim3(TKZ_ADJ_WG_NT), 0, s, n, batch, A, eig,
+                           Tout, tstride, Uout, ustride, flag);
+    else
+        hipLaunchKernelGGL((k_schur_hq<NB, LPM>), grid, dim3(64), 0, s, n, batch, A, eig, Tout, tstride, Uout, ustride,
+                           flag);
+    hipLaunchKernelGGL((k_schur_qr<NB, LPM>), grid, dim3(64), 0, s, n, batch, eig, Tout, tstride, Uout, ustride, flag,
+                       itmax);
+    if (vec)
+        hipLaunchKernelGGL((k_vectors_from_schur<NB, LPM>), grid, dim3(64), 0, s, n, batch, Tout, VLv, VRv, flag);
+    return hipGetLastError();
+}
+
 static hipError_t launch_schur_split(int n, int batch, const zd *A, zd *eig, zd *Tout, long long tstride,
                                      zd *Uout, long long ustride, int8_t *flag, int itmax, hipStream_t s,
                                      bool vec = false, zd *VLv = nullptr, zd *VRv = nullptr)
 {
     switch (split_nb(n)) {
-#define X(nb, lpm)                                                                                              \
-    case nb: {                                                                                                  \
-        const dim3 grid((batch + 64 / lpm - 1) / (64 / lpm));                                                   \
-        if (nb > 32)                                                                                            \
-            hipLaunchKernelGGL((k_schur_hq_wg<nb, TKZ_ADJ_WG_NT>), dim3(batch), dim3(TKZ_ADJ_WG_NT), 0, s, n, batch, \
-                               A, eig, Tout, tstride, Uout, ustride, flag);                                     \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_schur_hq<nb, lpm>), grid, dim3(64), 0, s, n, batch, A, eig, Tout, tstride, Uout, \
-                               ustride, flag);                                                                  \
-        hipLaunchKernelGGL((k_schur_qr<nb, lpm>), grid, dim3(64), 0, s, n, batch, eig, Tout, tstride, Uout,      \
-                           ustride, flag, itmax);                                                               \
-        if (vec)                                                                                                \
-            hipLaunchKernelGGL((k_vectors_from_schur<nb, lpm>), grid, dim3(64), 0, s, n, batch, Tout, VLv, VRv,  \
-                               flag);                                                                           \
-        return hipGetLastError();                                                                               \
-    }
+#define X(nb, lpm)                                                                                             \
+    case nb:                                                                                                   \
+        return launch_schur_split_nb<nb, lpm>(n, batch, A, eig, Tout, tstride, Uout, ustride, flag, itmax, s, vec, \
+                                              VLv, VRv);
         TKZ_SPLIT_NB(X)
 #undef X
     }

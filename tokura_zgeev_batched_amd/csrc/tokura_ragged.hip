@@ -442,33 +442,34 @@ __global__ __launch_bounds__(NT) void k_ragged_adjoint_wg(char *work, const zd *
 
 
 
+// the two launches of the Schur forward for size class CLS (LDS for NB)
+template <int NB, int LPM, int CLS>
+static void launch_ragged_schur_cls(const zd *A, char *work, zd *eig, zd *schur, int8_t *flag, int itmax, int cus,
+                                    hipStream_t s)
+{
+    if constexpr (NB > 32) {
+        static const int oh = resident_blocks(k_ragged_schur_hq_wg<NB, CLS>, TKZ_ADJ_WG_NT);
+        hipLaunchKernelGGL((k_ragged_schur_hq_wg<NB, CLS>), dim3(cus * oh * 4), dim3(TKZ_ADJ_WG_NT), 0, s, work, A, eig,
+                           schur, flag);
+    } else {
+        static const int oh = resident_blocks(k_ragged_schur_hq<NB, LPM, CLS>);
+        hipLaunchKernelGGL((k_ragged_schur_hq<NB, LPM, CLS>), dim3(cus * oh), dim3(64), 0, s, work, A, eig, schur, flag);
+    }
+    static const int oq = resident_blocks(k_ragged_schur_qr<NB, LPM, CLS>);
+    hipLaunchKernelGGL((k_ragged_schur_qr<NB, LPM, CLS>), dim3(cus * oq), dim3(64), 0, s, work, eig, schur, flag, itmax);
+}
+
 static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schur, int8_t *flag, int itmax, int cus,
                                       hipStream_t s)
 {
     hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(1), 0, s, work);
-#define TKZ_RS(nb, lpm, cls)                                                                                  \
-    {                                                                                                          \
-        if constexpr (nb > 32) {                                                                               \
-            static const int oh = resident_blocks(k_ragged_schur_hq_wg<nb, cls>, TKZ_ADJ_WG_NT);               \
-            hipLaunchKernelGGL((k_ragged_schur_hq_wg<nb, cls>), dim3(cus * oh * 4), dim3(TKZ_ADJ_WG_NT), 0, s,   \
-                               work, A, eig, schur, flag);                                                     \
-        } else {                                                                                               \
-            static const int oh = resident_blocks(k_ragged_schur_hq<nb, lpm, cls>);                            \
-            hipLaunchKernelGGL((k_ragged_schur_hq<nb, lpm, cls>), dim3(cus * oh), dim3(64), 0, s, work, A, eig, \
-                               schur, flag);                                                                   \
-        }                                                                                                      \
-        static const int oq = resident_blocks(k_ragged_schur_qr<nb, lpm, cls>);                                \
-        hipLaunchKernelGGL((k_ragged_schur_qr<nb, lpm, cls>), dim3(cus * oq), dim3(64), 0, s, work, eig, schur, \
-                           flag, itmax);                                                                       \
-    }
-    TKZ_RS(64, 64, 0)
-    TKZ_RS(60, 64, 1)
-    TKZ_RS(56, 64, 2)
-    TKZ_RS(48, 64, 3)
-    TKZ_RS(40, 64, 4)
-    TKZ_RS(32, 32, kClsB)
-    TKZ_RS(16, 16, kClsC)
-#undef TKZ_RS
+    launch_ragged_schur_cls<64, 64, 0>(A, work, eig, schur, flag, itmax, cus, s);
+    launch_ragged_schur_cls<60, 64, 1>(A, work, eig, schur, flag, itmax, cus, s);
+    launch_ragged_schur_cls<56, 64, 2>(A, work, eig, schur, flag, itmax, cus, s);
+    launch_ragged_schur_cls<48, 64, 3>(A, work, eig, schur, flag, itmax, cus, s);
+    launch_ragged_schur_cls<40, 64, 4>(A, work, eig, schur, flag, itmax, cus, s);
+    launch_ragged_schur_cls<32, 32, kClsB>(A, work, eig, schur, flag, itmax, cus, s);
+    launch_ragged_schur_cls<16, 16, kClsC>(A, work, eig, schur, flag, itmax, cus, s);
     return hipGetLastError();
 }
 

@@ -218,7 +218,7 @@ static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *f
     constexpr int GPW = 64 / LPM;
     const int blocks = (batch + GPW - 1) / GPW;
     *stage = 1;
-    if (NB >= 48 && !ONEWAVE)   // large matrices: one workgroup per matrix
+    if constexpr (NB >= 48 && !ONEWAVE)   // large matrices: one workgroup per matrix
         hipLaunchKernelGGL((k_hessenberg_packed_wg<NB, TKZ_HESS_NT>), dim3(batch), dim3(TKZ_HESS_NT), 0, s, n, batch,
                            A, work, flag, bal);
     else
@@ -232,6 +232,17 @@ static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *f
     return hipGetLastError();
 }
 
+template <int KIND, int NB, int LPM>
+static hipError_t launch_kind(int n, int batch, const zd *A, zd *eig, int8_t *flag, zd *work, int itmax, int bal,
+                              hipStream_t s, int *stage, hipEvent_t mid)
+{
+    if constexpr (KIND == VK_FUSED)
+        return launch_fused<NB, LPM>(n, batch, A, eig, flag, itmax, bal, s);
+    else
+        return launch_split<NB, LPM, KIND == VK_DUAL || KIND == VK_DUAL_1W, KIND == VK_DUAL_1W>(
+            n, batch, A, eig, flag, work, itmax, bal, s, stage, mid);
+}
+
 hipError_t launch_plain(int variant, int n, int batch, const void *A, void *eig, char *flag, void *work,
                         int itmax, int bal, hipStream_t s, int *stage, hipEvent_t mid)
 {
@@ -241,11 +252,8 @@ hipError_t launch_plain(int variant, int n, int batch, const void *A, void *eig,
     zd *wk = (zd *)work;
     *stage = 1;
     switch (variant) {
-#define X(id, kind, nb, lpm)                                                              \
-    case id:                                                                              \
-        if constexpr (VK_##kind == VK_FUSED) return launch_fused<nb, lpm>(n, batch, a, w, f, itmax, bal, s); \
-        else return launch_split<nb, lpm, VK_##kind == VK_DUAL || VK_##kind == VK_DUAL_1W, VK_##kind == VK_DUAL_1W>( \
-            n, batch, a, w, f, wk, itmax, bal, s, stage, mid);
+#define X(id, kind, nb, lpm) \
+    case id: return launch_kind<VK_##kind, nb, lpm>(n, batch, a, w, f, wk, itmax, bal, s, stage, mid);
         TKZ_VARIANTS(X)
 #undef X
     default:
