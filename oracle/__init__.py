@@ -8,6 +8,9 @@ Contents
 --------
 * ``method_zgeev``  — the C restatement of the GPU method (zgehd2 + zlahqr,
   eigenvalues only, no balancing), ``zgeev_method.c``.
+* ``method_zgeev_dual`` — the same with the two-bulge QR of the GPU's n > 32
+  eigenvalue path (two ZLAHQR sweeps per iteration with the shifts of one
+  trailing 2x2), ``oracle_zlahqr_dual``.
 * ``lapack_zgeev``  — host LAPACK zgeev (balanced, LAPACK default), the
   reference harness's oracle (stream_test_with_MKL/get_eig_MKL.c:26-51).
 * ``lapack_hrd_hseqr`` — LAPACK zgehrd + zhseqr('E','N') without balancing.
@@ -74,6 +77,8 @@ def lib():
     I = ctypes.c_int
     L.oracle_zgeev_batched.argtypes = [I, I, P, P, P, P, I]
     L.oracle_zgeev_batched.restype = None
+    L.oracle_zgeev_batched_dual.argtypes = [I, I, P, P, P, P, I]
+    L.oracle_zgeev_batched_dual.restype = None
     L.oracle_zschur.argtypes = [I, P, P, P, P]
     L.oracle_zschur.restype = I
     L.oracle_zgeev_adjoint.argtypes = [I, P, P, P, P]
@@ -211,6 +216,20 @@ def dlanv2(a, b, c, d, use_lapack: bool = False):
     v = [ctypes.c_double(x) for x in (a, b, c, d)] + [ctypes.c_double() for _ in range(6)]
     lib().oracle_dlanv2(*[ctypes.byref(x) for x in v])
     return tuple(x.value for x in v)
+
+
+def method_zgeev_dual(M, itmax: int = -1, with_sweeps: bool = False):
+    """Eigenvalues by the restated two-bulge QR of the GPU eigenvalue path
+    (oracle_zlahqr_dual after oracle_zgehd2): (eig[b, i], flag[b])."""
+    A = _colmajor(M)
+    b, n, _ = A.shape
+    eig = np.zeros((b, n), np.complex128)
+    flag = np.zeros(b, np.int8)
+    sweeps = np.zeros(b, np.int32)
+    lib().oracle_zgeev_batched_dual(n, b, _p(A), _p(eig), _p(flag), _p(sweeps), itmax)
+    if with_sweeps:
+        return eig, flag, sweeps
+    return eig, flag
 
 
 def method_zgeev(M, itmax: int = -1, with_sweeps: bool = False, balance: str = "N"):

@@ -18,6 +18,10 @@
  *                      30*max(10,n) sweeps); reference hseqr kernel
  *                      tokura_zhseqr_MWB_kernel.cuh:611-791 (whose real-part-only
  *                      deflation, top-left shift and n^4 cap are not reproduced)
+ *   oracle_zlahqr_dual  the two-bulge QR of the GPU's eigenvalue path: two
+ *                      ZLAHQR sweeps per iteration with the shifts of one
+ *                      trailing 2x2 (no reference counterpart: the reference
+ *                      chases one bulge per sweep)
  *   oracle_ztrevc   <- LAPACK ZTREVC   (right/left eigenvectors of the Schur form
  *                      by back-substitution), used by the adjoint checks
  *   oracle_zgeev_vectors  eigenvectors (Schur form, ZTREVC, back-transform,
@@ -338,6 +342,165 @@ int oracle_zlahqr(int wantt, int wantz, int n, zc *h, int ldh, zc *w, zc *z, int
 #undef W
 }
 
+/* ZLAHQR's shift for the active block [l, i] (1-based, as oracle_zlahqr):
+ * exceptional shifts when kdefl is a multiple of 20 (from the bottom) or of
+ * 10 (from the top), else the Wilkinson shift. */
+static zc zlahqr_shift(const zc *h, int ldh, int l, int i, int kdefl)
+{
+#define H(i, j) h[IDX((i) - 1, (j) - 1, ldh)]
+    const double dat1 = 0.75;
+    const int kexsh = 10;
+    zc t;
+    if (kdefl % (2 * kexsh) == 0) {
+        double s = dat1 * fabs(creal(H(i, i - 1)));
+        t = s + H(i, i);
+    } else if (kdefl % kexsh == 0) {
+        double s = dat1 * fabs(creal(H(l + 1, l)));
+        t = s + H(l, l);
+    } else {
+        t = H(i, i);
+        zc u = csqrt(H(i - 1, i)) * csqrt(H(i, i - 1));
+        double s = cabs1(u);
+        if (s != 0.0) {
+            zc x = 0.5 * (H(i - 1, i - 1) - t);
+            double sx = cabs1(x);
+            s = fmax(s, cabs1(x));
+            zc xs = x / s, us = u / s;
+            zc y = s * csqrt(xs * xs + us * us);
+            if (sx > 0.0) {
+                zc xn = x / sx;
+                if (creal(xn) * creal(y) + cimag(xn) * cimag(y) < 0.0) y = -y;
+            }
+            t = t - u * (u / (x + y));
+        }
+    }
+    return t;
+#undef H
+}
+
+/* One single-shift ZLAHQR sweep over the active block [l, i] (1-based) from
+ * row l with shift sig, eigenvalues only (transformations restricted to the
+ * block), ending with ZLAHQR's "make H(i, i-1) real" scaling of column i. */
+static void zlahqr_sweep(zc *h, int ldh, int l, int i, zc sig)
+{
+#define H(i, j) h[IDX((i) - 1, (j) - 1, ldh)]
+    zc v[2];
+    {
+        zc h11s = H(l, l) - sig;
+        double h21 = creal(H(l + 1, l));
+        double s = cabs1(h11s) + fabs(h21);
+        v[0] = h11s / s;
+        v[1] = h21 / s;
+    }
+    for (int k = l; k <= i - 1; ++k) {
+        if (k > l) { v[0] = H(k, k - 1); v[1] = H(k + 1, k - 1); }
+        zc t1;
+        oracle_zlarfg(2, &v[0], &v[1], &t1);
+        if (k > l) { H(k, k - 1) = v[0]; H(k + 1, k - 1) = 0; }
+        zc v2 = v[1];
+        double t2 = creal(t1 * v2);
+        for (int j = k; j <= i; ++j) {
+            zc sum = conj(t1) * H(k, j) + t2 * H(k + 1, j);
+            H(k, j) = H(k, j) - sum;
+            H(k + 1, j) = H(k + 1, j) - sum * v2;
+        }
+        int top = (k + 2 < i ? k + 2 : i);
+        for (int j = l; j <= top; ++j) {
+            zc sum = t1 * H(j, k) + t2 * H(j, k + 1);
+            H(j, k) = H(j, k) - sum;
+            H(j, k + 1) = H(j, k + 1) - sum * conj(v2);
+        }
+    }
+    zc temp = H(i, i - 1);
+    if (cimag(temp) != 0.0) {
+        double rtemp = cabs(temp);
+        H(i, i - 1) = rtemp;
+        temp = temp / rtemp;
+        for (int r = l; r <= i - 1; ++r) H(r, i) *= temp;
+    }
+#undef H
+}
+
+/* The two-bulge QR of the GPU's eigenvalue path, eigenvalues only (ILO = 1,
+ * IHI = n): per iteration, after ZLAHQR's deflation search over [l, i], two
+ * single-shift sweeps from row l with the shifts s1 (ZLAHQR's, exceptional
+ * ones included) and s2 (the other eigenvalue of the trailing 2x2,
+ * H(i-1,i-1) + H(i,i) - s1, or the Wilkinson shift when s1 is exceptional),
+ * both taken before the first sweep; an active block of fewer than 10 rows
+ * goes to oracle_zlahqr on that block, which is independent once
+ * H(l, l-1) = 0.  The GPU pipelines the second sweep three positions behind
+ * the first; the products it reorders commute, so the eigenvalues agree up to
+ * rounding.  Returns 0 or the 1-based index of the first unconverged
+ * eigenvalue; `sweeps` (optional) counts the single-shift sweeps. */
+int oracle_zlahqr_dual(int n, zc *h, int ldh, zc *w, int itmax_override, int *sweeps)
+{
+#define H(i, j) h[IDX((i) - 1, (j) - 1, ldh)]
+#define W(i) w[(i) - 1]
+    const int dual_min = 10;
+    int total = 0;
+    if (sweeps) *sweeps = 0;
+    if (n == 0) return 0;
+    if (n == 1) { W(1) = H(1, 1); return 0; }
+    const double smlnum = SAFMIN * ((double)n / ULP);
+    const int itmax = itmax_override >= 0 ? itmax_override : 30 * (n > 10 ? n : 10);
+    int kdefl = 0;
+    int i = n;
+    while (i >= 2) {
+        int l = 1, converged = 0;
+        for (int its = 0; its <= itmax; ++its) {
+            int k;
+            for (k = i; k >= l + 1; --k) {
+                if (cabs1(H(k, k - 1)) <= smlnum) break;
+                double tst = cabs1(H(k - 1, k - 1)) + cabs1(H(k, k));
+                if (tst == 0.0) {
+                    if (k - 2 >= 1) tst += fabs(creal(H(k - 1, k - 2)));
+                    if (k + 1 <= n) tst += fabs(creal(H(k + 1, k)));
+                }
+                if (fabs(creal(H(k, k - 1))) <= ULP * tst) {
+                    double ab = fmax(cabs1(H(k, k - 1)), cabs1(H(k - 1, k)));
+                    double ba = fmin(cabs1(H(k, k - 1)), cabs1(H(k - 1, k)));
+                    double aa = fmax(cabs1(H(k, k)), cabs1(H(k - 1, k - 1) - H(k, k)));
+                    double bb = fmin(cabs1(H(k, k)), cabs1(H(k - 1, k - 1) - H(k, k)));
+                    double s = aa + ab;
+                    if (ba * (ab / s) <= fmax(smlnum, ULP * (bb * (aa / s)))) break;
+                }
+            }
+            l = k;
+            if (l > 1) H(l, l - 1) = 0;
+            if (l >= i) { converged = 1; break; }
+            if (i - l + 1 < dual_min) {
+                int sw = 0;
+                const int info = oracle_zlahqr(0, 0, i - l + 1, &H(l, l), ldh, &W(l), NULL, 1, itmax_override, &sw);
+                total += sw;
+                if (info != 0) {
+                    if (sweeps) *sweeps = total;
+                    return l - 1 + info;
+                }
+                converged = 2;   /* eigenvalues l..i stored by oracle_zlahqr */
+                break;
+            }
+            ++kdefl;
+            const zc s1 = zlahqr_shift(h, ldh, l, i, kdefl);
+            const zc s2 = (kdefl % 10 == 0) ? zlahqr_shift(h, ldh, l, i, 1) : (H(i - 1, i - 1) + H(i, i)) - s1;
+            zlahqr_sweep(h, ldh, l, i, s1);
+            zlahqr_sweep(h, ldh, l, i, s2);
+            total += 2;
+        }
+        if (!converged) {
+            if (sweeps) *sweeps = total;
+            return i;
+        }
+        if (converged == 1) W(i) = H(i, i);
+        kdefl = 0;
+        i = l - 1;
+    }
+    if (i == 1) W(1) = H(1, 1);
+    if (sweeps) *sweeps = total;
+    return 0;
+#undef H
+#undef W
+}
+
 static int all_finite(int n, const zc *a)
 {
     for (size_t k = 0; k < (size_t)n * n; ++k)
@@ -368,6 +531,32 @@ void oracle_zgeev_batched(int n, int batch, const zc *A, zc *eig, signed char *f
                 /* unconverged eigenvalues: report the current diagonal */
                 for (int i = 0; i < info; ++i) w[i] = h[IDX(i, i, n)];
             }
+            flag[b] = info != 0 ? 1 : 0;
+        }
+        if (sweeps) sweeps[b] = sw;
+        free(h);
+    }
+}
+
+/* oracle_zgeev_batched with the two-bulge QR (oracle_zlahqr_dual). */
+void oracle_zgeev_batched_dual(int n, int batch, const zc *A, zc *eig, signed char *flag, int *sweeps,
+                               int itmax_override)
+{
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (int b = 0; b < batch; ++b) {
+        zc *h = (zc *)malloc(sizeof(zc) * (size_t)n * n);
+        memcpy(h, A + (size_t)b * n * n, sizeof(zc) * (size_t)n * n);
+        zc *w = eig + (size_t)b * n;
+        int sw = 0;
+        if (!all_finite(n, h)) {
+            for (int i = 0; i < n; ++i) w[i] = CMPLX(NAN, NAN);
+            flag[b] = 2;
+        } else {
+            oracle_zgehd2(n, h, n, NULL, NULL);
+            for (int i = 0; i < n; ++i) w[i] = h[IDX(i, i, n)];
+            int info = oracle_zlahqr_dual(n, h, n, w, itmax_override, &sw);
+            if (info != 0)
+                for (int i = 0; i < info; ++i) w[i] = h[IDX(i, i, n)];
             flag[b] = info != 0 ? 1 : 0;
         }
         if (sweeps) sweeps[b] = sw;

@@ -1,6 +1,8 @@
-"""GPU parity of the two-bulge QR variants (ids 19: NB 48, 20: NB 64, one
-matrix per wavefront; id 21: NB 32, two matrices per wavefront with a
-16-lane chain per bulge; two shifts per sweep pipelined in one sweep).
+"""GPU parity of the two-bulge QR variants (ids 19: NB 48, 20: NB 64, 22:
+NB 64 after the one-wavefront Hessenberg, 23: NB 56, 24: NB 40, 25: NB 60,
+one matrix per wavefront; id 21: NB 32, two matrices per wavefront with a
+16-lane chain per bulge; two shifts per sweep pipelined in one sweep), also
+against the oracle's restatement of the same sweeps (<= 1e-11).
 
 Bars as the plain path: max relative error <= 1e-10 against host LAPACK
 zgeev after optimal matching; condition-scaled error <= 100 on hard spectra
@@ -19,6 +21,7 @@ from tokura_zgeev_batched_amd.workloads import hard_batch
 pytestmark = pytest.mark.gpu
 
 TOL_LAPACK = 1e-10
+TOL_METHOD = 1e-11   # against the oracle's restatement of the same two-bulge QR
 DUAL = {48: 19, 64: 20}
 DUAL32 = 21
 DUAL_1W = 22   # NB 64, one-wavefront Hessenberg
@@ -208,3 +211,22 @@ def test_dual32_hard_spectra_and_flags():
     e, f = eigs(rand(rng, 3, 30), h)
     assert (f == 1).all()
     h.close()
+
+
+@pytest.mark.parametrize("variant,n", [(24, 34), (24, 40), (19, 41), (19, 48), (23, 49), (23, 56), (25, 57),
+                                       (25, 60), (20, 61), (20, 64), (22, 64)])
+def test_dual_vs_oracle_restatement(variant, n):
+    """Every two-bulge variant against the oracle's restatement of the same
+    sweeps (oracle_zlahqr_dual: two ZLAHQR sweeps per iteration, the shifts of
+    one trailing 2x2), at a bar ten times tighter than the LAPACK one."""
+    rng = np.random.default_rng(4300 + n)
+    M = rand(rng, 96, n)
+    h = dual_handle(n, variant)
+    e, f = eigs(M, h)
+    h.close()
+    assert not f.any()
+    wd, fd = oracle.method_zgeev_dual(M)
+    assert not fd.any()
+    err = oracle.max_relerr(wd, e)
+    print(f"variant {variant} n={n}: max rel diff vs the restated two-bulge QR {err:.3g}", flush=True)
+    assert err <= TOL_METHOD

@@ -361,3 +361,46 @@ def test_balanced_method_on_graded_matrices():
     a, _ = oracle.method_zgeev(R)
     b, _ = oracle.method_zgeev(R, balance="B")
     assert oracle.max_relerr(a, b) < 1e-12
+
+
+@pytest.mark.parametrize("n", [10, 11, 17, 33, 48, 64])
+def test_dual_restatement_vs_lapack_and_single_shift(n):
+    """The restated two-bulge QR: the same eigenvalues as LAPACK zgeev and the
+    single-shift restatement (<= 1e-10), and about 23% more single-shift
+    sweeps than ZLAHQR's one shift per sweep at n = 64 (both shifts of a
+    sweep pair come from one trailing 2x2)."""
+    rng = np.random.default_rng(1200 + n)
+    M = rng.standard_normal((40, n, n)) + 1j * rng.standard_normal((40, n, n))
+    wd, fd, sd = oracle.method_zgeev_dual(M, with_sweeps=True)
+    ws, fs, ss = oracle.method_zgeev(M, with_sweeps=True)
+    wl, _ = oracle.lapack_zgeev(M)
+    assert not fd.any() and not fs.any()
+    assert oracle.max_relerr(wl, wd) <= 1e-10
+    assert oracle.max_relerr(ws, wd) <= 1e-10
+    if n == 64:
+        assert 1.1 < sd.mean() / ss.mean() < 1.4
+
+
+def test_dual_restatement_closed_forms():
+    # companion of z^n - 1 (needs the exceptional shifts), triangular input
+    # (no sweep), a block-diagonal matrix (independent blocks after deflation)
+    for n in (12, 40, 64):
+        C = np.zeros((n, n), complex)
+        C[1:, :-1] = np.eye(n - 1)
+        C[0, -1] = 1.0
+        w, f = oracle.method_zgeev_dual(C[None])
+        assert not f.any()
+        assert oracle.max_relerr(np.exp(2j * np.pi * np.arange(n) / n)[None], w) <= 1e-11
+    rng = np.random.default_rng(77)
+    T = np.triu(rng.standard_normal((30, 30)) + 1j * rng.standard_normal((30, 30)))
+    w, f, sw = oracle.method_zgeev_dual(T[None], with_sweeps=True)
+    assert sw[0] == 0
+    np.testing.assert_array_equal(np.sort_complex(w[0]), np.sort_complex(np.diag(T)))
+    B = np.zeros((40, 40), complex)
+    B[:25, :25] = rng.standard_normal((25, 25)) + 1j * rng.standard_normal((25, 25))
+    B[25:, 25:] = rng.standard_normal((15, 15)) + 1j * rng.standard_normal((15, 15))
+    w, f = oracle.method_zgeev_dual(B[None])
+    ref = np.concatenate([np.linalg.eigvals(B[:25, :25]), np.linalg.eigvals(B[25:, 25:])])
+    assert oracle.max_relerr(ref[None], w) <= 1e-10
+    w, f = oracle.method_zgeev_dual(rng.standard_normal((2, 30, 30))[None][0].astype(complex), itmax=0)
+    assert (f == 1).all()
