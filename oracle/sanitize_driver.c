@@ -15,6 +15,8 @@
 typedef double _Complex zc;
 
 void oracle_zgeev_batched(int n, int batch, const zc *A, zc *eig, signed char *flag, int *sweeps, int itmax_override);
+void oracle_zgeev_batched_dual(int n, int batch, const zc *A, zc *eig, signed char *flag, int *sweeps,
+                               int itmax_override);
 void oracle_zgeev_batched_bal(int n, int batch, const zc *A, zc *eig, signed char *flag, int itmax_override,
                               char job);
 int oracle_zschur(int n, const zc *A, zc *T, zc *U, zc *w);
@@ -32,7 +34,7 @@ static double gauss(void)
 int main(void)
 {
     srand(7);
-    const int sizes[] = {1, 2, 3, 5, 17, 64};
+    const int sizes[] = {1, 2, 3, 5, 10, 17, 64};
     int checked = 0;
     for (unsigned k = 0; k < sizeof sizes / sizeof sizes[0]; ++k) {
         const int n = sizes[k], batch = 3;
@@ -50,6 +52,8 @@ int main(void)
         Ar[nn] = INFINITY;
         oracle_zgeev_batched(n, batch, A, eig, flag, sweeps, -1);
         oracle_zgeev_batched(n, batch, A, eig, flag, sweeps, 0);   /* sweep cap 0 */
+        oracle_zgeev_batched_dual(n, batch, A, eig, flag, sweeps, -1);
+        oracle_zgeev_batched_dual(n, batch, A, eig, flag, sweeps, 0);
         const char jobs[] = {'N', 'P', 'S', 'B'};
         for (int j = 0; j < 4; ++j) oracle_zgeev_batched_bal(n, batch, A, eig, flag, -1, jobs[j]);
         oracle_dgeev_batched(n, batch, Ar, wr, wi, info, 1, -1);
