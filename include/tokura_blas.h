@@ -180,8 +180,11 @@ int tokuraSetBalancing(tokurablas_t *handle, char job);
  * (the convention of torch.linalg.eigvals' autograd), with x_i, y_i the right
  * and left eigenvectors, obtained by back-substitution on T_j and back-
  * transformed by U_j.  When `schur` is NULL the Schur form is recomputed from
- * A (memory-light mode, A must then be the unmodified input).  flag as in the
- * forward pass; a matrix with flag != 0 gets dA = NaN. */
+ * A (memory-light mode, A must then be the unmodified input, and gbar must be
+ * in the eigenvalue order of tokura_zgeev_batched_gpu run with the variants
+ * that share the recomputation's arithmetic: tokuraSetVariant 5 for n <= 16,
+ * 8 for n <= 32, 20 above; the flag is then written by the backward).  flag
+ * as in the forward pass; a matrix with flag != 0 gets dA = NaN. */
 size_t tokura_get_zgeev_schur_worksize(int n, int batch_count);
 int tokura_zgeev_batched_schur_gpu(tokurablas_t *handle, int n, int batch_count,
                                    const hipDoubleComplex *A, hipDoubleComplex *eigenvalues,
