@@ -33,8 +33,8 @@ int default_variant(int n)
 {
     // what tokuraAutotune picks on an MI355X at batches of 20,000 and 50,000
     if (n <= 4) return 0;    // fused, 4 lanes/matrix, 16 matrices/wave
-    if (n <= 8) return 1;    // fused 8 x 4
-    if (n <= 16) return 5;   // fused 16 x 16
+    if (n <= 8) return 28;   // split 8 x 4: packed Hessenberg to the workspace, then QR
+    if (n <= 16) return 27;  // split 16 x 8
     if (n <= 32) return 14;  // split 32 x 32
     if (n <= 40) return 24;  // split, one matrix per wavefront, two bulges per sweep, LDS for n = 40
     if (n <= 48) return 19;  // the same with LDS for n = 48

@@ -190,7 +190,10 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
     X(22, DUAL_1W, 64, 64)              \
     X(23, DUAL, 56, 64)                 \
     X(24, DUAL, 40, 64)                 \
-    X(25, DUAL, 60, 64)
+    X(25, DUAL, 60, 64)                 \
+    X(26, SPLIT, 16, 16)                \
+    X(27, SPLIT, 16, 8)                 \
+    X(28, SPLIT, 8, 4)
 
 static const VariantInfo g_variants[] = {
 #define X(id, kind, nb, lpm) {id, VK_##kind, nb, lpm, 64 / lpm},
