@@ -148,7 +148,7 @@ __device__ __forceinline__ void qr_tile(zd *lds, const RaggedView &V, int4 td, z
     const PackedLayout pk;
     int info = 0;
     if constexpr (LPM == 64) {   // one matrix per wavefront: two bulges per sweep
-        if (n > 1) info = zlahqr_dual<64>(H, pk, n, t, itmax);
+        if (n > 1) info = zlahqr_dual<64, false, (NB > 56)>(H, pk, n, t, itmax);
     } else {
         if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
     }

@@ -153,7 +153,7 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
     int info = 0;
     if constexpr (DUAL) {
         static_assert(LPM == 64 || (LPM == 32 && NB <= 32), "two-bulge QR: a lane per position");
-        if (n > 1) info = zlahqr_dual<LPM>(H, pk, n, t, itmax);
+        if (n > 1) info = zlahqr_dual<LPM, false, (NB > 56)>(H, pk, n, t, itmax);
     } else {
         if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
     }
