@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A
 template <int CLS, int NB, int LPM>
 __global__ __launch_bounds__(64) void k_ragged_qr(char *work, zd *__restrict__ eig, int8_t *__restrict__ flag, int itmax)
 {
-    __shared__ zd lds[(64 / LPM) * packed_size(NB)];
+    __shared__ zd lds[(64 / LPM) * packed_size(NB) + (LPM == 64 && NB > 56)];   // + the idle-lane element
     const RaggedView V(work);
     const int t0 = V.h->tile_cls[CLS], t1 = V.h->tile_cls[CLS + 1];
     const int lane = threadIdx.x & 63;

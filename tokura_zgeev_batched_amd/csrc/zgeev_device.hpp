@@ -1195,9 +1195,10 @@ __device__ __forceinline__ double flip_sign(double x, unsigned m)
 //  * Packed layout only: column c starts c + 2 elements after column c - 1,
 //    so every address of a step is an increment of the previous step's.
 //  * DENSE (for LDS sized for n > 56, where nearly every lane holds a
-//    position): the bulk operands are loaded by every lane, idle lanes from
-//    their own column's first element, with no branch around the loads
-//    (n = 64: the QR 1.8% faster; at n = 40 the same form is 1.7% slower).
+//    position): every lane loads, updates and stores its bulk operands with
+//    no branch, idle lanes at H[packed_size(n)], one element past the matrix
+//    that the caller allocates for them (n = 64: the QR 1.8% faster with the
+//    loads alone; at n = 40 the same form is 1.7% slower).
 template <int LPM = 64, bool WANTT = false, bool DENSE = false>
 __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n, int t, int itmax_override,
                                            zd *Z = nullptr, int zld = 0)
@@ -1216,6 +1217,7 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
     const bool leader = (t & (HALF - 1)) == 0;   // the first lane of each half stores its window
     const int p = t;                   // bulk position of this lane
     const int cbp = lay.cb(p < n ? p : n - 1);
+    const int idle_slot = packed_size(n);   // DENSE: the idle lanes' element
     const bool zp = WANTT && Z != nullptr && p < n;   // this lane updates row p of Z
     zd *const zr = zp ? Z + p : Z;     // element (p, c) of Z at zr[c * zld]
     int kdefl = 0;
@@ -1307,12 +1309,12 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                 ax = row ? cbp + k : cbks + p;
                 ay = ax + (row ? 1 : k + 2);
                 if constexpr (DENSE) {
-                    if (!(row || col)) ax = ay = cbp;   // in bounds: the loads need no branch
+                    if (!(row || col)) ax = ay = idle_slot;
                 }
             };
             auto phase_apply = [&](const bool row, const bool col, const int ax, const int ay, zd x, zd y,
                                    const zd tr, const zd vr, const double t2r) {
-                if (row || col) {
+                if (DENSE || row || col) {
                     const unsigned m = row ? 0x80000000u : 0u;
                     const zd c1 = mk(tr.re, flip_sign(tr.im, m));
                     const zd vv = mk(vr.re, flip_sign(vr.im, m ^ 0x80000000u));

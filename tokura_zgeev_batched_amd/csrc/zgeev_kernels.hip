@@ -134,7 +134,7 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
                                                   int itmax)
 {
     constexpr int GPW = 64 / LPM, S = (NB + LPM - 1) / LPM, PS = packed_size(NB);
-    __shared__ zd lds[GPW * PS];
+    __shared__ zd lds[GPW * PS + (DUAL && NB > 56)];   // + the two-bulge QR's idle-lane element
     const int lane = threadIdx.x & 63;
     const int g = lane / LPM, t = lane % LPM;
     const long long mat = (long long)blockIdx.x * GPW + g;
