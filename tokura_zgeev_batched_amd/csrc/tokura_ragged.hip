@@ -148,7 +148,7 @@ __device__ __forceinline__ void qr_tile(zd *lds, const RaggedView &V, int4 td, z
     const PackedLayout pk;
     int info = 0;
     if constexpr (LPM == 64) {   // one matrix per wavefront: two bulges per sweep
-        if (n > 1) info = zlahqr_dual<64, false, (NB > 56)>(H, pk, n, t, itmax);
+        if (n > 1) info = zlahqr_dual<64, false, true>(H, pk, n, t, itmax);
     } else {
         if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
     }
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A
 template <int CLS, int NB, int LPM>
 __global__ __launch_bounds__(64) void k_ragged_qr(char *work, zd *__restrict__ eig, int8_t *__restrict__ flag, int itmax)
 {
-    __shared__ zd lds[(64 / LPM) * packed_size(NB) + (LPM == 64 && NB > 56)];   // + the idle-lane element
+    __shared__ zd lds[(64 / LPM) * packed_size(NB) + (LPM == 64)];   // + the idle-lane element
     const RaggedView V(work);
     const int t0 = V.h->tile_cls[CLS], t1 = V.h->tile_cls[CLS + 1];
     const int lane = threadIdx.x & 63;

@@ -1194,11 +1194,10 @@ __device__ __forceinline__ double flip_sign(double x, unsigned m)
 // Returns 0 or 1 + the 0-based index of the first unconverged eigenvalue.
 //  * Packed layout only: column c starts c + 2 elements after column c - 1,
 //    so every address of a step is an increment of the previous step's.
-//  * DENSE (for LDS sized for n > 56, where nearly every lane holds a
-//    position): every lane loads, updates and stores its bulk operands with
-//    no branch, idle lanes at H[packed_size(n)], one element past the matrix
-//    that the caller allocates for them (n = 64: the QR 1.8% faster with the
-//    loads alone; at n = 40 the same form is 1.7% slower).
+//  * DENSE (one matrix per wavefront): every lane loads, updates and stores
+//    its bulk operands with no branch, idle lanes at H[packed_size(n)], one
+//    element past the matrix that the caller allocates for them (the QR 5.9%
+//    faster at n = 64, 2.9% at n = 56, 2.2% at n = 48, bit-identical).
 template <int LPM = 64, bool WANTT = false, bool DENSE = false>
 __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n, int t, int itmax_override,
                                            zd *Z = nullptr, int zld = 0)

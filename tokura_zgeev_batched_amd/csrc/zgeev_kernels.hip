@@ -134,7 +134,7 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
                                                   int itmax)
 {
     constexpr int GPW = 64 / LPM, S = (NB + LPM - 1) / LPM, PS = packed_size(NB);
-    __shared__ zd lds[GPW * PS + (DUAL && NB > 56)];   // + the two-bulge QR's idle-lane element
+    __shared__ zd lds[GPW * PS + (DUAL && LPM == 64)];   // + the two-bulge QR's idle-lane element
     const int lane = threadIdx.x & 63;
     const int g = lane / LPM, t = lane % LPM;
     const long long mat = (long long)blockIdx.x * GPW + g;
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
     int info = 0;
     if constexpr (DUAL) {
         static_assert(LPM == 64 || (LPM == 32 && NB <= 32), "two-bulge QR: a lane per position");
-        if (n > 1) info = zlahqr_dual<LPM, false, (NB > 56)>(H, pk, n, t, itmax);
+        if (n > 1) info = zlahqr_dual<LPM, false, (LPM == 64)>(H, pk, n, t, itmax);
     } else {
         if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
     }
