@@ -183,13 +183,13 @@ template <int NB, int LPM>
 __global__ __launch_bounds__(64) void k_schur_qr(int n, int batch, zd *__restrict__ eig, zd *Tout, long long tstride,
                                                  zd *Uout, long long ustride, int8_t *flag, int itmax)
 {
-    constexpr int GPW = 64 / LPM, PER = packed_size(NB);
+    constexpr int GPW = 64 / LPM, PER = packed_size(NB) + 1;   // + the idle lanes' element
     __shared__ zd lds[GPW * PER];
     const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
     const long long mat = (long long)blockIdx.x * GPW + g;
     if (mat >= batch || flag[mat] == 2) return;
-    const int info = schur_qr_one<LPM>(lds + g * PER, n, t, Tout + mat * tstride, Uout + mat * ustride,
-                                       eig + mat * n, itmax);
+    const int info = schur_qr_one<LPM, (LPM == 64)>(lds + g * PER, n, t, Tout + mat * tstride,
+                                                    Uout + mat * ustride, eig + mat * n, itmax);
     if (t == 0) flag[mat] = info != 0 ? 1 : 0;
 }
 

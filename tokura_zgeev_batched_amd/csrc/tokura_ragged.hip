@@ -372,7 +372,7 @@ template <int NB, int LPM, int CLS>
 __global__ __launch_bounds__(64) void k_ragged_schur_qr(char *work, zd *__restrict__ eig, zd *schur, int8_t *flag,
                                                         int itmax)
 {
-    constexpr int GPW = 64 / LPM, PER = packed_size(NB);
+    constexpr int GPW = 64 / LPM, PER = packed_size(NB) + 1;   // + the idle lanes' element
     __shared__ zd lds[GPW * PER];
     const RaggedView V(work);
     int lo, hi;
@@ -388,8 +388,8 @@ __global__ __launch_bounds__(64) void k_ragged_schur_qr(char *work, zd *__restri
             const int b = V.order[j], n = V.nsz[b];
             if (flag[b] != 2) {
                 zd *Tm = schur + 2 * V.moff[b];
-                const int info = schur_qr_one<LPM>(lds + g * PER, n, t, Tm, Tm + (long long)n * n,
-                                                   eig + V.eoff[b], itmax);
+                const int info = schur_qr_one<LPM, (LPM == 64)>(lds + g * PER, n, t, Tm, Tm + (long long)n * n,
+                                                                eig + V.eoff[b], itmax);
                 if (t == 0) flag[b] = info != 0 ? 1 : 0;
             }
         }

@@ -1235,7 +1235,8 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
             if (i - l + 1 < DUAL_MIN) {
                 // the whole block [l, i] by single-shift sweeps
                 if constexpr (WANTT) {
-                    const int info = zlahqr_core<LPM, 1, true, false, PackedLayout, FullLayout, true>(
+                    int info;
+                    [[clang::always_inline]] info = zlahqr_core<LPM, 1, true, false, PackedLayout, FullLayout, true>(
                         H, lay, n, t, itmax_override, Z, FullLayout{zld}, i, l);
                     if (info != 0) return info;
                 } else {

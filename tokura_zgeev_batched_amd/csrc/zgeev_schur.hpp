@@ -268,7 +268,7 @@ __device__ __forceinline__ bool schur_hq_one(zd *H, zd *taus, const zd *__restri
 // (lane p owns row p, as it wrote it), then the eigenvalues to out and T
 // (zeros below the diagonal) back to Th, row by row by the lane that read
 // it.  Returns the QR INFO.
-template <int LPM>
+template <int LPM, bool DENSE = false>
 __device__ __forceinline__ int schur_qr_one(zd *H, int n, int t, zd *Th, zd *Ug, zd *__restrict__ out, int itmax)
 {
     const PackedLayout lay{};
@@ -278,8 +278,10 @@ __device__ __forceinline__ int schur_qr_one(zd *H, int n, int t, zd *Th, zd *Ug,
     lds_order();
     int info = 0;
     if constexpr (LPM == 64) {
-        if (schur_dual(n)) info = zlahqr_dual<LPM, true>(H, lay, n, t, itmax, Ug, n);
-        else if (n > 1) info = zlahqr_core<LPM, 1, true, false>(H, lay, n, t, itmax, Ug, FullLayout{n});
+        // DENSE: H has a spare element past the matrix for the idle lanes
+        if (schur_dual(n)) info = zlahqr_dual<LPM, true, DENSE>(H, lay, n, t, itmax, Ug, n);
+        else if (n > 1)   // inlined: as a call, its ABI would raise the kernel's VGPRs to 274 and add stack
+            [[clang::always_inline]] info = zlahqr_core<LPM, 1, true, false>(H, lay, n, t, itmax, Ug, FullLayout{n});
     } else {
         if (n > 1) info = zlahqr_core<LPM, 1, true, false>(H, lay, n, t, itmax, Ug, FullLayout{n});
     }
