@@ -680,7 +680,8 @@ __device__ __forceinline__ int schur_wg(zd *H, zd *Z, int zld, zd *taus, double 
     __syncthreads();
     if (w == 0) {
         pack_hessenberg<64>(H, lay, n, t);
-        const int info = zlahqr_dual<64, true>(H, PackedLayout{}, n, t, itmax, Z, zld);
+        // DENSE: the spare element past the packed matrix lies inside the full-layout buffer
+        const int info = zlahqr_dual<64, true, true>(H, PackedLayout{}, n, t, itmax, Z, zld);
         unpack_hessenberg<64>(H, lay, n, t);
         for (int c = 0; c < n; ++c)
             for (int r = c + 1 + t; r < n; r += 64) H[lay(r, c)] = mk(0.0, 0.0);
