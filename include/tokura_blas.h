@@ -23,6 +23,8 @@
  *   1  iteration cap reached (30*max(10,n) QR sweeps per eigenvalue, LAPACK's
  *      cap; the reference uses n^4 total)
  *   2  non-finite entry in the input matrix (eigenvalues are NaN)
+ *   3  (recompute backward only) the forward's eigenvalues are not those of
+ *      the matrix, so no cotangent can be paired (dA is NaN)
  *
  * One handle must not be used by two host threads at once (the reference
  * keeps per-call state in the handle, tokura_zgeev_batched.cu:224-235).
@@ -179,12 +181,22 @@ int tokuraSetBalancing(tokurablas_t *handle, char job);
  * writes dA[j] = dL/dA_j = sum_i gbar_ij * y_i x_i^H / conj(y_i^H x_i)
  * (the convention of torch.linalg.eigvals' autograd), with x_i, y_i the right
  * and left eigenvectors, obtained by back-substitution on T_j and back-
- * transformed by U_j.  When `schur` is NULL the Schur form is recomputed from
- * A (memory-light mode, A must then be the unmodified input, and gbar must be
- * in the eigenvalue order of tokura_zgeev_batched_gpu run with the variants
- * that share the recomputation's arithmetic: tokuraSetVariant 5 for n <= 16,
- * 8 for n <= 32, 20 above; the flag is then written by the backward).  flag
- * as in the forward pass; a matrix with flag != 0 gets dA = NaN. */
+ * transformed by U_j.  gbar is in the order of the forward's eigenvalues
+ * (the Schur diagonal).  `schur` must not be NULL (-3); `A` is not read.
+ * flag is the forward's; a matrix with flag != 0 gets dA = NaN.
+ *
+ * Recompute backward (memory-light mode, nothing kept from the forward but
+ * its eigenvalues): tokura_zgeev_batched_backward_recompute_gpu rebuilds the
+ * Schur form from A (which must be the forward's unmodified input) with
+ * `work` of tokura_get_zgeeveigenvaluesgetworspacesize(n, batch) bytes as
+ * scratch (T and the recomputed eigenvalues; U is parked in dA), pairs each
+ * recomputed eigenvalue with the nearest of `eigenvalues` (the forward's
+ * output, from any same-size variant, in its order), applies the cotangent
+ * gbar of that forward index, and writes dA as above.  flag: 0/1/2 of the
+ * recomputed forward, or 3 when the eigenvalues of matrix j are not those
+ * of A_j (a pair further apart than 1e-6 * max |lambda|; dA = NaN).
+ * Returns 0, -1 handle, -2 n, -3 A NULL, -4 eigenvalues or gbar NULL, -5 dA
+ * or work NULL, -6 batch_count <= 0, -7 flag NULL, 2 launch failure. */
 size_t tokura_get_zgeev_schur_worksize(int n, int batch_count);
 int tokura_zgeev_batched_schur_gpu(tokurablas_t *handle, int n, int batch_count,
                                    const hipDoubleComplex *A, hipDoubleComplex *eigenvalues,
@@ -193,6 +205,10 @@ int tokura_zgeev_batched_backward_gpu(tokurablas_t *handle, int n, int batch_cou
                                       const hipDoubleComplex *A, const hipDoubleComplex *schur,
                                       const hipDoubleComplex *gbar, hipDoubleComplex *dA,
                                       char *flag, hipStream_t stream);
+int tokura_zgeev_batched_backward_recompute_gpu(tokurablas_t *handle, int n, int batch_count,
+                                                const hipDoubleComplex *A, const hipDoubleComplex *eigenvalues,
+                                                const hipDoubleComplex *gbar, hipDoubleComplex *dA,
+                                                hipDoubleComplex *work, char *flag, hipStream_t stream);
 
 /* Ragged differentiation (no counterpart in the reference): the forward
  * with Schur vectors and the backward for a ragged batch prepared with

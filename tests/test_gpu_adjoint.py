@@ -6,7 +6,11 @@
   n > 32: two-bulge QR);
 * adjoint: vs the oracle's dL/dA (same method on the CPU), vs CPU
   torch.linalg.eigvals autograd in complex128, and vs central finite
-  differences (h = 1e-6) with gradient relative error <= 1e-7.
+  differences (h = 1e-6) with gradient relative error <= 1e-7;
+* recompute backward (tokura_zgeev_batched_backward_recompute_gpu): its
+  pairing of the forward's eigenvalues with its own Schur order, after the
+  default (autotuned) forward, after a shuffled forward, and a forward that
+  belongs to other matrices (flag 3).
 """
 import numpy as np
 import pytest
@@ -14,7 +18,7 @@ import torch
 
 import oracle
 import tokura_zgeev_batched_amd as tk
-from tokura_zgeev_batched_amd.autograd import backward_batched, eigvals_autograd, schur_batched
+from tokura_zgeev_batched_amd.autograd import backward_batched, backward_recompute, eigvals_autograd, schur_batched
 
 pytestmark = pytest.mark.gpu
 
@@ -44,14 +48,88 @@ def test_schur_forward(n):
     assert oracle.max_relerr(ref, e) <= 1e-10
 
 
-def test_recompute_forward_order_matches_schur():
-    rng = np.random.default_rng(5)
-    for n in (3, 16, 17, 24, 32, 33, 40, 48, 57, 64):
-        M = torch.tensor(rand(rng, 30, n), device="cuda")
-        e1, f1 = eigvals_autograd(M, mode="store")
-        e2, f2 = eigvals_autograd(M, mode="recompute")
-        # same arithmetic in both kernels: identical values in identical order
-        assert torch.equal(e1, e2), (n, float((e1 - e2).abs().max()))
+def _colmajor(M):
+    return torch.tensor(M, device="cuda").transpose(-1, -2).contiguous()
+
+
+@pytest.mark.parametrize("n", [1, 3, 5, 16, 17, 24, 32, 33, 40, 48, 57, 64])
+def test_recompute_after_default_forward_vs_oracle(n):
+    """The eigenvalue forward with the handle's default variants (whose order
+    need not be the Schur form's), then the recompute backward: dL/dA equals
+    the oracle's with the cotangents in the forward's order."""
+    rng = np.random.default_rng(700 + n)
+    B = 8
+    M = rand(rng, B, n)
+    g = rng.standard_normal((B, n)) + 1j * rng.standard_normal((B, n))
+    A = _colmajor(M)
+    print(f"[case] default forward + recompute backward n={n}", flush=True)
+    eig, flag = tk.eigvals(torch.tensor(M, device="cuda"))
+    dA, f2 = backward_recompute(n, B, A, eig, torch.tensor(g, device="cuda"))
+    torch.cuda.synchronize()
+    assert int(flag.abs().sum()) == 0 and int(f2.abs().sum()) == 0
+    G = dA.transpose(-1, -2).cpu().numpy()
+    e = eig.cpu().numpy()
+    for b in range(B):
+        Go = oracle_grad_in_order(M[b], g[b], e[b])
+        assert np.abs(G[b] - Go).max() <= 1e-9 * max(1.0, np.abs(Go).max()), b
+
+
+@pytest.mark.parametrize("n", [2, 9, 16, 20, 32, 36, 50, 64])
+def test_recompute_pairs_shuffled_forward(n):
+    """Forward eigenvalues and cotangents shuffled by the same random
+    permutation per matrix give the same dL/dA bits as the stored-Schur
+    backward with them in the Schur order."""
+    rng = np.random.default_rng(800 + n)
+    B = 12
+    M = rand(rng, B, n)
+    A = _colmajor(M)
+    g = torch.tensor(rng.standard_normal((B, n)) + 1j * rng.standard_normal((B, n)), device="cuda")
+    e_s, schur, fl = schur_batched(A, n, B)
+    dA_store, _ = backward_batched(n, B, g, schur, fl.clone())
+    perm = torch.tensor(np.stack([rng.permutation(n) for _ in range(B)]), device="cuda")
+    e_p = torch.gather(e_s, 1, perm)
+    g_p = torch.gather(g, 1, perm)
+    print(f"[case] shuffled forward n={n}", flush=True)
+    dA_rc, f2 = backward_recompute(n, B, A, e_p, g_p)
+    torch.cuda.synchronize()
+    assert int(f2.abs().sum()) == 0
+    assert torch.equal(dA_rc, dA_store)
+
+
+@pytest.mark.parametrize("n", [4, 16, 30, 64])
+def test_recompute_flags_foreign_eigenvalues(n):
+    """Eigenvalues of other matrices: flag 3 and dA = NaN for those
+    matrices only."""
+    rng = np.random.default_rng(850 + n)
+    B = 6
+    M = rand(rng, B, n)
+    A = _colmajor(M)
+    eig, _ = tk.eigvals(torch.tensor(M, device="cuda"))
+    bad = eig.clone()
+    bad[1] = eig[2]
+    bad[4] = eig[4] * 3.0
+    g = torch.ones((B, n), dtype=torch.complex128, device="cuda")
+    dA, f = backward_recompute(n, B, A, bad, g)
+    torch.cuda.synchronize()
+    assert f.cpu().numpy().tolist() == [0, 3, 0, 0, 3, 0]
+    G = dA.cpu().numpy()
+    assert np.isnan(G[[1, 4]]).all() and np.isfinite(G[[0, 2, 3, 5]]).all()
+
+
+def test_recompute_entry_validation():
+    h = tk.default_handle()
+    f = tk.lib().tokura_zgeev_batched_backward_recompute_gpu
+    x = torch.zeros(64, dtype=torch.complex128, device="cuda")
+    p = x.data_ptr()
+    assert f(None, 4, 1, p, p, p, p, p, p, None) == -1
+    assert f(h.ptr, 65, 1, p, p, p, p, p, p, None) == -2
+    assert f(h.ptr, 4, 1, None, p, p, p, p, p, None) == -3
+    assert f(h.ptr, 4, 1, p, None, p, p, p, p, None) == -4
+    assert f(h.ptr, 4, 1, p, p, p, p, None, p, None) == -5
+    assert f(h.ptr, 4, 0, p, p, p, p, p, p, None) == -6
+    assert f(h.ptr, 4, 1, p, p, p, p, p, None, None) == -7
+    # the stored-Schur entry needs the Schur form
+    assert tk.lib().tokura_zgeev_batched_backward_gpu(h.ptr, 4, 1, p, None, p, p, p, None) == -3
 
 
 def oracle_grad_in_order(Mb, g, order_eigs):
@@ -133,25 +211,24 @@ def test_adjoint_vs_finite_differences():
     assert err4.max() <= 1e-6
 
 
-@pytest.mark.parametrize("n", [17, 32, 33, 40, 47, 50, 64])
+@pytest.mark.parametrize("n", [5, 17, 32, 33, 40, 47, 50, 64])
 def test_store_and_recompute_gradients_bitwise(n):
-    """The stored-Schur backward (for n > 16 a 256-thread workgroup per matrix,
-    LDS sized for n <= 32, 40, 48 or 64, after a forward that accumulates U in
-    global memory) and the recompute backward (one wavefront, everything in
-    LDS) share the per-element arithmetic: identical dL/dA bits."""
+    """The stored-Schur backward and the recompute backward (the same
+    two-launch Schur form rebuilt with T in the workspace and U in dA, then
+    the same adjoint arithmetic) give identical dL/dA bits when the recompute
+    is handed the Schur forward's eigenvalues."""
     rng = np.random.default_rng(900 + n)
     M = rand(rng, 40, n)
+    A = _colmajor(M)
     g = torch.tensor(rng.standard_normal((40, n)) + 1j * rng.standard_normal((40, n)), device="cuda")
-    grads = []
     print(f"[case] store vs recompute gradients n={n}", flush=True)
-    for mode in ("store", "recompute"):
-        Mt = torch.tensor(M, device="cuda", requires_grad=True)
-        eig, flag = eigvals_autograd(Mt, mode=mode)
-        assert int(flag.abs().sum()) == 0
-        (torch.conj(g) * eig).real.sum().backward()
-        grads.append(Mt.grad)
+    e_s, schur, fl = schur_batched(A, n, 40)
+    assert int(fl.abs().sum()) == 0
+    d1, _ = backward_batched(n, 40, g, schur, fl.clone())
+    d2, f2 = backward_recompute(n, 40, A, e_s, g)
     torch.cuda.synchronize()
-    assert torch.equal(grads[0], grads[1])
+    assert int(f2.abs().sum()) == 0
+    assert torch.equal(d1, d2)
 
 
 def test_adjoint_flags_nan():
@@ -189,3 +266,16 @@ def test_diff_config_sample_full_size():
     for k in range(len(idx)):
         Go = oracle_grad_in_order(Mh[k], gh[k], eh[k])
         assert np.abs(Gh[k] - Go).max() <= 1e-9 * max(1.0, np.abs(Go).max())
+    # recompute mode on the whole batch: the default forward, then the
+    # recompute backward; a sample against the oracle
+    Md = M.detach()
+    A = Md.transpose(-1, -2).contiguous()
+    e_d, f_d = tk.eigvals(Md)
+    dA, f2 = backward_recompute(n, B, A, e_d, gv)
+    torch.cuda.synchronize()
+    assert int(f_d.abs().sum()) == 0 and int(f2.abs().sum()) == 0
+    Gr = dA.transpose(-1, -2)[idx].cpu().numpy()
+    er = e_d[idx].cpu().numpy()
+    for k in range(len(idx)):
+        Go = oracle_grad_in_order(Mh[k], gh[k], er[k])
+        assert np.abs(Gr[k] - Go).max() <= 1e-9 * max(1.0, np.abs(Go).max())

@@ -28,7 +28,7 @@ import os
 import torch
 
 __all__ = ["lib", "Handle", "zgeev_batched", "eigvals", "workspace_size", "MAX_N", "zgeev_vectors", "eig",
-           "RaggedBatch", "eigvals_autograd", "schur_batched", "backward_batched"]
+           "RaggedBatch", "eigvals_autograd", "schur_batched", "backward_batched", "backward_recompute"]
 
 MAX_N = 64
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -62,6 +62,7 @@ _SIGNATURES = {
     "tokura_get_zgeev_schur_worksize": ([_I, _I], _SZ),
     "tokura_zgeev_batched_schur_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P], _I),
     "tokura_zgeev_batched_backward_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P, _P], _I),
+    "tokura_zgeev_batched_backward_recompute_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P, _P, _P], _I),
     "tokura_get_zgeev_vectors_worksize": ([_I, _I], ctypes.c_size_t),
     "tokura_zgeev_batched_vectors_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P, _P, _P], _I),
     "tokura_get_zgeev_ragged_schur_worksize": ([_I, _P], _SZ),
@@ -309,4 +310,4 @@ def eig(M: torch.Tensor, left: bool = False, right: bool = True, handle: Handle 
 
 
 from .ragged import RaggedBatch  # noqa: E402
-from .autograd import eigvals_autograd, schur_batched, backward_batched  # noqa: E402
+from .autograd import eigvals_autograd, schur_batched, backward_batched, backward_recompute  # noqa: E402

@@ -96,37 +96,6 @@ __global__ __launch_bounds__(NT) void k_zgeev_vectors_wg(int n, int batch, const
                                                VR ? VR + mat * nn : nullptr);
 }
 
-// Recompute backward for n > 32, one matrix per workgroup of NT threads:
-// schur_wg with U in LDS (the arithmetic of the two-launch forward), U
-// parked in dA, then adjoint_wg's phases.  Writes the flag (0/1/2); flagged
-// matrices get dA = NaN.
-template <int NT>
-__global__ __launch_bounds__(NT) void k_zgeev_adjoint_rc_wg(int n, const zd *__restrict__ A,
-                                                           const zd *__restrict__ gbar, zd *dA,
-                                                           int8_t *__restrict__ flag, int itmax)
-{
-    using G = AdjGeom<64>;
-    __shared__ zd lds[G::PER];
-    __shared__ double red[8];
-    __shared__ int ish;
-    const long long mat = blockIdx.x, nn = (long long)n * n;
-    const int tid = (int)threadIdx.x;
-    zd *B1 = lds, *B2 = lds + G::MAT, *taus = B2 + G::MAT, *gv = taus + 64;
-    const FullLayout lay{n | 1};
-    zd *dst = dA + mat * nn;
-    const int info = schur_wg<NT>(B1, B2, lay.ld, taus, red, &ish, A + mat * nn, n, itmax);
-    const int f = info < 0 ? 2 : (info != 0 ? 1 : 0);
-    if (tid == 0) flag[mat] = (int8_t)f;
-    if (f != 0) {
-        fill_nan<NT>(dst, (int)nn, tid);
-        return;
-    }
-    for (int e = tid; e < (int)nn; e += NT) dst[e] = B2[lay(e % n, e / n)];   // park U in dA
-    for (int i = tid; i < n; i += NT) gv[i] = gbar[mat * n + i];
-    __syncthreads();
-    adjoint_wg_phases<NT>(B1, B2, gv, n, dst, dst);
-}
-
 // Schur form in two launches (schur_hq_one, schur_qr_one), LPM lanes per
 // matrix.  The matrix's T and U slots are Tout + mat*tstride and
 // Uout + mat*ustride: the Schur-form output (T then U per matrix) or, for
@@ -222,10 +191,12 @@ __global__ __launch_bounds__(64) void k_vectors_from_schur(int n, int batch, con
     vectors_from_schur<LPM, true>(B1, lay, VRm ? VRm : VLm, FullLayout{n}, n, t, VLm, VRm);
 }
 
-template <int NB, int LPM, bool RECOMPUTE>
-__global__ __launch_bounds__(64) void k_zgeev_adjoint(int n, int batch, const zd *__restrict__ A,
-                                                      const zd *__restrict__ schur, const zd *__restrict__ gbar,
-                                                      zd *dA, int8_t *__restrict__ flag, int itmax)
+// Stored-Schur backward for n <= 16: LPM lanes per matrix, 64/LPM matrices
+// per wavefront.
+template <int NB, int LPM>
+__global__ __launch_bounds__(64) void k_zgeev_adjoint(int n, int batch, const zd *__restrict__ schur,
+                                                      const zd *__restrict__ gbar, zd *dA,
+                                                      const int8_t *__restrict__ flag)
 {
     using G = AdjGeom<NB>;
     constexpr int GPW = 64 / LPM;
@@ -233,11 +204,10 @@ __global__ __launch_bounds__(64) void k_zgeev_adjoint(int n, int batch, const zd
     const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
     const long long mat = (long long)blockIdx.x * GPW + g;
     if (mat >= batch) return;
-    zd *B1 = lds + g * G::PER, *B2 = B1 + G::MAT, *taus = B2 + G::MAT, *gv = taus + NB;
+    zd *B1 = lds + g * G::PER, *B2 = B1 + G::MAT, *gv = B2 + G::MAT + NB;
     const long long nn = (long long)n * n;
-    adjoint_one<NB, LPM, RECOMPUTE>(B1, B2, taus, gv, n, t, A ? A + mat * nn : nullptr,
-                                    schur ? schur + mat * 2 * nn : nullptr, gbar + mat * n, dA + mat * nn,
-                                    flag + mat, itmax);
+    const zd *Tm = schur + mat * 2 * nn;
+    adjoint_one<NB, LPM>(B1, B2, gv, n, t, Tm, Tm + nn, gbar + mat * n, nullptr, dA + mat * nn, flag[mat]);
 }
 
 // Stored-Schur backward for n > 16: one matrix per workgroup of
@@ -253,10 +223,63 @@ __global__ __launch_bounds__(NT) void k_zgeev_adjoint_wg(int n, const zd *__rest
     __shared__ zd lds[G::PER];
     const long long mat = blockIdx.x, nn = (long long)n * n;
     zd *B1 = lds, *B2 = lds + G::MAT, *gv = B2 + G::MAT;
-    adjoint_wg<NT>(B1, B2, gv, n, schur + mat * 2 * nn, gbar + mat * n, dA + mat * nn, flag[mat]);
+    const zd *Tm = schur + mat * 2 * nn;
+    (void)adjoint_wg<NT>(B1, B2, gv, n, Tm, Tm + nn, gbar + mat * n, dA + mat * nn, flag[mat]);
 }
 
-#define TKZ_ADJ_NB(X) X(16, 16) X(32, 32)   // n > 32: the workgroup kernels
+// Recompute backward, second launch: the Schur form of the first launch has
+// T in Tslot and U in dA (n x n each per matrix) and its eigenvalues in eo;
+// the forward's eigenvalues ef (any order) are paired with T's diagonal
+// (match_eigenvalues), so cotangent i of the forward drives eigenvalue perm^-1
+// of the Schur form; dA overwrites U in place.  Flag: the forward's 0/1/2,
+// or 3 when the eigenvalues do not match (then dA = NaN).
+template <int NB, int LPM>
+__global__ __launch_bounds__(64) void k_adjoint_matched(int n, int batch, const zd *__restrict__ Tslot,
+                                                        const zd *__restrict__ ef, const zd *__restrict__ gbar,
+                                                        zd *dA, int8_t *__restrict__ flag)
+{
+    using G = AdjGeom<NB>;
+    constexpr int GPW = 64 / LPM, S = (NB + LPM - 1) / LPM;
+    __shared__ zd lds[GPW * G::PER];
+    __shared__ int perm_s[GPW * NB];
+    const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
+    const long long mat = (long long)blockIdx.x * GPW + g;
+    if (mat >= batch) return;
+    zd *B1 = lds + g * G::PER, *B2 = B1 + G::MAT, *efl = B2 + G::MAT, *gv = efl + NB;
+    int *perm = perm_s + g * NB;
+    const long long nn = (long long)n * n;
+    const zd *Tm = Tslot + mat * nn;
+    zd *Um = dA + mat * nn;
+    const FullLayout lay{n | 1};
+    int f = flag[mat];
+    if (f == 0) {
+        load_full<LPM>(B1, lay, Tm, n, t);
+        for (int i = t; i < n; i += LPM) efl[i] = ef[mat * n + i];
+        lds_order();
+        if (!match_eigenvalues<LPM, S>(B1, lay, efl, perm, n, t)) {
+            f = 3;
+            if (t == 0) flag[mat] = 3;
+        }
+    }
+    adjoint_one<NB, LPM>(B1, B2, gv, n, t, Tm, Um, gbar + mat * n, perm, Um, f, true);
+}
+
+template <int NT, int NB>
+__global__ __launch_bounds__(NT) void k_adjoint_matched_wg(int n, const zd *__restrict__ Tslot,
+                                                           const zd *__restrict__ ef, const zd *__restrict__ gbar,
+                                                           zd *dA, int8_t *__restrict__ flag)
+{
+    using G = AdjGeom<NB>;
+    __shared__ zd lds[G::PER];
+    __shared__ int perm[NB];
+    __shared__ int ish;
+    const long long mat = blockIdx.x, nn = (long long)n * n;
+    zd *B1 = lds, *B2 = lds + G::MAT, *efl = B2 + G::MAT, *gv = efl + NB;
+    zd *Um = dA + mat * nn;
+    const int f = adjoint_wg<NT>(B1, B2, gv, n, Tslot + mat * nn, Um, gbar + mat * n, Um, flag[mat], ef + mat * n,
+                                 efl, perm, &ish);
+    if (threadIdx.x == 0 && f == 3) flag[mat] = 3;
+}
 
 static int adj_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 
@@ -344,13 +367,15 @@ static hipError_t launch_vectors(int n, int batch, const zd *A, zd *eig, zd *VL,
     return hipErrorInvalidValue;
 }
 
-static hipError_t launch_adjoint(int n, int batch, const zd *A, const zd *schur, const zd *gbar, zd *dA,
-                                 int8_t *flag, int itmax, hipStream_t s)
+// n > 16: one 256-thread workgroup per matrix, LDS sized in steps (two per CU
+// for n <= 48, three for n <= 40, four for n <= 32)
+static int adj_wg_nb(int n) { return n <= 32 ? 32 : (n <= 40 ? 40 : (n <= 48 ? 48 : 64)); }
+
+static hipError_t launch_adjoint(int n, int batch, const zd *schur, const zd *gbar, zd *dA, const int8_t *flag,
+                                 hipStream_t s)
 {
-    if (schur && n > 16) {   // one 256-thread workgroup per matrix, LDS sized in steps (two per CU for n <= 48,
-                             // three for n <= 40, four for n <= 32)
-        const int nb = n <= 32 ? 32 : (n <= 40 ? 40 : (n <= 48 ? 48 : 64));
-        switch (nb) {
+    if (n > 16) {
+        switch (adj_wg_nb(n)) {
 #define X(b)                                                                                                   \
     case b:                                                                                                    \
         hipLaunchKernelGGL((k_zgeev_adjoint_wg<TKZ_ADJ_WG_NT, b>), dim3(batch), dim3(TKZ_ADJ_WG_NT), 0, s, n, schur, \
@@ -361,25 +386,36 @@ static hipError_t launch_adjoint(int n, int batch, const zd *A, const zd *schur,
         }
         return hipGetLastError();
     }
-    if (!schur && n > 32) {   // recompute: one 256-thread workgroup per matrix
-        hipLaunchKernelGGL((k_zgeev_adjoint_rc_wg<TKZ_ADJ_WG_NT>), dim3(batch), dim3(TKZ_ADJ_WG_NT), 0, s, n, A, gbar,
-                           dA, flag, itmax);
-        return hipGetLastError();
-    }
-    switch (adj_nb(n)) {
-#define X(nb, lpm)                                                                                               \
-    case nb:                                                                                                     \
-        if (schur)                                                                                               \
-            hipLaunchKernelGGL((k_zgeev_adjoint<nb, lpm, false>), dim3((batch + 64 / lpm - 1) / (64 / lpm)),      \
-                               dim3(64), 0, s, n, batch, A, schur, gbar, dA, flag, itmax);                       \
-        else                                                                                                     \
-            hipLaunchKernelGGL((k_zgeev_adjoint<nb, lpm, true>), dim3((batch + 64 / lpm - 1) / (64 / lpm)),       \
-                               dim3(64), 0, s, n, batch, A, schur, gbar, dA, flag, itmax);                       \
-        return hipGetLastError();
-        TKZ_ADJ_NB(X)
+    hipLaunchKernelGGL((k_zgeev_adjoint<16, 16>), dim3((batch + 3) / 4), dim3(64), 0, s, n, batch, schur, gbar, dA,
+                       flag);
+    return hipGetLastError();
+}
+
+// Recompute backward: the two-launch Schur forward with T parked in `work`
+// (n^2 complex per matrix, then the recomputed eigenvalues, n per matrix:
+// the reference's workspace size) and U in dA, then the matched adjoint.
+static hipError_t launch_adjoint_recompute(int n, int batch, const zd *A, const zd *ef, const zd *gbar, zd *dA,
+                                           zd *work, int8_t *flag, int itmax, hipStream_t s)
+{
+    const long long nn = (long long)n * n;
+    zd *eo = work + nn * batch;
+    hipError_t e = launch_schur_split(n, batch, A, eo, work, nn, dA, nn, flag, itmax, s);
+    if (e != hipSuccess) return e;
+    if (n > 16) {
+        switch (adj_wg_nb(n)) {
+#define X(b)                                                                                                    \
+    case b:                                                                                                     \
+        hipLaunchKernelGGL((k_adjoint_matched_wg<TKZ_ADJ_WG_NT, b>), dim3(batch), dim3(TKZ_ADJ_WG_NT), 0, s, n, work, \
+                           ef, gbar, dA, flag);                                                                 \
+        break;
+            X(32) X(40) X(48) X(64)
 #undef X
+        }
+        return hipGetLastError();
     }
-    return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_adjoint_matched<16, 16>), dim3((batch + 3) / 4), dim3(64), 0, s, n, batch, work, ef, gbar,
+                       dA, flag);
+    return hipGetLastError();
 }
 
 }  // namespace tkz
@@ -414,15 +450,34 @@ extern "C" int tokura_zgeev_batched_backward_gpu(tokurablas_t *handle, int n, in
                                                  const hipDoubleComplex *gbar, hipDoubleComplex *dA, char *flag,
                                                  hipStream_t stream)
 {
+    (void)A;   // the stored Schur form alone defines the backward
     if (handle == nullptr) return -1;
     if (!(n > 0 && n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE)) return -2;
-    if (A == nullptr && schur == nullptr) return -3;
+    if (schur == nullptr) return -3;
     if (gbar == nullptr) return -4;
     if (dA == nullptr) return -5;
     if (!(batch_count > 0)) return -6;
     if (flag == nullptr) return -7;
-    hipError_t e = launch_adjoint(n, batch_count, (const zd *)A, (const zd *)schur, (const zd *)gbar, (zd *)dA,
-                                  (int8_t *)flag, handle->max_sweeps, stream);
+    hipError_t e = launch_adjoint(n, batch_count, (const zd *)schur, (const zd *)gbar, (zd *)dA,
+                                  (const int8_t *)flag, stream);
+    return e == hipSuccess ? 0 : 2;
+}
+
+extern "C" int tokura_zgeev_batched_backward_recompute_gpu(tokurablas_t *handle, int n, int batch_count,
+                                                           const hipDoubleComplex *A,
+                                                           const hipDoubleComplex *eigenvalues,
+                                                           const hipDoubleComplex *gbar, hipDoubleComplex *dA,
+                                                           hipDoubleComplex *work, char *flag, hipStream_t stream)
+{
+    if (handle == nullptr) return -1;
+    if (!(n > 0 && n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE)) return -2;
+    if (A == nullptr) return -3;
+    if (eigenvalues == nullptr || gbar == nullptr) return -4;
+    if (dA == nullptr || work == nullptr) return -5;
+    if (!(batch_count > 0)) return -6;
+    if (flag == nullptr) return -7;
+    hipError_t e = launch_adjoint_recompute(n, batch_count, (const zd *)A, (const zd *)eigenvalues, (const zd *)gbar,
+                                            (zd *)dA, (zd *)work, (int8_t *)flag, handle->max_sweeps, stream);
     return e == hipSuccess ? 0 : 2;
 }
 

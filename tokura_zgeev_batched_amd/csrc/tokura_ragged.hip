@@ -409,13 +409,14 @@ __global__ __launch_bounds__(64) void k_ragged_adjoint(char *work, const zd *__r
     int lo, hi;
     class_range(V, CLS, lo, hi);
     const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
-    zd *B1 = lds + g * G::PER, *B2 = B1 + G::MAT, *taus = B2 + G::MAT, *gv = taus + NB;
+    zd *B1 = lds + g * G::PER, *B2 = B1 + G::MAT, *gv = B2 + G::MAT + NB;
     for (int base = lo + (int)blockIdx.x * GPW; base < hi; base += (int)gridDim.x * GPW) {
         const int j = base + g;
         if (j < hi) {
-            const int b = V.order[j];
-            adjoint_one<NB, LPM, false>(B1, B2, taus, gv, V.nsz[b], t, nullptr, schur + 2 * V.moff[b],
-                                        gbar + V.eoff[b], dA + V.moff[b], flag + b, itmax);
+            const int b = V.order[j], n = V.nsz[b];
+            const zd *Tm = schur + 2 * V.moff[b];
+            adjoint_one<NB, LPM>(B1, B2, gv, n, t, Tm, Tm + (long long)n * n, gbar + V.eoff[b], nullptr,
+                                 dA + V.moff[b], flag[b]);
         }
         lds_order();
     }
@@ -434,8 +435,9 @@ __global__ __launch_bounds__(NT) void k_ragged_adjoint_wg(char *work, const zd *
     const int lo = C0 == 0 ? 0 : V.h->cls_end[C0 - 1], hi = V.h->cls_end[C1];
     zd *B1 = lds, *B2 = lds + G::MAT, *gv = B2 + G::MAT;
     for (int j = lo + (int)blockIdx.x; j < hi; j += (int)gridDim.x) {
-        const int b = V.order[j];
-        adjoint_wg<NT>(B1, B2, gv, V.nsz[b], schur + 2 * V.moff[b], gbar + V.eoff[b], dA + V.moff[b], flag[b]);
+        const int b = V.order[j], n = V.nsz[b];
+        const zd *Tm = schur + 2 * V.moff[b];
+        (void)adjoint_wg<NT>(B1, B2, gv, n, Tm, Tm + (long long)n * n, gbar + V.eoff[b], dA + V.moff[b], flag[b]);
         __syncthreads();
     }
 }

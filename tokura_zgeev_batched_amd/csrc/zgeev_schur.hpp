@@ -468,40 +468,97 @@ __device__ __forceinline__ void trevc_right_upper(const zd *B1, zd *B2, const Fu
     }
 }
 
-// dL/dA of one matrix into dst (n x n column-major) for cotangents gbar_m,
-// with the buffers B1, B2, taus, gv of AdjGeom<NB>.  RECOMPUTE: the Schur
-// form is rebuilt from Amat and the flag written to *flagp; otherwise it is
-// read from schur_m (T then U) and *flagp is the forward's flag.  A matrix
-// with a non-zero flag gets dA = NaN.
-template <int NB, int LPM, bool RECOMPUTE>
-__device__ __forceinline__ void adjoint_one(zd *B1, zd *B2, zd *taus, zd *gv, int n, int t,
-                                            const zd *__restrict__ Amat, const zd *__restrict__ schur_m,
-                                            const zd *__restrict__ gbar_m, zd *dst, int8_t *flagp, int itmax)
+// Pairs the eigenvalues of a Schur form (T's diagonal, this function's
+// order) with those of a forward pass (ef[0..n), any order): perm[i] = the
+// index j of ef closest to T(i,i) in |Re| + |Im|.  Each position i finds its
+// nearest j; when two positions pick the same j (near-equal eigenvalues), the
+// group's first lane assigns greedily in position order instead, every j
+// used once.  Returns false when a pair lies further apart than 1e-6 times
+// the largest |T(i,i)| (then the eigenvalues are not those of this matrix).
+// Positions i = t + s*LPM (S per lane); perm and ef are LDS, all of the group
+// reads them after the call.
+template <int LPM, int S>
+__device__ __forceinline__ bool match_eigenvalues(const zd *B1, const FullLayout lay, const zd *ef, int *perm, int n,
+                                                  int t)
+{
+    double scale = 0.0;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int i = t + s * LPM;
+        if (i < n) scale = fmax(scale, cabs1(B1[lay(i, i)]));
+    }
+    scale = group_max<LPM>(scale);
+    const double tol = 1e-6 * scale;
+    bool clash = false, far = false;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int i = t + s * LPM;
+        if (i < n) {
+            const zd w = B1[lay(i, i)];
+            int best = 0;
+            double bd = cabs1(zsub(w, ef[0]));
+            for (int j = 1; j < n; ++j) {
+                const double d = cabs1(zsub(w, ef[j]));
+                if (d < bd) { bd = d; best = j; }
+            }
+            perm[i] = best;
+            far |= !(bd <= tol);
+        }
+    }
+    lds_order();
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int i = t + s * LPM;
+        if (i < n)
+            for (int k = 0; k < n; ++k) clash |= (k != i && perm[k] == perm[i]);
+    }
+    if (group_any<LPM>(clash)) {
+        lds_order();
+        far = false;
+        if (t == 0) {
+            uint64_t used = 0;
+            for (int i = 0; i < n; ++i) {
+                const zd w = B1[lay(i, i)];
+                int best = -1;
+                double bd = 0.0;
+                for (int j = 0; j < n; ++j) {
+                    if ((used >> j) & 1ull) continue;
+                    const double d = cabs1(zsub(w, ef[j]));
+                    if (best < 0 || d < bd) { bd = d; best = j; }
+                }
+                used |= 1ull << best;
+                perm[i] = best;
+                far |= !(bd <= tol);
+            }
+        }
+    }
+    lds_order();
+    return !group_any<LPM>(far);
+}
+
+// dL/dA of one matrix into dst (n x n column-major) from its Schur form, T
+// at Tg and U at Ug (n x n column-major each), for cotangents gbar_m, with
+// the buffers B1, B2, taus, gv of AdjGeom<NB>.  f is the forward's flag; a
+// matrix with f != 0 gets dA = NaN.  perm (LDS, may be null): cotangent i is
+// gbar_m[perm[i]].  dst may be Ug: every read of Ug precedes, in the same
+// lane, the first write of dst.
+template <int NB, int LPM>
+__device__ __forceinline__ void adjoint_one(zd *B1, zd *B2, zd *gv, int n, int t, const zd *__restrict__ Tg,
+                                            const zd *Ug, const zd *__restrict__ gbar_m, const int *perm, zd *dst,
+                                            int f, bool loaded = false)
 {
     constexpr int S = (NB + LPM - 1) / LPM;
     const FullLayout lay{n | 1};
     const long long nn = (long long)n * n;
-    const zd *Ug;   // global copy of U
-    int f;
-    if constexpr (RECOMPUTE) {
-        const int info = schur_in_lds<NB, LPM>(B1, B2, taus, lay, Amat, n, t, itmax);
-        f = info < 0 ? 2 : (info != 0 ? 1 : 0);
-        if (t == 0) *flagp = (int8_t)f;
-        if (f == 0) {
-            store_full<LPM>(B2, lay, dst, n, t);   // park U in dA
-            __threadfence();
-        }
-        Ug = dst;
-    } else {
-        f = *flagp;
-        if (f == 0) load_full<LPM>(B1, lay, schur_m, n, t);
-        Ug = schur_m + nn;
-    }
     if (f != 0) {
         fill_nan<LPM>(dst, (int)nn, t);
         return;
     }
-    for (int i = t; i < n; i += LPM) gv[i] = gbar_m[i];
+    if (!loaded) {
+        load_full<LPM>(B1, lay, Tg, n, t);
+        lds_order();
+    }
+    for (int i = t; i < n; i += LPM) gv[i] = gbar_m[perm ? perm[i] : i];
     trevc_right_upper<LPM, S>(B1, B2, lay, n, t);
     // ---- Y (buf1 strictly lower, y_i(i) = 1): y_i^H (T - t_ii) = 0, rows r > i
     trevc_left_lower<LPM, S>(B1, lay, n, t);
@@ -614,30 +671,52 @@ __device__ __forceinline__ void adjoint_wg_phases(zd *B1, zd *B2, const zd *gv, 
     }
 }
 
-// dL/dA of one matrix (n <= 64) from its stored Schur form by a workgroup of
-// NT threads (NT/64 wavefronts): adjoint_one's quantities with the same
-// per-element arithmetic, so bitwise its results, spread over the
-// wavefronts.  X (wavefront 0) and Y (wavefront 1) are solved side by side;
-// then wavefront w forms rows a = w (mod NT/64) of W, rows of P = W U^H over
-// W's rows (a row of P needs only its own row of W, so no other wavefront
-// reads what is overwritten), and columns c = w (mod NT/64) of dA = U P.
-// Buffers (AdjGeom<NB>): B1 = T/Y -> W -> P, B2 = X -> U, gv = the cotangent.
-// `f` is the matrix's forward flag (uniform over the workgroup).
+// dL/dA of one matrix (n <= 64) from its Schur form (T at Tg, U at Ug, n x n
+// column-major each) by a workgroup of NT threads (NT/64 wavefronts):
+// adjoint_one's quantities with the same per-element arithmetic, so bitwise
+// its results, spread over the wavefronts.  X (wavefront 0) and Y (wavefront
+// 1) are solved side by side; then wavefront w forms rows a = w (mod NT/64)
+// of W, rows of P = W U^H over W's rows (a row of P needs only its own row
+// of W, so no other wavefront reads what is overwritten), and columns
+// c = w (mod NT/64) of dA = U P.  Buffers (AdjGeom<NB>): B1 = T/Y -> W -> P,
+// B2 = X -> U, gv = the cotangent.  `f` is the matrix's forward flag
+// (uniform over the workgroup).  With ef (the forward's eigenvalues, global)
+// the cotangents are matched to T's diagonal first (match_eigenvalues by
+// wavefront 0, perm and efl: n ints and n complex of LDS); a failed match
+// returns 3 with dA = NaN.  Returns the flag the matrix ends with.  dst may be
+// Ug (every read of Ug precedes a workgroup barrier before the first write
+// of dst).
 template <int NT>
-__device__ __forceinline__ void adjoint_wg(zd *B1, zd *B2, zd *gv, int n, const zd *__restrict__ schur_m,
-                                           const zd *__restrict__ gbar_m, zd *__restrict__ dst, int f)
+__device__ __forceinline__ int adjoint_wg(zd *B1, zd *B2, zd *gv, int n, const zd *__restrict__ Tg, const zd *Ug,
+                                          const zd *__restrict__ gbar_m, zd *dst, int f,
+                                          const zd *__restrict__ ef = nullptr, zd *efl = nullptr,
+                                          int *perm = nullptr, int *ishared = nullptr)
 {
     const int tid = (int)threadIdx.x;
     const FullLayout lay{n | 1};
     const int nn = n * n;
     if (f != 0) {
         fill_nan<NT>(dst, nn, tid);
-        return;
+        return f;
     }
-    for (int e = tid; e < nn; e += NT) B1[lay(e % n, e / n)] = schur_m[e];
-    for (int i = tid; i < n; i += NT) gv[i] = gbar_m[i];
+    for (int e = tid; e < nn; e += NT) B1[lay(e % n, e / n)] = Tg[e];
+    if (ef) {
+        for (int i = tid; i < n; i += NT) efl[i] = ef[i];
+        __syncthreads();
+        if (tid < 64) {
+            const bool ok = match_eigenvalues<64, 1>(B1, lay, efl, perm, n, tid);
+            if (tid == 0) *ishared = ok ? 0 : 3;
+        }
+        __syncthreads();
+        if (*ishared != 0) {
+            fill_nan<NT>(dst, nn, tid);
+            return 3;
+        }
+    }
+    for (int i = tid; i < n; i += NT) gv[i] = gbar_m[perm && ef ? perm[i] : i];
     __syncthreads();
-    adjoint_wg_phases<NT>(B1, B2, gv, n, schur_m + nn, dst);
+    adjoint_wg_phases<NT>(B1, B2, gv, n, Ug, dst);
+    return 0;
 }
 
 // The Schur form of one matrix for n > 32 in LDS by a workgroup of NT
