@@ -31,6 +31,9 @@
  *                      scaling), and oracle_zgeev_batched_bal = balancing then
  *                      the method; optional, the reference never balances
  *                      (README.md:79)
+ *   norm_range_scale <- ZGEEV's scaling of A into [sqrt(safmin)/eps, its
+ *                      inverse] before the reduction (a power of two here;
+ *                      the reference does not scale, README.md:76-79)
  *
  * Flag codes per matrix (reference README.md:72 defines only 0 == success):
  *   0 converged, 1 iteration cap reached, 2 non-finite input.
@@ -508,8 +511,34 @@ static int all_finite(int n, const zc *a)
     return 1;
 }
 
+/* ZGEEV's norm-range scaling (LAPACK zgeev scales A when its largest entry
+ * lies outside [SMLNUM, BIGNUM], SMLNUM = sqrt(safmin)/eps = 2^-459, BIGNUM =
+ * 1/SMLNUM) as the GPU path applies it: the test on m = max(|Re|, |Im|) over
+ * the entries, the factor the power of two 2^e that brings m into
+ * [2^-458, 2^-457) or [2^458, 2^459) (exact), where LAPACK multiplies by
+ * SMLNUM/ANRM or BIGNUM/ANRM (ZLASCL).  Scales a in place and returns e (0:
+ * in range, nothing done); the eigenvalues are multiplied by 2^-e. */
+static int norm_range_scale(int n, zc *a)
+{
+    double m = 0.0;
+    for (size_t k = 0; k < (size_t)n * n; ++k) m = fmax(m, fmax(fabs(creal(a[k])), fabs(cimag(a[k]))));
+    if (!(m > 0.0) || (m >= 0x1p-459 && m <= 0x1p459)) return 0;
+    const int k = ilogb(m);
+    const int e = m < 0x1p-459 ? -458 - k : 458 - k;
+    const double f = ldexp(1.0, e);
+    for (size_t j = 0; j < (size_t)n * n; ++j) a[j] = CMPLX(creal(a[j]) * f, cimag(a[j]) * f);
+    return e;
+}
+
+static void norm_range_unscale(int n, zc *w, int e)
+{
+    if (e == 0) return;
+    const double f = ldexp(1.0, -e);
+    for (int i = 0; i < n; ++i) w[i] = CMPLX(creal(w[i]) * f, cimag(w[i]) * f);
+}
+
 /* The batched method: Hessenberg reduction then eigenvalues-only QR, per
- * matrix, as the GPU path computes it.  A: batch x (n x n) column-major,
+ * matrix, as the GPU path computes it (after the norm-range scaling).  A: batch x (n x n) column-major,
  * contiguous; eig[i + j*n]; flag[j].  sweeps (optional) per matrix. */
 void oracle_zgeev_batched(int n, int batch, const zc *A, zc *eig, signed char *flag,
                           int *sweeps, int itmax_override)
@@ -524,6 +553,7 @@ void oracle_zgeev_batched(int n, int batch, const zc *A, zc *eig, signed char *f
             for (int i = 0; i < n; ++i) w[i] = CMPLX(NAN, NAN);
             flag[b] = 2;
         } else {
+            const int e = norm_range_scale(n, h);
             oracle_zgehd2(n, h, n, NULL, NULL);
             for (int i = 0; i < n; ++i) w[i] = h[IDX(i, i, n)];
             int info = oracle_zlahqr(0, 0, n, h, n, w, NULL, n, itmax_override, &sw);
@@ -531,6 +561,7 @@ void oracle_zgeev_batched(int n, int batch, const zc *A, zc *eig, signed char *f
                 /* unconverged eigenvalues: report the current diagonal */
                 for (int i = 0; i < info; ++i) w[i] = h[IDX(i, i, n)];
             }
+            norm_range_unscale(n, w, e);
             flag[b] = info != 0 ? 1 : 0;
         }
         if (sweeps) sweeps[b] = sw;
@@ -552,11 +583,13 @@ void oracle_zgeev_batched_dual(int n, int batch, const zc *A, zc *eig, signed ch
             for (int i = 0; i < n; ++i) w[i] = CMPLX(NAN, NAN);
             flag[b] = 2;
         } else {
+            const int e = norm_range_scale(n, h);
             oracle_zgehd2(n, h, n, NULL, NULL);
             for (int i = 0; i < n; ++i) w[i] = h[IDX(i, i, n)];
             int info = oracle_zlahqr_dual(n, h, n, w, itmax_override, &sw);
             if (info != 0)
                 for (int i = 0; i < info; ++i) w[i] = h[IDX(i, i, n)];
+            norm_range_unscale(n, w, e);
             flag[b] = info != 0 ? 1 : 0;
         }
         if (sweeps) sweeps[b] = sw;
@@ -818,12 +851,14 @@ void oracle_zgeev_batched_bal(int n, int batch, const zc *A, zc *eig, signed cha
             flag[b] = 2;
         } else {
             int ilo, ihi;
+            const int e = norm_range_scale(n, h);
             oracle_zgebal(job, n, h, n, &ilo, &ihi, scale);
             oracle_zgehd2(n, h, n, NULL, NULL);
             for (int i = 0; i < n; ++i) w[i] = h[IDX(i, i, n)];
             int info = oracle_zlahqr(0, 0, n, h, n, w, NULL, n, itmax_override, NULL);
             if (info != 0)
                 for (int i = 0; i < info; ++i) w[i] = h[IDX(i, i, n)];
+            norm_range_unscale(n, w, e);
             flag[b] = info != 0 ? 1 : 0;
         }
         free(h);

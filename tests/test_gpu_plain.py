@@ -95,14 +95,59 @@ def test_closed_forms(handle):
     assert not flag.any() and np.all(e == 0)
 
 
-def test_scaled_inputs(handle):
-    rng = np.random.default_rng(4)
-    M = rand(rng, 16, 24)
-    base, _ = oracle.lapack_zgeev(M)
-    for s in (1e-120, 1e-30, 1e30, 1e120):
+SCALES = (1e-305, 1e-300, 1e-290, 1e-250, 1e-200, 1e-150, 1e-120, 1e-30, 1e30, 1e120, 1e150, 1e200, 1e250,
+          1e290, 1e300)
+
+
+@pytest.mark.parametrize("n", [4, 12, 24, 40, 57, 64])
+def test_scaled_inputs(handle, n):
+    """ZGEEV's norm-range scaling: entries from 1e-305 to 1e300 against host
+    LAPACK zgeev on the same matrices, every default variant's size range
+    (fused, split, two-bulge, pool)."""
+    rng = np.random.default_rng(4 + n)
+    M = rand(rng, 16, n)
+    for s in SCALES:
+        ref, _ = oracle.lapack_zgeev(M * s)
         e, flag = gpu_eigs(M * s, handle)
-        assert not flag.any()
-        assert oracle.max_relerr(base * s, e) <= TOL_LAPACK, s
+        assert not flag.any(), s
+        assert oracle.max_relerr(ref, e) <= TOL_LAPACK, s
+        me, _ = oracle.method_zgeev(M * s)
+        assert oracle.max_relerr(me, e) <= TOL_METHOD, s
+
+
+def test_scaled_inputs_every_variant():
+    rng = np.random.default_rng(41)
+    for n in (9, 33, 64):
+        M = rand(rng, 70, n)
+        for s in (1e-300, 1e-290, 1e290):
+            ref, _ = oracle.lapack_zgeev(M * s)
+            for v in range(40):
+                h = tk.Handle()
+                if tk.lib().tokuraSetVariant(h.ptr, n, v) != 0:
+                    h.close()
+                    continue
+                e, flag = gpu_eigs(M * s, h)
+                h.close()
+                assert not flag.any(), (n, s, v)
+                assert oracle.max_relerr(ref, e) <= TOL_LAPACK, (n, s, v)
+
+
+def test_scaled_inputs_ragged():
+    from tokura_zgeev_batched_amd.ragged import RaggedBatch, pack
+    rng = np.random.default_rng(42)
+    sizes = rng.integers(1, 65, 120)
+    scales = np.array(SCALES)[rng.integers(0, len(SCALES), 120)]
+    mats = [rand(rng, 1, int(n))[0] * s for n, s in zip(sizes, scales)]
+    flat, offs = pack(mats)
+    rb = RaggedBatch(offs)
+    eig, flag = rb.solve(torch.tensor(flat, device="cuda"))
+    torch.cuda.synchronize()
+    assert int(flag.abs().sum()) == 0
+    e = eig.cpu().numpy()
+    for b, m in enumerate(mats):
+        ref, _ = oracle.lapack_zgeev(m[None])
+        got = e[None, rb.eig_offsets[b]:rb.eig_offsets[b + 1]]
+        assert oracle.max_relerr(ref, got) <= TOL_LAPACK, (b, m.shape[0], scales[b])
 
 
 def test_flags(handle):

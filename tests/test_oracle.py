@@ -404,3 +404,25 @@ def test_dual_restatement_closed_forms():
     assert oracle.max_relerr(ref[None], w) <= 1e-10
     w, f = oracle.method_zgeev_dual(rng.standard_normal((2, 30, 30))[None][0].astype(complex), itmax=0)
     assert (f == 1).all()
+
+
+def test_norm_range_scaling_vs_lapack():
+    """ZGEEV's norm-range scaling in the restatement: at entries near 1e-300 the
+    QR's deflation threshold (safmin n / ulp, about 1e-291) is no longer far
+    below the sub-diagonal, and products of two entries underflow; LAPACK
+    zgeev scales A into [2^-459, 2^459] first.  The restatement (single- and
+    two-bulge QR) must match LAPACK zgeev at every scale, and its power-of-two
+    factor must leave in-range matrices untouched: eig(2^k A) = 2^k eig(A)
+    bit for bit while 2^k A stays in range."""
+    rng = np.random.default_rng(4)
+    M = rng.standard_normal((12, 24, 24)) + 1j * rng.standard_normal((12, 24, 24))
+    base, _ = oracle.method_zgeev(M)
+    for s in (1e-305, 1e-300, 1e-290, 1e-250, 1e-200, 1e-139, 1e139, 1e200, 1e250, 1e290, 1e300):
+        ref, _ = oracle.lapack_zgeev(M * s)
+        for fn in (oracle.method_zgeev, oracle.method_zgeev_dual):
+            e, f = fn(M * s)
+            assert not f.any()
+            assert oracle.max_relerr(ref, e) <= 1e-12, (s, fn.__name__)
+    for k in (-400, -100, 100, 400):
+        e, _ = oracle.method_zgeev(M * 2.0 ** k)
+        np.testing.assert_array_equal(e, base * 2.0 ** k)

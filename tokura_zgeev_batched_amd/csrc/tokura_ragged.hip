@@ -107,9 +107,11 @@ __device__ __forceinline__ void cls_range(const RaggedView &V, int c, int &lo, i
     hi = V.h->cls_end[c];
 }
 
+// The norm-range factor of the eigenvalues goes to the matrix's first
+// eigenvalue slot, which the QR reads before writing the eigenvalues.
 template <int NB, int LPM>
 __device__ __forceinline__ void hess_tile(zd *lds, const RaggedView &V, int4 td, const zd *__restrict__ A,
-                                          int8_t *__restrict__ flag, int bal)
+                                          int8_t *__restrict__ flag, int bal, zd *__restrict__ eig)
 {
     constexpr int LDP = NB | 1;
     const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
@@ -118,8 +120,15 @@ __device__ __forceinline__ void hess_tile(zd *lds, const RaggedView &V, int4 td,
     const int n = V.nsz[b];
     zd *H = lds + g * NB * LDP;
     const FullLayout lay{n | 1};
-    const bool ok = load_full<LPM>(H, lay, A + V.moff[b], n, t);
+    double mx;
+    const bool ok = load_full<LPM>(H, lay, A + V.moff[b], n, t, &mx);
     lds_order();
+    const int se = ok ? zgeev_scale_exp(mx) : 0;
+    if (se) {
+        scale_full<LPM>(H, lay, n, t, se);
+        lds_order();
+    }
+    if (ok && t == 0) eig[V.eoff[b]] = mk(ldexp(1.0, -se), 0.0);
     if (ok && bal) zgebal_group<LPM, (NB + LPM - 1) / LPM>(H, lay, n, t, bal);
     if (ok) hessenberg<LPM>(H, lay, n, t);
     store_packed<LPM>(H, lay, V.H + V.hoff[b], n, t);
@@ -143,6 +152,7 @@ __device__ __forceinline__ void qr_tile(zd *lds, const RaggedView &V, int4 td, z
     zd *H = lds + g * PS;
     const zd *src = V.H + V.hoff[b];
     const int ps = packed_size(n);
+    const double us = out[0].re;   // the Hessenberg's norm-range factor
     for (int e = t; e < ps; e += LPM) H[e] = src[e];
     lds_order();
     const PackedLayout pk;
@@ -152,7 +162,7 @@ __device__ __forceinline__ void qr_tile(zd *lds, const RaggedView &V, int4 td, z
     } else {
         if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
     }
-    for (int r = t; r < n; r += LPM) out[r] = H[pk(r, r)];
+    for (int r = t; r < n; r += LPM) out[r] = us != 1.0 ? zscale(us, H[pk(r, r)]) : H[pk(r, r)];
     if (t == 0) flag[b] = info != 0 ? 1 : 0;
     lds_order();
 }
@@ -162,7 +172,7 @@ __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // Hessenberg of the small classes (B, C): one wavefront per tile.
 template <int LB, int LC>
 __global__ __launch_bounds__(64) void k_ragged_hess(const zd *__restrict__ A, char *work, int8_t *__restrict__ flag,
-                                                    int bal)
+                                                    int bal, zd *__restrict__ eig)
 {
     constexpr int SZ = cmax((64 / LB) * 32 * 33, (64 / LC) * 16 * 17);
     __shared__ zd lds[SZ];
@@ -170,8 +180,8 @@ __global__ __launch_bounds__(64) void k_ragged_hess(const zd *__restrict__ A, ch
     const int nt = V.h->ntiles;
     for (int tile = V.h->tile_small0 + blockIdx.x; tile < nt; tile += gridDim.x) {
         const int4 td = V.tiles[tile];
-        if (td.x == kClsB) hess_tile<32, LB>(lds, V, td, A, flag, bal);
-        else hess_tile<16, LC>(lds, V, td, A, flag, bal);
+        if (td.x == kClsB) hess_tile<32, LB>(lds, V, td, A, flag, bal, eig);
+        else hess_tile<16, LC>(lds, V, td, A, flag, bal, eig);
         lds_order();
     }
 }
@@ -180,7 +190,7 @@ __global__ __launch_bounds__(64) void k_ragged_hess(const zd *__restrict__ A, ch
 // matrix, quads of threads sharing each row/column, LDS sized for the class.
 template <int NB, int CLS>
 __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A, char *work,
-                                                        int8_t *__restrict__ flag, int bal)
+                                                        int8_t *__restrict__ flag, int bal, zd *__restrict__ eig)
 {
     __shared__ zd lds[NB * (NB | 1)];
     __shared__ double red[8];
@@ -194,12 +204,24 @@ __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A
         const FullLayout lay{n | 1};
         const zd *src = A + V.moff[b];
         bool bad = false;
+        double mx = 0.0;
         for (int e = tid; e < n * n; e += 256) {
             const zd v = src[e];
             bad |= !zfinite(v);
+            mx = fmax(mx, fmax(fabs(v.re), fabs(v.im)));
             lds[lay(e % n, e / n)] = v;
         }
+        mx = group_max<64>(mx);
+        if ((tid & 63) == 0) red[tid >> 6] = mx;
         const int ok = __syncthreads_or(bad) == 0;
+        for (int w = 0; w < 4; ++w) mx = fmax(mx, red[w]);
+        const int se = ok ? zgeev_scale_exp(mx) : 0;
+        if (se) {
+            const double f = ldexp(1.0, se);
+            for (int e = tid; e < n * n; e += 256) lds[lay(e % n, e / n)] = zscale(f, lds[lay(e % n, e / n)]);
+        }
+        __syncthreads();   // red is reused below; the scaled matrix is complete
+        if (ok && tid == 0) eig[V.eoff[b]] = mk(ldexp(1.0, -se), 0.0);
         if (ok && bal) {             // balancing by the first wavefront
             if (tid < 64) zgebal_group<64, 1>(lds, lay, n, tid, bal);
             __syncthreads();
@@ -228,9 +250,7 @@ __global__ __launch_bounds__(64) void k_ragged_qr(char *work, zd *__restrict__ e
     const int t0 = V.h->tile_cls[CLS], t1 = V.h->tile_cls[CLS + 1];
     const int lane = threadIdx.x & 63;
     for (;;) {
-        int k = 0;
-        if (lane == 0) k = atomicAdd(&V.h->qctr[CLS], 1);
-        const int tile = t0 + __shfl(k, 0);
+        const int tile = t0 + __builtin_amdgcn_readfirstlane(atomicAdd(&V.h->qctr[CLS], lane == 0 ? 1 : 0));
         if (tile >= t1) break;
         qr_tile<NB, LPM>(lds, V, V.tiles[tile], eig, flag, itmax);
     }
@@ -253,14 +273,14 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
 #define X(c, nb)                                                                                          \
     {                                                                                                      \
         static const int occ = resident_blocks(k_ragged_hess_wg<nb, c>, 256);                              \
-        hipLaunchKernelGGL((k_ragged_hess_wg<nb, c>), dim3(cus * occ * 4), dim3(256), 0, s, A, work, flag, bal); \
+        hipLaunchKernelGGL((k_ragged_hess_wg<nb, c>), dim3(cus * occ * 4), dim3(256), 0, s, A, work, flag, bal, eig); \
     }
     TKZ_BIG_CLS(X)
 #undef X
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     static const int occ_h = resident_blocks(k_ragged_hess<LB, LC>);
-    hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag, bal);
+    hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag, bal, eig);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     // largest class first: the tail of one launch is at most one tile of it
@@ -379,9 +399,9 @@ __global__ __launch_bounds__(64) void k_ragged_schur_qr(char *work, zd *__restri
     cls_range(V, CLS, lo, hi);
     const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
     for (;;) {
-        int k = 0;
-        if (lane == 0) k = atomicAdd(&V.h->qctr[CLS], 1);
-        const int base = lo + __shfl(k, 0) * GPW;
+        // every lane takes part in the atomic (lane 0 adds 1, the others 0),
+        // so no branch on the lane index precedes the loop's back edge
+        const int base = lo + __builtin_amdgcn_readfirstlane(atomicAdd(&V.h->qctr[CLS], lane == 0 ? 1 : 0)) * GPW;
         if (base >= hi) break;
         const int j = base + g;
         if (j < hi) {

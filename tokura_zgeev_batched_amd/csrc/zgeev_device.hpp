@@ -633,23 +633,55 @@ __device__ void form_q(zd *Q, const FullLayout ql, zd *A, const FullLayout lay, 
 }
 
 // Coalesced load of matrix `src` (n*n column-major) into LDS; returns
-// whether all entries are finite (group-uniform).
+// whether all entries are finite (group-uniform).  With mx, also the largest
+// |Re| or |Im| of the matrix (group-uniform).
 template <int LPM>
-__device__ __forceinline__ bool load_full(zd *H, const FullLayout lay, const zd *__restrict__ src, int n, int t)
+__device__ __forceinline__ bool load_full(zd *H, const FullLayout lay, const zd *__restrict__ src, int n, int t,
+                                          double *mx = nullptr)
 {
     const int nn = n * n;
     bool bad = false;
+    double m = 0.0;
     int r = t % n, c = t / n;
     const int dr = LPM % n, dc = LPM / n;
     for (int e = t; e < nn; e += LPM) {
         zd v = src[e];
         bad |= !zfinite(v);
+        m = fmax(m, fmax(fabs(v.re), fabs(v.im)));
         H[lay(r, c)] = v;
         r += dr;
         c += dc;
         if (r >= n) { r -= n; ++c; }
     }
+    if (mx) *mx = group_max<LPM>(m);
     return !group_any<LPM>(bad);
+}
+
+// ZGEEV's norm-range scaling: LAPACK scales A when its largest entry lies
+// outside [SMLNUM, BIGNUM], SMLNUM = sqrt(safmin)/eps = 2^-459, BIGNUM =
+// 1/SMLNUM, so that the QR's deflation threshold (safmin n / ulp) and its
+// products of two entries stay clear of underflow and overflow.  Here the
+// test is on m = max(|Re|, |Im|) over the entries and the factor is the power
+// of two 2^e (exact) that brings m into [2^-458, 2^-457) or [2^458, 2^459);
+// e = 0 when m is in range (then nothing is multiplied).  The eigenvalues
+// are multiplied by 2^-e at the end.
+__host__ __device__ inline int zgeev_scale_exp(double m)
+{
+    if (!(m > 0.0) || (m >= 0x1p-459 && m <= 0x1p459)) return 0;
+    const int k = ilogb(m);
+    return m < 0x1p-459 ? -458 - k : 458 - k;
+}
+
+// Multiplies the n x n LDS matrix by 2^e (positions t, t+LPM, ... of the
+// column-major order).
+template <int LPM>
+__device__ __forceinline__ void scale_full(zd *H, const FullLayout lay, int n, int t, int e)
+{
+    const double f = ldexp(1.0, e);
+    for (int k = t; k < n * n; k += LPM) {
+        zd &v = H[lay(k % n, k / n)];
+        v = zscale(f, v);
+    }
 }
 
 // Store the upper Hessenberg part of an LDS matrix in packed order
@@ -1198,9 +1230,17 @@ __device__ __forceinline__ double flip_sign(double x, unsigned m)
 //    its bulk operands with no branch, idle lanes at H[packed_size(n)], one
 //    element past the matrix that the caller allocates for them (the QR 5.9%
 //    faster at n = 64, 2.9% at n = 56, 2.2% at n = 48, bit-identical).
-template <int LPM = 64, bool WANTT = false, bool DENSE = false>
+//  * Hook (eigenvalues only): after every deflation, hook->moved(H, idle, i,
+//    t) may move the leading packed_size(i + 1) elements (columns 0..i, all
+//    that the rest of the QR reads) elsewhere and repoint H and the idle
+//    lanes' element; rows > i are final at that point.
+struct NoMove {
+    __device__ __forceinline__ void moved(zd *&, int &, int, int) {}
+};
+template <int LPM = 64, bool WANTT = false, bool DENSE = false, class Hook = NoMove>
 __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n, int t, int itmax_override,
-                                           zd *Z = nullptr, int zld = 0)
+                                           zd *Z = nullptr, int zld = 0, Hook *hook = nullptr,
+                                           int idle0 = -1)
 {
     static_assert(LPM == 32 || LPM == 64, "two-bulge QR: 32 or 64 lanes per matrix");
     constexpr int HALF = LPM / 2;
@@ -1216,7 +1256,7 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
     const bool leader = (t & (HALF - 1)) == 0;   // the first lane of each half stores its window
     const int p = t;                   // bulk position of this lane
     const int cbp = lay.cb(p < n ? p : n - 1);
-    const int idle_slot = packed_size(n);   // DENSE: the idle lanes' element
+    int idle_slot = idle0 >= 0 ? idle0 : packed_size(n);   // DENSE: the idle lanes' element
     const bool zp = WANTT && Z != nullptr && p < n;   // this lane updates row p of Z
     zd *const zr = zp ? Z + p : Z;     // element (p, c) of Z at zr[c * zld]
     int kdefl = 0;
@@ -1522,6 +1562,9 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
         if (!conv) return i + 1;
         kdefl = 0;
         i = l - 1;
+        if constexpr (!WANTT && !std::is_same<Hook, NoMove>::value) {
+            if (i >= 1) hook->moved(H, idle_slot, i, t);
+        }
     }
     return 0;
 }

@@ -19,6 +19,7 @@
 //             twice as many matrices are resident per CU at n = 64.
 #include "zgeev_device.hpp"
 #include "zgeev_launch.hpp"
+#include "zgeev_pool.hpp"
 
 namespace tkz {
 
@@ -42,7 +43,8 @@ __global__ __launch_bounds__(64) void k_zgeev_fused(int n, int batch, const zd *
     if (mat >= batch) return;
     zd *H = lds + g * NB * G::LDP;
     const FullLayout lay{n | 1};
-    const bool ok = load_full<LPM>(H, lay, A + mat * (long long)n * n, n, t);
+    double mx;
+    const bool ok = load_full<LPM>(H, lay, A + mat * (long long)n * n, n, t, &mx);
     lds_order();
     zd *out = eig + mat * n;
     if (!ok) {
@@ -50,19 +52,28 @@ __global__ __launch_bounds__(64) void k_zgeev_fused(int n, int batch, const zd *
         if (t == 0) flag[mat] = 2;
         return;
     }
+    const int se = zgeev_scale_exp(mx);
+    if (se) {
+        scale_full<LPM>(H, lay, n, t, se);
+        lds_order();
+    }
     if (bal) zgebal_group<LPM, G::S>(H, lay, n, t, bal);
     hessenberg<LPM>(H, lay, n, t);
     int info = 0;
     if (n > 1) info = zlahqr_eigs<LPM, G::S>(H, lay, n, t, itmax);
-    for (int r = t; r < n; r += LPM) out[r] = H[lay(r, r)];
+    const double us = ldexp(1.0, -se);
+    for (int r = t; r < n; r += LPM) out[r] = se ? zscale(us, H[lay(r, r)]) : H[lay(r, r)];
     if (t == 0) flag[mat] = info != 0 ? 1 : 0;
 }
 
 // Stage 1 of the split path: Hessenberg in LDS, packed store to `work`.
+// The norm-range factor 2^-e by which the QR multiplies the eigenvalues is
+// passed to it in the matrix's first eigenvalue slot, which the QR reads
+// before it writes the eigenvalues.
 template <int NB, int LPM>
 __global__ __launch_bounds__(64) void k_hessenberg_packed(int n, int batch, const zd *__restrict__ A,
                                                           zd *__restrict__ work, int8_t *__restrict__ flag,
-                                                          int bal)
+                                                          int bal, zd *__restrict__ eig)
 {
     using G = Geometry<NB, LPM>;
     __shared__ zd lds[G::GPW * NB * G::LDP];
@@ -72,8 +83,15 @@ __global__ __launch_bounds__(64) void k_hessenberg_packed(int n, int batch, cons
     if (mat >= batch) return;
     zd *H = lds + g * NB * G::LDP;
     const FullLayout lay{n | 1};
-    const bool ok = load_full<LPM>(H, lay, A + mat * (long long)n * n, n, t);
+    double mx;
+    const bool ok = load_full<LPM>(H, lay, A + mat * (long long)n * n, n, t, &mx);
     lds_order();
+    const int se = ok ? zgeev_scale_exp(mx) : 0;
+    if (se) {
+        scale_full<LPM>(H, lay, n, t, se);
+        lds_order();
+    }
+    if (ok && t == 0) eig[mat * n] = mk(ldexp(1.0, -se), 0.0);
     if (ok && bal) zgebal_group<LPM, G::S>(H, lay, n, t, bal);
     if (ok) hessenberg<LPM>(H, lay, n, t);
     store_packed<LPM>(H, lay, work + mat * (long long)packed_size(n), n, t);
@@ -92,12 +110,17 @@ __global__ __launch_bounds__(64) void k_hessenberg_packed(int n, int batch, cons
 // Stage 1 of the split path with a whole workgroup of NT threads per matrix
 // (teams of NT/64 threads share each row/column): at NB = 64 two workgroups
 // of 256 threads fit per CU, two waves per SIMD.
+// qctr (may be null): the QR's work counter, zeroed here for the pool
+// variants' QR kernel that follows on the same stream.
+// The norm-range factor goes to the matrix's first eigenvalue slot, as in
+// k_hessenberg_packed.
 template <int NB, int NT>
 __global__ __launch_bounds__(NT) void k_hessenberg_packed_wg(int n, int batch, const zd *__restrict__ A,
                                                              zd *__restrict__ work, int8_t *__restrict__ flag,
-                                                             int bal)
+                                                             int bal, zd *__restrict__ eig, int *qctr = nullptr)
 {
     constexpr int LDP = NB | 1;
+    if (qctr && blockIdx.x == 0 && threadIdx.x == 0) *qctr = 0;
     __shared__ zd lds[NB * LDP];
     __shared__ double red[2 * NT / 64];
     const long long mat = blockIdx.x;
@@ -106,12 +129,24 @@ __global__ __launch_bounds__(NT) void k_hessenberg_packed_wg(int n, int batch, c
     const long long nn = (long long)n * n;
     const zd *src = A + mat * nn;
     bool bad = false;
+    double mx = 0.0;
     for (int e = tid; e < nn; e += NT) {
         const zd v = src[e];
         bad |= !zfinite(v);
+        mx = fmax(mx, fmax(fabs(v.re), fabs(v.im)));
         lds[lay(e % n, e / n)] = v;
     }
+    mx = group_max<64>(mx);
+    if ((tid & 63) == 0) red[tid >> 6] = mx;
     const int ok = __syncthreads_or(bad) == 0;
+    for (int w = 0; w < NT / 64; ++w) mx = fmax(mx, red[w]);
+    const int se = ok ? zgeev_scale_exp(mx) : 0;
+    if (se) {
+        const double f = ldexp(1.0, se);
+        for (int e = tid; e < nn; e += NT) lds[lay(e % n, e / n)] = zscale(f, lds[lay(e % n, e / n)]);
+    }
+    __syncthreads();   // red is reused by the reduction below; the scaled matrix is complete
+    if (ok && tid == 0) eig[mat * n] = mk(ldexp(1.0, -se), 0.0);
     if (ok && bal) {                 // balancing by the first wavefront
         if (tid < 64) zgebal_group<64, (NB + 63) / 64>(lds, lay, n, tid, bal);
         __syncthreads();
@@ -147,6 +182,7 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
     zd *H = lds + g * PS;
     const int ps = packed_size(n);
     const zd *src = work + mat * (long long)ps;
+    const double us = out[0].re;   // the Hessenberg kernel's norm-range factor
     for (int e = t; e < ps; e += LPM) H[e] = src[e];
     lds_order();
     const PackedLayout pk;
@@ -157,8 +193,109 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
     } else {
         if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
     }
-    for (int r = t; r < n; r += LPM) out[r] = H[pk(r, r)];
+    for (int r = t; r < n; r += LPM) out[r] = us != 1.0 ? zscale(us, H[pk(r, r)]) : H[pk(r, r)];
     if (t == 0) flag[mat] = info != 0 ? 1 : 0;
+}
+
+// Stage 2 of the pool variants: one persistent workgroup of W wavefronts per
+// CU whose LDS is one arena of slots (zgeev_pool.hpp).  Each wavefront
+// claims the next matrix from the counter *qctr (zeroed by the Hessenberg
+// kernel), waits for a slot that holds it and runs the two-bulge QR, moving
+// to smaller slots as the matrix deflates; eigenvalues are those of
+// k_qr_packed<NB, 64, true> bit for bit.  Every wavefront leaves the loop
+// when the counter passes the batch.
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_qr_pool(int n, int batch, const zd *__restrict__ work,
+                                                    zd *__restrict__ eig, int8_t *__restrict__ flag, int itmax,
+                                                    int *qctr, PoolCfg P)
+{
+    __shared__ zd arena[kArena];
+    __shared__ unsigned free_mask;
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) free_mask = (1u << P.nslots) - 1u;
+    __syncthreads();
+    const int ps = packed_size(n);
+    const unsigned fit = pool_fit(P, ps + 1);
+    if (fit == 0) return;   // the host builds layouts whose first slot holds the matrix
+    for (;;) {
+        // lane 0 adds 1, the others 0 (see zgeev_pool.hpp on the control flow)
+        const int j = __builtin_amdgcn_readfirstlane(atomicAdd(qctr, lane == 0 ? 1 : 0));
+        if (j >= batch) break;
+        zd *out = eig + (long long)j * n;
+        if (__builtin_amdgcn_readfirstlane(flag[j]) == 2) {
+            if (lane < n) out[lane] = mk(__builtin_nan(""), __builtin_nan(""));
+            continue;
+        }
+        const double us = out[0].re;   // the Hessenberg kernel's norm-range factor
+        const int s = pool_acquire(&free_mask, fit, lane);
+        const int info = pool_qr_one(P, &free_mask, arena, s, work + (long long)j * ps, n, out, us, itmax, lane);
+        flag[j] = info != 0 ? 1 : 0;   // every lane stores the same byte
+    }
+}
+
+static const PoolLayout g_pool_layouts[] = {
+#define X(id, w, a0, a1, a2, a3, a4, a5) {w, {a0, a1, a2, a3, a4, a5}},
+    TKZ_POOL_LAYOUTS(X)
+#undef X
+};
+
+static int device_cus()
+{
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] == 0) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c < 1) c = 256;
+        cus[dev] = c;
+    }
+    return cus[dev];
+}
+
+// The 256-thread Hessenberg with LDS for the smallest of 40, 48, 56, 60, 64
+// that holds n (the arithmetic does not depend on the LDS size).
+static void launch_hess_wg(int n, int batch, const zd *A, zd *work, int8_t *flag, int bal, zd *eig, int *qctr,
+                           hipStream_t s)
+{
+    const int nb = n <= 40 ? 40 : (n <= 48 ? 48 : (n <= 56 ? 56 : (n <= 60 ? 60 : 64)));
+    switch (nb) {
+#define X(b)                                                                                                    \
+    case b:                                                                                                     \
+        hipLaunchKernelGGL((k_hessenberg_packed_wg<b, TKZ_HESS_NT>), dim3(batch), dim3(TKZ_HESS_NT), 0, s, n, batch, \
+                           A, work, flag, bal, eig, qctr);                                                      \
+        break;
+        X(40) X(48) X(56) X(60) X(64)
+#undef X
+    }
+}
+
+// Pool variant: the 256-thread Hessenberg, then k_qr_pool with layout `pl`
+// on one workgroup per CU.  The QR's counter sits in `work` after the packed
+// matrices (inside the reference's workspace size, which leaves at least
+// 16 (n^2 + n - packed_size(n)) bytes per matrix there).
+static hipError_t launch_pool(int pl, int n, int batch, const zd *A, zd *eig, int8_t *flag, zd *work, int itmax,
+                              int bal, hipStream_t s, int *stage, hipEvent_t mid)
+{
+    PoolCfg P;
+    if (!build_pool_cfg(n, g_pool_layouts[pl], P) || P.cap[0] < packed_size(n) + 1) return hipErrorInvalidValue;
+    int *qctr = (int *)(work + (long long)packed_size(n) * batch);
+    *stage = 1;
+    launch_hess_wg(n, batch, A, work, flag, bal, eig, qctr, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (mid) (void)hipEventRecord(mid, s);   // stage-timing marker: a failed record only spoils that reading
+    *stage = 2;
+    const dim3 grid(device_cus());
+    switch (g_pool_layouts[pl].waves) {
+#define X(w)                                                                                                     \
+    case w:                                                                                                      \
+        hipLaunchKernelGGL((k_qr_pool<w>), grid, dim3(64 * w), 0, s, n, batch, work, eig, flag, itmax, qctr, P); \
+        break;
+        X(8) X(12) X(16)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -193,7 +330,17 @@ __global__ __launch_bounds__(64) void k_qr_packed(int n, int batch, const zd *__
     X(25, DUAL, 60, 64)                 \
     X(26, SPLIT, 16, 16)                \
     X(27, SPLIT, 16, 8)                 \
-    X(28, SPLIT, 8, 4)
+    X(28, SPLIT, 8, 4)                  \
+    X(29, POOL, 64, 64)                 \
+    X(30, POOL, 64, 64)                 \
+    X(31, POOL, 64, 64)                 \
+    X(32, POOL, 64, 64)                 \
+    X(33, POOL, 64, 64)                 \
+    X(34, POOL, 64, 64)                 \
+    X(35, POOL, 64, 64)                 \
+    X(36, POOL, 64, 64)
+// pool variants: layout id - kFirstPool of TKZ_POOL_LAYOUTS
+constexpr int kFirstPool = 29;
 
 static const VariantInfo g_variants[] = {
 #define X(id, kind, nb, lpm) {id, VK_##kind, nb, lpm, 64 / lpm},
@@ -223,9 +370,10 @@ static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *f
     *stage = 1;
     if constexpr (NB >= 48 && !ONEWAVE)   // large matrices: one workgroup per matrix
         hipLaunchKernelGGL((k_hessenberg_packed_wg<NB, TKZ_HESS_NT>), dim3(batch), dim3(TKZ_HESS_NT), 0, s, n, batch,
-                           A, work, flag, bal);
+                           A, work, flag, bal, eig);
     else
-        hipLaunchKernelGGL((k_hessenberg_packed<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, A, work, flag, bal);
+        hipLaunchKernelGGL((k_hessenberg_packed<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, A, work, flag, bal,
+                           eig);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (mid) (void)hipEventRecord(mid, s);   // stage-timing marker: a failed record only spoils that reading
@@ -236,10 +384,12 @@ static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *f
 }
 
 template <int KIND, int NB, int LPM>
-static hipError_t launch_kind(int n, int batch, const zd *A, zd *eig, int8_t *flag, zd *work, int itmax, int bal,
-                              hipStream_t s, int *stage, hipEvent_t mid)
+static hipError_t launch_kind(int id, int n, int batch, const zd *A, zd *eig, int8_t *flag, zd *work, int itmax,
+                              int bal, hipStream_t s, int *stage, hipEvent_t mid)
 {
-    if constexpr (KIND == VK_FUSED)
+    if constexpr (KIND == VK_POOL)
+        return launch_pool(id - kFirstPool, n, batch, A, eig, flag, work, itmax, bal, s, stage, mid);
+    else if constexpr (KIND == VK_FUSED)
         return launch_fused<NB, LPM>(n, batch, A, eig, flag, itmax, bal, s);
     else
         return launch_split<NB, LPM, KIND == VK_DUAL || KIND == VK_DUAL_1W, KIND == VK_DUAL_1W>(
@@ -256,7 +406,7 @@ hipError_t launch_plain(int variant, int n, int batch, const void *A, void *eig,
     *stage = 1;
     switch (variant) {
 #define X(id, kind, nb, lpm) \
-    case id: return launch_kind<VK_##kind, nb, lpm>(n, batch, a, w, f, wk, itmax, bal, s, stage, mid);
+    case id: return launch_kind<VK_##kind, nb, lpm>(id, n, batch, a, w, f, wk, itmax, bal, s, stage, mid);
         TKZ_VARIANTS(X)
 #undef X
     default:

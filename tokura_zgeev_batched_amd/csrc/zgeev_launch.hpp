@@ -8,7 +8,10 @@ namespace tkz {
 // workgroup Hessenberg for NB >= 48); VK_DUAL_1W: the same QR after the
 // one-wavefront Hessenberg (the arithmetic of the Schur-form kernels, so its
 // eigenvalues come in their order)
-enum VariantKind { VK_FUSED = 0, VK_SPLIT = 1, VK_DUAL = 2, VK_DUAL_1W = 3 };
+// VK_POOL: the 256-thread Hessenberg, then the two-bulge QR on a persistent
+// workgroup per CU whose LDS is an arena of slots that matrices move down
+// through as they deflate (its eigenvalues are VK_DUAL's bit for bit)
+enum VariantKind { VK_FUSED = 0, VK_SPLIT = 1, VK_DUAL = 2, VK_DUAL_1W = 3, VK_POOL = 4 };
 
 // ZGEBAL job bits of tokuraSetBalancing ('P' = permute, 'S' = scale, 'B' = both)
 enum BalanceBits { BAL_PERMUTE = 1, BAL_SCALE = 2 };

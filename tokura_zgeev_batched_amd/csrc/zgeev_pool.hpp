@@ -1,0 +1,221 @@
+// An LDS arena shared by the wavefronts of one persistent workgroup per CU,
+// for the eigenvalue-only QR of matrices held one per wavefront.
+//
+// The QR of an n x n Hessenberg matrix needs packed_size(n) elements of LDS
+// at its start (34.3 KB at n = 64: four matrices per CU when every
+// wavefront has its own fixed buffer), but after each deflation only the
+// leading columns 0..i are ever read again, a prefix of the packed layout.
+// Averaged over the QR's time that is about half of the initial size.  The
+// arena is cut into slots of a few sizes (a "layout"); a wavefront takes a
+// slot that fits its work item, and when the active part of its matrix fits
+// a smaller free slot it copies the prefix there and frees the larger one
+// for the next matrix.  More matrices are then resident per CU, so the SIMDs
+// interleave more serial chase chains.  The arithmetic of the QR does not
+// change (the eigenvalues are bit-identical to the fixed-buffer kernels').
+//
+// Slots are handed out through a bit mask in LDS (one bit per slot, set =
+// free).  A wavefront never waits while holding a slot, so every wait ends
+// when a working wavefront finishes its matrix (the QR is bounded by its
+// sweep cap).
+//
+// Control flow: every lane of the wavefront executes each atomic, lane 0 with
+// the real operand and the others with the operation's identity (add 0, and
+// with all ones, or with 0), and every decision is taken on a value read
+// with readfirstlane, so no branch on the lane index sits between the
+// wave-wide reads of these loops.  (A branch "if (lane == 0) atomic" before
+// the loop's back edge let the compiler send lane 0 down that branch alone
+// while the other lanes started the next claim without it.)
+#pragma once
+#include "zgeev_device.hpp"
+
+namespace tkz {
+
+constexpr int kPoolMaxSlots = 16;
+// complex elements of the arena: 160 KB of LDS minus 256 bytes of control
+constexpr int kArena = (160 * 1024 - 256) / 16;
+
+// A slot layout, slots sorted by capacity (complex elements), largest first.
+struct PoolCfg {
+    int nslots;
+    int cap[kPoolMaxSlots];
+    int off[kPoolMaxSlots];
+};
+
+// Slots whose capacity is at least `need`: the leading bits, since the
+// capacities are sorted in decreasing order.
+__device__ __forceinline__ unsigned pool_fit(const PoolCfg &P, int need)
+{
+    int k = 0;
+    while (k < P.nslots && P.cap[k] >= need) ++k;
+    return (1u << k) - 1u;
+}
+
+// One attempt to take a free slot among `cand` (the smallest-capacity one);
+// returns the slot or -1.  Called by the whole wavefront (uniform result).
+__device__ __forceinline__ int pool_try(unsigned *free_mask, unsigned cand, int lane)
+{
+    unsigned m = __builtin_amdgcn_readfirstlane(__hip_atomic_load(free_mask, __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_WORKGROUP) & cand);
+    while (m) {
+        const int s = 31 - __builtin_clz(m);
+        const unsigned old = __builtin_amdgcn_readfirstlane(atomicAnd(free_mask, lane == 0 ? ~(1u << s) : ~0u));
+        if (old & (1u << s)) return s;
+        m = old & cand;   // taken meanwhile: the others that are still free
+    }
+    return -1;
+}
+
+// Waits for a free slot among `cand` (non-empty).
+__device__ __forceinline__ int pool_acquire(unsigned *free_mask, unsigned cand, int lane)
+{
+    for (;;) {
+        const int s = pool_try(free_mask, cand, lane);
+        if (s >= 0) return s;
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Frees slot s after every LDS access of the wavefront to it (the LDS
+// serves one wavefront's operations in order; the fence keeps the compiler
+// from moving accesses past the atomic).
+__device__ __forceinline__ void pool_release(unsigned *free_mask, int s, int lane)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    atomicOr(free_mask, lane == 0 ? 1u << s : 0u);
+}
+
+// zlahqr_dual's hook: after a deflation that leaves columns 0..i active,
+// move them to a smaller free slot when one fits (the idle lanes' spare
+// element included), writing the eigenvalues of the rows that became final
+// (i+1 .. whi-1) from the old slot first.
+struct PoolMover {
+    const PoolCfg *P;
+    unsigned *free_mask;
+    zd *arena;
+    zd *out;      // eigenvalues of the matrix (global)
+    int slot;     // current slot
+    int whi;      // rows whi.. of out are written
+    zd *H;        // current base
+    int lane;
+    double us;    // norm-range factor of the eigenvalues (1 when none)
+
+    __device__ __forceinline__ zd eig(zd h) const { return us != 1.0 ? zscale(us, h) : h; }
+
+    __device__ __forceinline__ void moved(zd *&Hq, int &idle, int i, int t)
+    {
+        const int ps = packed_size(i + 1);
+        const unsigned cand = pool_fit(*P, ps + 1) & ~pool_fit(*P, P->cap[slot]);
+        if (!cand) return;
+        const int s2 = pool_try(free_mask, cand, lane);
+        if (s2 < 0) return;
+        const PackedLayout pk;
+        if (i + 1 + t < whi) out[i + 1 + t] = eig(Hq[pk(i + 1 + t, i + 1 + t)]);   // whi <= n <= 64
+        whi = i + 1;
+        zd *H2 = arena + P->off[s2];
+        for (int e = t; e < ps; e += 64) H2[e] = Hq[e];
+        pool_release(free_mask, slot, lane);
+        lds_order();
+        slot = s2;
+        Hq = H2;
+        H = H2;
+        idle = P->cap[s2] - 1;
+    }
+};
+
+// The eigenvalue-only QR of one packed Hessenberg matrix (n x n at src, the
+// packed layout) by the calling wavefront, from slot `slot` of the arena
+// (capacity >= packed_size(n) + 1), moving to smaller slots as it deflates;
+// eigenvalues times us to out, returns the QR INFO.  The slot held at the end
+// is released.
+__device__ __forceinline__ int pool_qr_one(const PoolCfg &P, unsigned *free_mask, zd *arena, int slot,
+                                           const zd *__restrict__ src, int n, zd *__restrict__ out, double us,
+                                           int itmax, int lane)
+{
+    zd *H = arena + P.off[slot];
+    const int ps = packed_size(n);
+    for (int e = lane; e < ps; e += 64) H[e] = src[e];
+    lds_order();
+    PoolMover mv{&P, free_mask, arena, out, slot, n, H, lane, us};
+    int info = 0;
+    if (n > 1) info = zlahqr_dual<64, false, true, PoolMover>(H, PackedLayout{}, n, lane, itmax, nullptr, 0, &mv,
+                                                              P.cap[slot] - 1);
+    const PackedLayout pk;
+    if (lane < mv.whi) out[lane] = mv.eig(mv.H[pk(lane, lane)]);   // whi <= n <= 64
+    pool_release(free_mask, mv.slot, lane);
+    return info;
+}
+
+}  // namespace tkz
+
+namespace tkz {
+
+// Slot layouts for the same-size pool variants: wavefronts per workgroup and
+// the share of the arena given to slots of six tiers, for active sizes
+// ceil(f n) with f = 1, 7/8, 3/4, 5/8, 1/2, 1/4 (each slot one element
+// larger than the packed matrix, for the idle lanes).  A matrix spends about
+// the fraction (s_k^2 - s_{k+1}^2) / n^2 of its QR time with an active size
+// in (s_{k+1}, s_k] (a sweep costs steps in proportion to the active size),
+// and a new matrix needs a tier-0 slot, so the tier-0 count bounds the rate
+// at which matrices enter; the autotuner picks among these layouts.
+constexpr int kPoolTiers = 6;
+struct PoolLayout {
+    int waves;
+    float share[kPoolTiers];
+};
+static constexpr int kTierEighths[kPoolTiers] = {8, 7, 6, 5, 4, 2};
+#define TKZ_POOL_LAYOUTS(X)                                       \
+    X(0, 12, 0.63f, 0.00f, 0.24f, 0.00f, 0.11f, 0.015f)           \
+    X(1, 16, 0.63f, 0.00f, 0.24f, 0.00f, 0.11f, 0.015f)           \
+    X(2, 12, 0.63f, 0.17f, 0.12f, 0.00f, 0.055f, 0.03f)           \
+    X(3, 12, 0.63f, 0.17f, 0.00f, 0.09f, 0.055f, 0.03f)           \
+    X(4, 12, 0.84f, 0.00f, 0.00f, 0.00f, 0.11f, 0.03f)            \
+    X(5, 12, 0.42f, 0.33f, 0.12f, 0.00f, 0.11f, 0.015f)           \
+    X(6, 8, 0.63f, 0.00f, 0.24f, 0.00f, 0.11f, 0.015f)            \
+    X(7, 12, 0.63f, 0.00f, 0.12f, 0.09f, 0.11f, 0.03f)
+
+// Builds the slot table of a layout for matrices of size n; returns false
+// when it cannot hold one matrix.  Tiers that coincide for small n merge.
+inline bool build_pool_cfg(int n, const PoolLayout &L, PoolCfg &P)
+{
+    int used = 0;
+    P.nslots = 0;
+    int last_cap = -1;
+    for (int k = 0; k < kPoolTiers; ++k) {
+        const int tn = (kTierEighths[k] * n + 7) / 8;
+        const int cap = packed_size(tn) + 1;
+        int cnt = (int)(L.share[k] * (float)kArena) / cap;
+        if (k == 0 && cnt < 1) cnt = 1;
+        if (cap == last_cap) cnt = 0;   // the same size as the tier above
+        if (P.nslots + cnt > kPoolMaxSlots) cnt = kPoolMaxSlots - P.nslots;
+        if (used + cnt * cap > kArena) cnt = (kArena - used) / cap;
+        for (int c = 0; c < cnt; ++c) {
+            P.cap[P.nslots] = cap;
+            used += cap;
+            ++P.nslots;
+        }
+        if (cnt > 0) last_cap = cap;
+    }
+    // the shares round down: fill what is left with slots of the largest
+    // tier that still fits and that the layout uses (capacities stay sorted)
+    for (int k = 0; k < kPoolTiers && P.nslots < kPoolMaxSlots; ++k) {
+        if (L.share[k] <= 0.f) continue;
+        const int cap = packed_size((kTierEighths[k] * n + 7) / 8) + 1;
+        if (used + cap > kArena) continue;
+        int pos = 0;   // insert after the slots of capacity >= cap
+        while (pos < P.nslots && P.cap[pos] >= cap) ++pos;
+        for (int s = P.nslots; s > pos; --s) P.cap[s] = P.cap[s - 1];
+        P.cap[pos] = cap;
+        ++P.nslots;
+        used += cap;
+        --k;   // the same tier may take another slot
+    }
+    used = 0;
+    for (int s = 0; s < P.nslots; ++s) {
+        P.off[s] = used;
+        used += P.cap[s];
+    }
+    for (int s = P.nslots; s < kPoolMaxSlots; ++s) P.cap[s] = P.off[s] = 0;
+    return P.nslots > 0 && P.cap[0] == packed_size(n) + 1 && used <= kArena;
+}
+
+}  // namespace tkz
