@@ -25,6 +25,8 @@
  *   2  non-finite entry in the input matrix (eigenvalues are NaN)
  *   3  (recompute backward only) the forward's eigenvalues are not those of
  *      the matrix, so no cotangent can be paired (dA is NaN)
+ *   4  (ragged entries only) `work` does not hold the schedule prepared for
+ *      the call (nothing computed)
  *
  * One handle must not be used by two host threads at once (the reference
  * keeps per-call state in the handle, tokura_zgeev_batched.cu:224-235).
@@ -131,13 +133,20 @@ int tokuraSetVariant(tokurablas_t *handle, int n, int variant);
  * Hessenberg offsets) and copies it into `work` with an asynchronous copy on
  * `stream`.  The schedule stays valid for any number of solves of batches
  * with the same offsets; it uses the handle's current class configuration.
- * tokura_zgeev_ragged_gpu launches the Hessenberg pass and one persistent QR
- * kernel whose wavefronts pull tiles from an atomic queue;
- * tokura_zgeev_ragged_graph does the same through a hipGraph that the handle
- * captures on first use and replays while pointers, batch, class
- * configuration and sweep cap are unchanged.
+ * tokura_zgeev_ragged_gpu launches, on `stream`: a check of the schedule
+ * (one fill kernel), the Hessenberg pass (one launch per LDS size: five for
+ * n > 32, one for n <= 32), and ONE persistent QR kernel for the whole batch
+ * (a workgroup per CU whose LDS is shared by matrices of every size; its
+ * wavefronts pull tiles, largest first, from one atomic queue, and a matrix
+ * moves to a smaller LDS region as it deflates); tokura_zgeev_ragged_graph
+ * does the same through a hipGraph that the handle captures on first use and
+ * replays while pointers, batch, class configuration, sweep cap and
+ * schedule are unchanged.
  * Return codes as tokura_zgeev_batched_gpu, plus -8 when `work` has not been
- * prepared for this batch_count. */
+ * prepared for this batch_count on this handle.  When `work` no longer holds
+ * the schedule that prepare wrote (the buffer was freed and reallocated at
+ * the same address, or written over), every matrix gets flag 4 and nothing
+ * is computed. */
 size_t tokura_get_zgeev_ragged_worksize(int batch_count, const int64_t *offsets_host);
 int tokura_zgeev_ragged_prepare(tokurablas_t *handle, int batch_count, const int64_t *offsets_host,
                                 void *work, size_t work_bytes, hipStream_t stream);
@@ -149,13 +158,16 @@ int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, const hipDo
                               hipStream_t stream);
 
 /* Class configuration of the ragged path: lanes per matrix of class A1
- * (n 49..64), A2 (33..48), B (17..32), C (1..16); a wavefront holds
- * 64/lanes matrices of a class.  tokuraAutotuneRagged times every configuration on the caller's
- * prepared batch (buffers as for tokura_zgeev_ragged_gpu; the stream is
- * synchronised), keeps the fastest and re-prepares `work` for it. */
+ * (n 49..64), A2 (33..48), B (17..32), C (1..16) (a wavefront holds
+ * 64/lanes matrices of a class) and wavefronts per workgroup of the
+ * persistent QR (tokuraGetRaggedWaves).  tokuraAutotuneRagged times every
+ * configuration on the caller's prepared batch (buffers as for
+ * tokura_zgeev_ragged_gpu; the stream is synchronised), keeps the fastest
+ * and re-prepares `work` for it. */
 int tokuraSetRaggedConfig(tokurablas_t *handle, int config);
 int tokuraGetRaggedConfig(tokurablas_t *handle, int *config, int *lanes_a1, int *lanes_a2, int *lanes_b,
                           int *lanes_c);
+int tokuraGetRaggedWaves(tokurablas_t *handle, int *waves);
 int tokuraAutotuneRagged(tokurablas_t *handle, int batch_count, const int64_t *offsets_host,
                          const hipDoubleComplex *A, hipDoubleComplex *eigenvalues, void *work,
                          size_t work_bytes, char *flag, hipStream_t stream);

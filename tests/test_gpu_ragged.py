@@ -35,7 +35,7 @@ def check_against_lapack(mats, eig_flat, eoff, tol=1e-10):
     return worst
 
 
-@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+@pytest.mark.parametrize("cfg", list(range(13)))
 def test_ragged_every_config(cfg):
     rng = np.random.default_rng(100 + cfg)
     sizes = rng.integers(1, 65, 700)
@@ -134,7 +134,7 @@ def test_ragged_autotune_and_bad_offsets(handle):
     rb = RaggedBatch(offs, handle=handle)
     A = torch.tensor(flat, device="cuda")
     cfg, lanes = rb.autotune(A)
-    assert 0 <= cfg < 4
+    assert 0 <= cfg < 13
     eig, flag = rb.solve(A)
     check_against_lapack(mats, eig.cpu().numpy(), rb.eig_offsets)
     bad = offs.copy()
@@ -147,3 +147,61 @@ def test_ragged_autotune_and_bad_offsets(handle):
     # a work buffer that was never prepared
     assert L.tokura_zgeev_ragged_gpu(handle.ptr, 3, P(A.data_ptr()), P(w.data_ptr()), P(w.data_ptr()),
                                      P(w.data_ptr()), P(0)) == -8
+
+
+def test_ragged_stale_schedule_flags_4():
+    """A work buffer whose schedule was written over since prepare (as a freed
+    buffer reallocated at the same address would be): no matrix is computed,
+    every flag is 4, for the plain and the graph entry; a new prepare
+    restores it."""
+    rng = np.random.default_rng(11)
+    sizes = rng.integers(1, 65, 300)
+    mats = rand_mats(rng, sizes)
+    flat, offs = pack(mats)
+    h = tk.Handle()
+    rb = RaggedBatch(offs, handle=h)
+    A = torch.tensor(flat, device="cuda")
+    e0, f0 = rb.solve(A, graph=True)
+    e0 = e0.clone()
+    torch.cuda.synchronize()
+    assert int(f0.abs().sum()) == 0
+    rb.work[:4096].zero_()
+    for graph in (False, True):
+        eig, flag = rb.solve(A, graph=graph)
+        torch.cuda.synchronize()
+        assert (flag.cpu().numpy() == 4).all(), graph
+    rb.prepare()
+    eig, flag = rb.solve(A, graph=True)
+    torch.cuda.synchronize()
+    assert int(flag.abs().sum()) == 0 and torch.equal(eig, e0)
+    h.close()
+
+
+def test_ragged_reprepare_other_offsets_same_buffer():
+    """Preparing the same work buffer again for other offsets (same batch)
+    replaces the schedule; the cached graph is captured again."""
+    rng = np.random.default_rng(13)
+    h = tk.Handle()
+    sizes1 = rng.integers(1, 65, 200)
+    mats1 = rand_mats(rng, sizes1)
+    flat1, offs1 = pack(mats1)
+    rb = RaggedBatch(offs1, handle=h)
+    e1, f1 = rb.solve(torch.tensor(flat1, device="cuda"), graph=True)
+    check_against_lapack(mats1, e1.cpu().numpy(), rb.eig_offsets)
+    sizes2 = np.sort(sizes1)            # same batch, other sizes per position
+    mats2 = rand_mats(rng, sizes2)
+    flat2, offs2 = pack(mats2)
+    rb2 = RaggedBatch.__new__(RaggedBatch)
+    rb2.__dict__.update(rb.__dict__)
+    rb2.offsets = offs2
+    rb2.sizes = sizes2
+    rb2.eig_offsets = np.concatenate([[0], np.cumsum(sizes2)]).astype(np.int64)
+    ws = int(tk.lib().tokura_get_zgeev_ragged_worksize(rb2.batch, offs2.ctypes.data_as(ctypes.c_void_p)))
+    if ws > rb.work.numel():
+        rb2.work = torch.empty(ws, dtype=torch.uint8, device="cuda")
+    rb2.prepare()
+    e2, f2 = rb2.solve(torch.tensor(flat2, device="cuda"), graph=True)
+    torch.cuda.synchronize()
+    assert int(f2.abs().sum()) == 0
+    check_against_lapack(mats2, e2.cpu().numpy(), rb2.eig_offsets)
+    h.close()

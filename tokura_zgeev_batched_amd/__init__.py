@@ -10,7 +10,8 @@ passes torch device pointers and the current HIP stream through ctypes:
 * ``eigvals``          convenience: (B, n, n) tensor -> (B, n) eigenvalues + flags
 * ``zgeev_vectors`` / ``eig``  eigenvalues with right and/or left eigenvectors
                        (ZGEEV with JOBVL/JOBVR = 'V', no balancing)
-* ``RaggedBatch``      mixed sizes through one persistent kernel (+ hipGraph)
+* ``RaggedBatch``      mixed sizes: Hessenberg launches per LDS size, then one
+                       persistent QR kernel for the whole batch (+ hipGraph)
 * ``eigvals_autograd`` differentiable eigenvalues (``torch.autograd.Function``
                        whose backward is the library's adjoint kernel)
 
@@ -55,6 +56,7 @@ _SIGNATURES = {
     "tokura_get_zgeev_ragged_worksize": ([_I, _P], _SZ),
     "tokuraSetRaggedConfig": ([_P, _I], _I),
     "tokuraGetRaggedConfig": ([_P] + [ctypes.POINTER(_I)] * 5, _I),
+    "tokuraGetRaggedWaves": ([_P, ctypes.POINTER(_I)], _I),
     "tokuraAutotuneRagged": ([_P, _I, _P, _P, _P, _P, _SZ, _P, _P], _I),
     "tokura_zgeev_ragged_prepare": ([_P, _I, _P, _P, _SZ, _P], _I),
     "tokura_zgeev_ragged_gpu": ([_P, _I, _P, _P, _P, _P, _P], _I),

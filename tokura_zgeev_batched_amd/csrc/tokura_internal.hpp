@@ -18,12 +18,14 @@ struct RaggedGraphCache {
     int cfg = -1;
     int max_sweeps = -1;
     int balance = 0;
+    int gen = 0;
 };
 
 struct RaggedPrepared {
     void *work;
     int cfg;
     int batch;
+    int gen;      // the prepare that wrote the schedule (also in its header)
 };
 
 struct tokurablas_t {
@@ -38,7 +40,9 @@ struct tokurablas_t {
     RaggedGraphCache graph;
     void *ragged_host_stage = nullptr;                        // pinned staging of the schedule
     size_t ragged_host_bytes = 0;
-    hipStream_t ragged_stage_stream = nullptr;
+    hipEvent_t ragged_stage_event = nullptr;                  // after the last staging copy
+    bool ragged_staged = false;                               // that copy may still be in flight
+    int ragged_gen = 0;                                       // prepares so far
     std::vector<RaggedPrepared> prepared;
     // stage timing: events (start, between stages, end) per timed call
     bool stage_timing = false;

@@ -1,25 +1,29 @@
 // Ragged batches: n varies per matrix (1..64).  No counterpart in the
 // reference, which requires one n per call (README.md:5, tokura_blas.h:12).
 //
-// Pipeline (all launches are captured together by tokura_zgeev_ragged_graph):
-//  1. k_ragged_reset    zeroes the queue heads (a fill kernel, graph-safe).
-//  2. Householder Hessenberg of every matrix, packed result written to `work`:
-//     k_ragged_hess_wg  one 256-thread workgroup per matrix for n > 32,
-//     k_ragged_hess     one wavefront per tile of small matrices (n <= 32).
-//  3. k_ragged_qr       one persistent launch per size class, largest first,
-//                       each with the LDS of its class only: every wavefront
-//                       pulls the class's next tile from an atomic queue until
-//                       the queue is empty, so matrices that need more QR
-//                       sweeps never hold up a whole launch; tiles are queued
-//                       in decreasing size (longest-processing-time first).
-// A tile is the work of one wavefront: 64/LPM matrices of one size class.
-// The classes are n 61..64, 57..60, 49..56, 41..48, 33..40 (one matrix per
-// wavefront, the two-bulge QR: two shifts per sweep, one chain per half-wave,
-// each class with the LDS of its largest n, so that smaller matrices keep more
-// wavefronts resident), B (17..32) and C (1..16).  The lanes-per-matrix of B
-// and C form a "class configuration" that the autotuner picks: smaller
-// classes put more matrices on a wavefront, so the serial reflector chain of
-// every chase step serves several matrices at once.
+// Launches of one eigenvalue call (tokura_zgeev_ragged_graph captures them
+// together):
+//  1. k_ragged_reset     checks that `work` holds this handle's schedule for
+//                        this batch and zeroes the queue head (a fill kernel,
+//                        graph-safe).
+//  2. Householder Hessenberg of every matrix, packed result to `work`, one
+//     launch per LDS size:
+//     k_ragged_hess_wg   one 256-thread workgroup per matrix, n > 32 (five
+//                        launches, LDS for n <= 64, 60, 56, 48, 40),
+//     k_ragged_hess      one wavefront per tile of small matrices (n <= 32).
+//  3. k_ragged_qr_pool   ONE persistent QR kernel for the whole batch: a
+//                        workgroup of W wavefronts per CU whose LDS is one
+//                        arena of 32-element chunks (ChunkPool).  Each
+//                        wavefront pulls the next tile from one atomic queue
+//                        (tiles in decreasing size, longest first), waits for
+//                        a run of chunks that holds it and runs its QR; a
+//                        matrix of n > 32 (one per wavefront, two bulges per
+//                        sweep) moves to a smaller run as it deflates, so
+//                        matrices of all sizes share each CU's LDS and no
+//                        size class waits for another to drain.
+// A tile is the work of one wavefront: one matrix of n > 32, or 64/LPM
+// matrices of a small class, B (17..32) or C (1..16), whose lanes per matrix
+// form, with W, the "class configuration" that the autotuner picks.
 //
 // `work` layout (built on the host by tokura_zgeev_ragged_prepare):
 //   RaggedHeader | tiles int4[ntiles] | order int[B] | n int[B] |
@@ -31,6 +35,7 @@
 #include <vector>
 
 #include "tokura_internal.hpp"
+#include "zgeev_pool.hpp"
 #include "zgeev_schur.hpp"
 
 namespace tkz {
@@ -47,31 +52,47 @@ __host__ __device__ constexpr int cls_of(int n)
     return n > 60 ? 0 : (n > 56 ? 1 : (n > 48 ? 2 : (n > 40 ? 3 : (n > 32 ? 4 : (n > 16 ? 5 : 6)))));
 }
 
+constexpr int kRaggedMagic = 0x52474b54;   // "TKGR"
+
 struct RaggedHeader {
-    int qctr[8];       // QR queue heads, one per size class
+    int qctr[8];       // queue heads: [0] the QR of the eigenvalue path, one per size class for the Schur QR
     int cls_end[8];    // class c's matrices: order[cls_end[c-1] .. cls_end[c]) (order[0 ..) for c = 0)
     int tile_cls[8];   // class c's tiles: [tile_cls[c], tile_cls[c+1]); tile_cls[kNumCls] = ntiles
     int ntiles;
     int batch;
     int cfg;
     int tile_small0;   // first tile of classes B/C
+    int magic;         // kRaggedMagic
+    int gen;           // the handle's prepare count when this schedule was written
     long long off_tiles, off_order, off_n, off_moff, off_eoff, off_hoff, off_H;
     long long total_bytes;
 };
-static_assert(sizeof(RaggedHeader) == 176, "header size");
+static_assert(sizeof(RaggedHeader) == 184, "header size");
 
-// lanes per matrix of the classes n > 48, 33..48, B (17..32), C (1..16)
-#define TKZ_RAGGED_CFGS(X) \
-    X(0, 64, 64, 16, 8)    \
-    X(1, 64, 64, 32, 8)    \
-    X(2, 64, 64, 32, 16)   \
-    X(3, 64, 64, 16, 16)
-static const int g_cfg_lpm[][4] = {
-#define X(id, a1, a2, b, c) {a1, a2, b, c},
+// class configurations: lanes per matrix of the classes n > 48, 33..48 (one
+// matrix per wavefront), B (17..32), C (1..16); wavefronts per workgroup of
+// the persistent QR; a deflating matrix moves to a smaller LDS run when its
+// active part needs at most mv/8 of its run
+#define TKZ_RAGGED_CFGS(X)      \
+    X(0, 64, 64, 16, 8, 12, 5)  \
+    X(1, 64, 64, 32, 8, 12, 5)  \
+    X(2, 64, 64, 32, 16, 12, 5) \
+    X(3, 64, 64, 16, 16, 12, 5) \
+    X(4, 64, 64, 32, 8, 16, 5)  \
+    X(5, 64, 64, 32, 8, 8, 5)   \
+    X(6, 64, 64, 16, 8, 16, 5)  \
+    X(7, 64, 64, 32, 16, 16, 5) \
+    X(8, 64, 64, 32, 32, 12, 5) \
+    X(9, 64, 64, 32, 32, 8, 5)  \
+    X(10, 64, 64, 32, 32, 12, 6) \
+    X(11, 64, 64, 32, 8, 8, 6)  \
+    X(12, 64, 64, 32, 32, 12, 4)
+static const int g_cfg_lpm[][6] = {
+#define X(id, a1, a2, b, c, w, mv) {a1, a2, b, c, w, mv},
     TKZ_RAGGED_CFGS(X)
 #undef X
 };
-static constexpr int kNumRaggedCfg = 4;
+static constexpr int kNumRaggedCfg = 13;
 
 static int cfg_lpm(int cfg, int cls) { return cls < kClsB ? 64 : g_cfg_lpm[cfg][cls == kClsB ? 2 : 3]; }
 
@@ -94,10 +115,34 @@ struct RaggedView {
     }
 };
 
-__global__ void k_ragged_reset(char *work)
+// Zeroes the queue heads when `work` holds the schedule that this handle's
+// prepare wrote for this batch (magic, generation and batch agree).
+// Otherwise (a buffer freed and reallocated at the same address, or written
+// over since) every matrix gets flag 4 and the header is emptied, so that no
+// kernel of the call finds work in it.
+__global__ void k_ragged_reset(char *work, int gen, int batch, int8_t *flag)
 {
     RaggedHeader *h = (RaggedHeader *)work;
-    for (int c = 0; c < 8; ++c) h->qctr[c] = 0;
+    const bool valid = h->magic == kRaggedMagic && h->gen == gen && h->batch == batch;
+    if (valid) {
+        if (blockIdx.x == 0 && threadIdx.x < 8) h->qctr[threadIdx.x] = 0;
+        return;
+    }
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < batch; b += gridDim.x * blockDim.x) flag[b] = 4;
+    if (blockIdx.x == 0 && threadIdx.x < 8) {
+        h->cls_end[threadIdx.x] = 0;
+        h->tile_cls[threadIdx.x] = 0;
+        if (threadIdx.x == 0) {
+            h->ntiles = 0;
+            h->tile_small0 = 0;
+        }
+    }
+}
+
+static hipError_t launch_reset(char *work, int gen, int batch, int8_t *flag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ragged_reset, dim3((batch + 255) / 256), dim3(256), 0, s, work, gen, batch, flag);
+    return hipGetLastError();
 }
 
 // matrices of size class c: order[lo .. hi)
@@ -135,36 +180,155 @@ __device__ __forceinline__ void hess_tile(zd *lds, const RaggedView &V, int4 td,
     if (t == 0) flag[b] = ok ? 0 : 2;
 }
 
+// The QR of the small matrices of one tile (class B or C: 64/LPM matrices
+// of n <= NB per wavefront, LPM lanes each) from the LDS run at `base`,
+// `stride` elements per matrix; eigenvalues times the Hessenberg's
+// norm-range factor.
 template <int NB, int LPM>
-__device__ __forceinline__ void qr_tile(zd *lds, const RaggedView &V, int4 td, zd *__restrict__ eig,
-                                        int8_t *__restrict__ flag, int itmax)
+__device__ __forceinline__ void small_tile(zd *base, int stride, const RaggedView &V, int4 td,
+                                           zd *__restrict__ eig, int8_t *__restrict__ flag, int itmax)
 {
-    constexpr int PS = packed_size(NB), S = (NB + LPM - 1) / LPM;
+    constexpr int S = (NB + LPM - 1) / LPM;
     const int lane = threadIdx.x & 63, g = lane / LPM, t = lane % LPM;
-    if (g >= td.y) return;
-    const int b = V.order[td.z + g];
-    const int n = V.nsz[b];
-    zd *out = eig + V.eoff[b];
-    if (flag[b] == 2) {
-        for (int r = t; r < n; r += LPM) out[r] = mk(__builtin_nan(""), __builtin_nan(""));
-        return;
+    if (g < td.y) {
+        const int b = V.order[td.z + g];
+        const int n = V.nsz[b];
+        zd *out = eig + V.eoff[b];
+        if (flag[b] == 2) {
+            for (int r = t; r < n; r += LPM) out[r] = mk(__builtin_nan(""), __builtin_nan(""));
+        } else {
+            zd *H = base + g * stride;
+            const zd *src = V.H + V.hoff[b];
+            const int ps = packed_size(n);
+            const double us = out[0].re;   // the Hessenberg's norm-range factor
+            for (int e = t; e < ps; e += LPM) H[e] = src[e];
+            lds_order();
+            const PackedLayout pk;
+            int info = 0;
+            if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
+            for (int r = t; r < n; r += LPM) out[r] = us != 1.0 ? zscale(us, H[pk(r, r)]) : H[pk(r, r)];
+            if (t == 0) flag[b] = info != 0 ? 1 : 0;
+        }
     }
-    zd *H = lds + g * PS;
-    const zd *src = V.H + V.hoff[b];
-    const int ps = packed_size(n);
-    const double us = out[0].re;   // the Hessenberg's norm-range factor
-    for (int e = t; e < ps; e += LPM) H[e] = src[e];
     lds_order();
-    const PackedLayout pk;
-    int info = 0;
-    if constexpr (LPM == 64) {   // one matrix per wavefront: two bulges per sweep
-        if (n > 1) info = zlahqr_dual<64, false, true>(H, pk, n, t, itmax);
-    } else {
-        if (n > 1) info = zlahqr_eigs<LPM, S>(H, pk, n, t, itmax);
+}
+
+// Both ends of the tile queue in one 64-bit counter (qctr[0..1]): the low
+// half counts tiles taken from the front (largest first), the high half
+// tiles taken from the back (smallest first).  A claim adds 1 to its half
+// and is valid while front + back < ntiles, so no tile is taken twice.
+__device__ __forceinline__ unsigned long long *queue_ends(const RaggedView &V)
+{
+    return (unsigned long long *)&V.h->qctr[0];
+}
+__device__ __forceinline__ void claim(const RaggedView &V, bool back, int lane, int nt, int &tile)
+{
+    const unsigned long long inc = lane == 0 ? (back ? 1ull << 32 : 1ull) : 0ull;
+    const unsigned long long o = atomicAdd(queue_ends(V), inc);
+    const unsigned f = __builtin_amdgcn_readfirstlane((unsigned)o);
+    const unsigned b = __builtin_amdgcn_readfirstlane((unsigned)(o >> 32));
+    tile = (long long)f + (long long)b < (long long)nt ? (back ? nt - 1 - (int)b : (int)f) : -1;
+}
+
+// The QR of every tile of the batch in one persistent launch: one workgroup
+// of W wavefronts per CU whose LDS is a chunked arena (ChunkPool).  A
+// wavefront takes the next tile from the front of the queue (tiles in
+// decreasing size), and the lowest run of chunks that holds it; while none
+// is free it takes tiles from the back of the queue (the smallest) and runs
+// those in whatever room there is, so the small matrices fill the gaps that
+// the large ones leave.  A tile is one matrix of n > 32 (the two-bulge
+// sweeps, moving up to a smaller run as the matrix deflates: ChunkMover) or
+// several small matrices.  A wavefront waits only while it holds no
+// chunks, so every wait ends; every wavefront leaves when the two ends of
+// the queue meet.
+template <int W, int LB, int LC, int MV>
+__global__ __launch_bounds__(64 * W) void k_ragged_qr_pool(char *work, zd *__restrict__ eig,
+                                                           int8_t *__restrict__ flag, int itmax)
+{
+    __shared__ zd arena[kChunks * kChunk];
+    __shared__ unsigned long long bits[kBitWords];
+    __shared__ unsigned lock;
+    const RaggedView V(work);
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x < kBitWords) bits[threadIdx.x] = run_mask((int)threadIdx.x, 0, kChunks);
+    if (threadIdx.x == 0) lock = 0u;
+    __syncthreads();
+    const ChunkPool P{bits, &lock, arena, lane};
+    const int nt = V.h->ntiles;
+    // tile t: its descriptor (uniform) and the chunks its QR needs
+    auto desc = [&](int t) {
+        const int4 tv = V.tiles[t];
+        return make_int4(__builtin_amdgcn_readfirstlane(tv.x), __builtin_amdgcn_readfirstlane(tv.y),
+                         __builtin_amdgcn_readfirstlane(tv.z), __builtin_amdgcn_readfirstlane(tv.w));
+    };
+    auto need = [&](int4 td) {
+        return td.x < kClsB ? chunks_for(packed_size(td.w) + 1) : chunks_for(td.y * packed_size(td.w));
+    };
+    // the QR of tile td from chunk run s0 (k chunks), which it frees
+    auto run = [&](int4 td, int s0, int k) {
+        if (td.x < kClsB) {   // one matrix of n > 32
+            const int b = __builtin_amdgcn_readfirstlane(V.order[td.z]);
+            const int n = __builtin_amdgcn_readfirstlane(V.nsz[b]);
+            zd *out = eig + V.eoff[b];
+            if (__builtin_amdgcn_readfirstlane(flag[b]) == 2) {
+                if (lane < n) out[lane] = mk(__builtin_nan(""), __builtin_nan(""));
+                P.give(s0, k);
+                return;
+            }
+            const double us = out[0].re;   // the Hessenberg's norm-range factor
+            zd *H = arena + s0 * kChunk;
+            const zd *src = V.H + V.hoff[b];
+            for (int e = lane; e < packed_size(n); e += 64) H[e] = src[e];
+            lds_order();
+            ChunkMover mv{&P, out, s0, k, n, H, us, MV};
+            int info = 0;
+            if (n > 1)
+                info = zlahqr_dual<64, false, true, ChunkMover>(H, PackedLayout{}, n, lane, itmax, nullptr, 0, &mv,
+                                                                k * kChunk - 1);
+            const PackedLayout pk;
+            if (lane < mv.whi) out[lane] = mv.eig(mv.H[pk(lane, lane)]);   // whi <= n <= 64
+            P.give(mv.s, mv.k);
+            flag[b] = info != 0 ? 1 : 0;   // every lane stores the same byte
+        } else {              // a tile of small matrices
+            const int stride = packed_size(td.w);
+            if (td.x == kClsB) small_tile<32, LB>(arena + s0 * kChunk, stride, V, td, eig, flag, itmax);
+            else small_tile<16, LC>(arena + s0 * kChunk, stride, V, td, eig, flag, itmax);
+            P.give(s0, k);
+        }
+    };
+    // pend: the front tile this wavefront has taken but not yet placed
+    int pend = -1, pk = 0;
+    int4 pd = make_int4(0, 0, 0, 0);
+    bool back_open = true;
+    for (;;) {
+        if (pend < 0) {
+            claim(V, false, lane, nt, pend);
+            if (pend < 0) break;   // the two ends met: no tile is left anywhere
+            pd = desc(pend);
+            pk = need(pd);
+        }
+        int4 td;
+        int k, s0 = P.try_alloc(pk, false);
+        if (s0 >= 0) {
+            td = pd;
+            k = pk;
+            pend = -1;
+        } else {
+            int bt = -1;
+            if (back_open) {
+                claim(V, true, lane, nt, bt);
+                back_open = bt >= 0;
+            }
+            if (bt < 0) {
+                __builtin_amdgcn_s_sleep(2);
+                continue;
+            }
+            td = desc(bt);
+            k = need(td);
+            s0 = P.alloc_wait(k);
+        }
+        run(td, s0, k);   // the only call: one copy of the QR code
     }
-    for (int r = t; r < n; r += LPM) out[r] = us != 1.0 ? zscale(us, H[pk(r, r)]) : H[pk(r, r)];
-    if (t == 0) flag[b] = info != 0 ? 1 : 0;
-    lds_order();
 }
 
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
@@ -238,34 +402,15 @@ __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A
     }
 }
 
-// QR of one size class: a persistent kernel whose LDS is sized for that
-// class alone (so the small classes run more wavefronts per CU than the
-// n = 64 class); every wavefront pulls the class's next tile from its queue
-// until the queue is empty.
-template <int CLS, int NB, int LPM>
-__global__ __launch_bounds__(64) void k_ragged_qr(char *work, zd *__restrict__ eig, int8_t *__restrict__ flag, int itmax)
-{
-    __shared__ zd lds[(64 / LPM) * packed_size(NB) + (LPM == 64)];   // + the idle-lane element
-    const RaggedView V(work);
-    const int t0 = V.h->tile_cls[CLS], t1 = V.h->tile_cls[CLS + 1];
-    const int lane = threadIdx.x & 63;
-    for (;;) {
-        const int tile = t0 + __builtin_amdgcn_readfirstlane(atomicAdd(&V.h->qctr[CLS], lane == 0 ? 1 : 0));
-        if (tile >= t1) break;
-        qr_tile<NB, LPM>(lds, V, V.tiles[tile], eig, flag, itmax);
-    }
-}
-
 // The classes n > 32: (class, NB)
 #define TKZ_BIG_CLS(X) X(0, 64) X(1, 60) X(2, 56) X(3, 48) X(4, 40)
 
-template <int LA1, int LA2, int LB, int LC>
+template <int LA1, int LA2, int LB, int LC, int W, int MV>
 static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int bal, int cus,
-                                    hipStream_t s)
+                                    int gen, int batch, hipStream_t s)
 {
     static_assert(LA1 == 64 && LA2 == 64, "classes n > 32: one matrix per wavefront");
-    hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(1), 0, s, work);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_reset(work, gen, batch, flag, s);
     if (e != hipSuccess) return e;
     // resident workgroups per CU, queried once (outside any graph capture the
     // first call is made from tokura_zgeev_ragged_graph's capture stream, where
@@ -283,27 +428,17 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag, bal, eig);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // largest class first: the tail of one launch is at most one tile of it
-#define X(c, nb)                                                                                          \
-    {                                                                                                      \
-        static const int occ = resident_blocks(k_ragged_qr<c, nb, 64>);                                    \
-        hipLaunchKernelGGL((k_ragged_qr<c, nb, 64>), dim3(cus * occ), dim3(64), 0, s, work, eig, flag, itmax); \
-    }
-    TKZ_BIG_CLS(X)
-#undef X
-    static const int occ_q5 = resident_blocks(k_ragged_qr<kClsB, 32, LB>);
-    static const int occ_q6 = resident_blocks(k_ragged_qr<kClsC, 16, LC>);
-    hipLaunchKernelGGL((k_ragged_qr<kClsB, 32, LB>), dim3(cus * occ_q5), dim3(64), 0, s, work, eig, flag, itmax);
-    hipLaunchKernelGGL((k_ragged_qr<kClsC, 16, LC>), dim3(cus * occ_q6), dim3(64), 0, s, work, eig, flag, itmax);
+    // one workgroup per CU: the arena is the CU's whole LDS
+    hipLaunchKernelGGL((k_ragged_qr_pool<W, LB, LC, MV>), dim3(cus), dim3(64 * W), 0, s, work, eig, flag, itmax);
     return hipGetLastError();
 }
 
 static hipError_t launch_ragged(int cfg, const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int bal,
-                                int cus, hipStream_t s)
+                                int cus, int gen, int batch, hipStream_t s)
 {
     switch (cfg) {
-#define X(id, a1, a2, b, c) \
-    case id: return launch_ragged_cfg<a1, a2, b, c>(A, work, eig, flag, itmax, bal, cus, s);
+#define X(id, a1, a2, b, c, w, mv) \
+    case id: return launch_ragged_cfg<a1, a2, b, c, w, mv>(A, work, eig, flag, itmax, bal, cus, gen, batch, s);
         TKZ_RAGGED_CFGS(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -482,9 +617,10 @@ static void launch_ragged_schur_cls(const zd *A, char *work, zd *eig, zd *schur,
 }
 
 static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schur, int8_t *flag, int itmax, int cus,
-                                      hipStream_t s)
+                                      int gen, int batch, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ragged_reset, dim3(1), dim3(1), 0, s, work);
+    hipError_t e = launch_reset(work, gen, batch, flag, s);
+    if (e != hipSuccess) return e;
     launch_ragged_schur_cls<64, 64, 0>(A, work, eig, schur, flag, itmax, cus, s);
     launch_ragged_schur_cls<60, 64, 1>(A, work, eig, schur, flag, itmax, cus, s);
     launch_ragged_schur_cls<56, 64, 2>(A, work, eig, schur, flag, itmax, cus, s);
@@ -496,8 +632,11 @@ static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schu
 }
 
 static hipError_t launch_ragged_adjoint(const zd *schur, const zd *gbar, zd *dA, char *work, int8_t *flag, int itmax,
-                                        int cus, hipStream_t s)
+                                        int cus, int gen, int batch, hipStream_t s)
 {
+    // the schedule check (a stale `work` gives flag 4 and no work)
+    hipError_t e = launch_reset(work, gen, batch, flag, s);
+    if (e != hipSuccess) return e;
     // n > 16 by 256-thread workgroups; the LDS is sized in steps where a
     // smaller footprint keeps more workgroups resident (n <= 48: two per CU,
     // n <= 40: three, n <= 32: four; the classes above 48 share the n = 64 size)
@@ -605,6 +744,8 @@ void ragged_release(tokurablas_t *h)
     if (h->ragged_host_stage) (void)hipHostFree(h->ragged_host_stage);
     h->ragged_host_stage = nullptr;
     h->ragged_host_bytes = 0;
+    if (h->ragged_stage_event) (void)hipEventDestroy(h->ragged_stage_event);
+    h->ragged_stage_event = nullptr;
     h->prepared.clear();
 }
 
@@ -637,8 +778,17 @@ extern "C" int tokura_zgeev_ragged_prepare(tokurablas_t *handle, int batch_count
     if (build_schedule(batch_count, offsets_host, cfg, S) != 0) return -3;
     if ((long long)work_bytes < S.hdr.total_bytes) return -5;
     const size_t meta = (size_t)S.hdr.off_H;
-    // the staging buffer may still feed an earlier copy: wait for it
-    if (handle->ragged_host_stage && hipStreamSynchronize(handle->ragged_stage_stream) != hipSuccess) return 3;
+    // the staging buffer may still feed an earlier copy: wait for that copy
+    // (an event of the handle, recorded after it on the caller's stream)
+    if (handle->ragged_stage_event == nullptr &&
+        hipEventCreateWithFlags(&handle->ragged_stage_event, hipEventDisableTiming) != hipSuccess) {
+        handle->ragged_stage_event = nullptr;
+        return 3;
+    }
+    if (handle->ragged_staged && hipEventSynchronize(handle->ragged_stage_event) != hipSuccess) return 3;
+    handle->ragged_staged = false;
+    S.hdr.magic = kRaggedMagic;
+    S.hdr.gen = ++handle->ragged_gen;
     if (handle->ragged_host_bytes < meta) {
         if (handle->ragged_host_stage) (void)hipHostFree(handle->ragged_host_stage);
         handle->ragged_host_stage = nullptr;
@@ -656,20 +806,26 @@ extern "C" int tokura_zgeev_ragged_prepare(tokurablas_t *handle, int batch_count
     std::memcpy(st + S.hdr.off_eoff, S.eoff.data(), 8LL * batch_count);
     std::memcpy(st + S.hdr.off_hoff, S.hoff.data(), 8LL * batch_count);
     if (hipMemcpyAsync(work, st, meta, hipMemcpyHostToDevice, stream) != hipSuccess) return 3;
-    handle->ragged_stage_stream = stream;
-    // remember which configuration this work buffer was prepared for
+    if (hipEventRecord(handle->ragged_stage_event, stream) != hipSuccess) {
+        // the copy may still read the staging buffer: wait for it here
+        if (hipStreamSynchronize(stream) != hipSuccess) return 3;
+    } else {
+        handle->ragged_staged = true;
+    }
+    // remember which configuration and generation this work buffer holds
     bool found = false;
     for (auto &p : handle->prepared)
-        if (p.work == work) { p.cfg = cfg; p.batch = batch_count; found = true; }
-    if (!found) handle->prepared.push_back({work, cfg, batch_count});
+        if (p.work == work) { p.cfg = cfg; p.batch = batch_count; p.gen = S.hdr.gen; found = true; }
+    if (!found) handle->prepared.push_back({work, cfg, batch_count, S.hdr.gen});
     return 0;
 }
 
-static int prepared_cfg(tokurablas_t *h, void *work, int batch)
+// The entry prepared for (work, batch), or null.
+static const RaggedPrepared *prepared_entry(tokurablas_t *h, void *work, int batch)
 {
     for (auto &p : h->prepared)
-        if (p.work == work && p.batch == batch) return p.cfg;
-    return -1;
+        if (p.work == work && p.batch == batch) return &p;
+    return nullptr;
 }
 
 extern "C" int tokura_zgeev_ragged_gpu(tokurablas_t *handle, int batch_count, const hipDoubleComplex *A,
@@ -681,11 +837,11 @@ extern "C" int tokura_zgeev_ragged_gpu(tokurablas_t *handle, int batch_count, co
     if (work == nullptr) return -5;
     if (batch_count <= 0) return -6;
     if (flag == nullptr) return -7;
-    const int cfg = prepared_cfg(handle, work, batch_count);
-    if (cfg < 0) return -8;
-    hipError_t e = launch_ragged(cfg, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
+    const RaggedPrepared *pe = prepared_entry(handle, work, batch_count);
+    if (pe == nullptr) return -8;
+    hipError_t e = launch_ragged(pe->cfg, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
                                  handle->max_sweeps, handle->balance, handle->cu_count > 0 ? handle->cu_count : 1,
-                                 stream);
+                                 pe->gen, batch_count, stream);
     return e == hipSuccess ? 0 : 2;
 }
 
@@ -698,12 +854,13 @@ extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, 
     if (work == nullptr) return -5;
     if (batch_count <= 0) return -6;
     if (flag == nullptr) return -7;
-    const int cfg = prepared_cfg(handle, work, batch_count);
-    if (cfg < 0) return -8;
+    const RaggedPrepared *pe = prepared_entry(handle, work, batch_count);
+    if (pe == nullptr) return -8;
+    const int cfg = pe->cfg, gen = pe->gen;
     RaggedGraphCache &G = handle->graph;
     const bool hit = G.exec && G.A == (const void *)A && G.eig == (void *)eigenvalues && G.work == work &&
                      G.flag == flag && G.batch == batch_count && G.cfg == cfg && G.max_sweeps == handle->max_sweeps &&
-                     G.balance == handle->balance;
+                     G.balance == handle->balance && G.gen == gen;
     if (!hit) {
         if (G.exec) (void)hipGraphExecDestroy(G.exec);
         if (G.graph) (void)hipGraphDestroy(G.graph);
@@ -714,7 +871,7 @@ extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, 
         if (e == hipSuccess) {
             hipError_t el = launch_ragged(cfg, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
                                           handle->max_sweeps, handle->balance,
-                                          handle->cu_count > 0 ? handle->cu_count : 1, cs);
+                                          handle->cu_count > 0 ? handle->cu_count : 1, gen, batch_count, cs);
             e = hipStreamEndCapture(cs, &G.graph);
             if (el != hipSuccess) e = el;
         }
@@ -728,6 +885,7 @@ extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, 
         G.A = A; G.eig = eigenvalues; G.work = work; G.flag = flag; G.batch = batch_count; G.cfg = cfg;
         G.max_sweeps = handle->max_sweeps;
         G.balance = handle->balance;
+        G.gen = gen;
     }
     return hipGraphLaunch(G.exec, stream) == hipSuccess ? 0 : 2;
 }
@@ -750,6 +908,13 @@ extern "C" int tokuraGetRaggedConfig(tokurablas_t *handle, int *cfg, int *lpm_a1
     if (lpm_a2) *lpm_a2 = g_cfg_lpm[c][1];
     if (lpm_b) *lpm_b = g_cfg_lpm[c][2];
     if (lpm_c) *lpm_c = g_cfg_lpm[c][3];
+    return 0;
+}
+
+extern "C" int tokuraGetRaggedWaves(tokurablas_t *handle, int *waves)
+{
+    if (handle == nullptr) return -1;
+    if (waves) *waves = g_cfg_lpm[handle->ragged_cfg][4];
     return 0;
 }
 
@@ -810,9 +975,11 @@ extern "C" int tokura_zgeev_ragged_schur_gpu(tokurablas_t *handle, int batch_cou
     if (work == nullptr || schur == nullptr) return -5;
     if (batch_count <= 0) return -6;
     if (flag == nullptr) return -7;
-    if (prepared_cfg(handle, work, batch_count) < 0) return -8;
+    const RaggedPrepared *pe = prepared_entry(handle, work, batch_count);
+    if (pe == nullptr) return -8;
     hipError_t e = launch_ragged_schur((const zd *)A, (char *)work, (zd *)eigenvalues, (zd *)schur, (int8_t *)flag,
-                                       handle->max_sweeps, handle->cu_count > 0 ? handle->cu_count : 1, stream);
+                                       handle->max_sweeps, handle->cu_count > 0 ? handle->cu_count : 1, pe->gen,
+                                       batch_count, stream);
     return e == hipSuccess ? 0 : 2;
 }
 
@@ -826,8 +993,10 @@ extern "C" int tokura_zgeev_ragged_backward_gpu(tokurablas_t *handle, int batch_
     if (dA == nullptr || work == nullptr) return -5;
     if (batch_count <= 0) return -6;
     if (flag == nullptr) return -7;
-    if (prepared_cfg(handle, work, batch_count) < 0) return -8;
+    const RaggedPrepared *pe = prepared_entry(handle, work, batch_count);
+    if (pe == nullptr) return -8;
     hipError_t e = launch_ragged_adjoint((const zd *)schur, (const zd *)gbar, (zd *)dA, (char *)work, (int8_t *)flag,
-                                         handle->max_sweeps, handle->cu_count > 0 ? handle->cu_count : 1, stream);
+                                         handle->max_sweeps, handle->cu_count > 0 ? handle->cu_count : 1, pe->gen,
+                                         batch_count, stream);
     return e == hipSuccess ? 0 : 2;
 }

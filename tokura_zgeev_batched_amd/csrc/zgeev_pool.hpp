@@ -219,3 +219,148 @@ inline bool build_pool_cfg(int n, const PoolLayout &L, PoolCfg &P)
 }
 
 }  // namespace tkz
+
+namespace tkz {
+
+// ---- chunked arena for work items of any size (the ragged path) ----------
+// The arena is cut into kChunks chunks of kChunk complex elements, with one
+// bit per chunk (set = free) in kBitWords 64-bit LDS words and a lock word.
+// A new work item takes the lowest run of free chunks that holds it; a
+// deflating matrix moves to the highest run that holds its active part, so
+// the regions that items start in stay at the bottom, where the next
+// (LPT order: equal or smaller) item fits the hole it leaves.  Searches and
+// claims run under the lock (one wavefront at a time, a few dozen
+// instructions, no wait inside); frees need no lock (they only set bits the
+// wavefront owns).  Every lane runs each atomic, the lanes that have no word
+// to change with the identity operand.
+constexpr int kChunk = 32;
+constexpr int kChunks = (160 * 1024 - 512) / (16 * kChunk);   // 319
+constexpr int kBitWords = (kChunks + 63) / 64;                 // 5
+
+__device__ __forceinline__ int chunks_for(int elems) { return (elems + kChunk - 1) / kChunk; }
+
+// Bits [s, s+k) of word q (64 bits each) as a mask; 0 when disjoint.
+__device__ __forceinline__ unsigned long long run_mask(int q, int s, int k)
+{
+    const int lo = s > 64 * q ? s - 64 * q : 0, hi = s + k < 64 * (q + 1) ? s + k - 64 * q : 64;
+    if (hi <= lo) return 0ull;
+    const unsigned long long top = hi == 64 ? ~0ull : ((1ull << hi) - 1ull);
+    return top & ~((1ull << lo) - 1ull);
+}
+
+struct ChunkPool {
+    unsigned long long *bits;
+    unsigned *lock;
+    zd *arena;
+    int lane;
+
+    __device__ __forceinline__ bool try_lock() const
+    {
+        return (__builtin_amdgcn_readfirstlane(atomicOr(lock, lane == 0 ? 1u : 0u)) & 1u) == 0u;
+    }
+    __device__ __forceinline__ void unlock() const
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        atomicAnd(lock, lane == 0 ? ~1u : ~0u);
+    }
+    // Start of a run of k free chunks (1 <= k <= kChunks): the lowest
+    // (top = false) or the highest; -1 if none.  Lane l tests the starts
+    // l, l + 64, ...; every lane reads the bit words.
+    __device__ __forceinline__ int find(int k, bool top) const
+    {
+        unsigned long long w[kBitWords];
+#pragma unroll
+        for (int q = 0; q < kBitWords; ++q)
+            w[q] = __hip_atomic_load(&bits[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        int found = -1;
+#pragma unroll
+        for (int jj = 0; jj < kBitWords; ++jj) {
+            const int j = top ? kBitWords - 1 - jj : jj;
+            const int s = lane + 64 * j;
+            bool ok = s + k <= kChunks;
+#pragma unroll
+            for (int q = 0; q < kBitWords; ++q) {
+                const unsigned long long m = run_mask(q, s, k);
+                ok = ok && (w[q] & m) == m;
+            }
+            const unsigned long long b = __ballot(ok);
+            if (found < 0 && b != 0ull) found = 64 * j + (top ? 63 - __clzll((long long)b) : __ffsll((long long)b) - 1);
+        }
+        return __builtin_amdgcn_readfirstlane(found);
+    }
+    // Marks chunks [s, s+k) used (lanes 0..kBitWords-1 each change one word).
+    __device__ __forceinline__ void take(int s, int k) const
+    {
+        const int q = lane % kBitWords;
+        atomicAnd(&bits[q], lane < kBitWords ? ~run_mask(q, s, k) : ~0ull);
+    }
+    __device__ __forceinline__ void give(int s, int k) const
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        const int q = lane % kBitWords;
+        atomicOr(&bits[q], lane < kBitWords ? run_mask(q, s, k) : 0ull);
+    }
+    // The lowest run of k chunks, waiting until one is free.
+    __device__ __forceinline__ int alloc_wait(int k) const
+    {
+        for (;;) {
+            if (try_lock()) {
+                const int s = find(k, false);
+                if (s >= 0) take(s, k);
+                unlock();
+                if (s >= 0) return s;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    // The highest (top) or lowest run of k free chunks, or -1, without
+    // waiting.
+    __device__ __forceinline__ int try_alloc(int k, bool top) const
+    {
+        if (!try_lock()) return -1;
+        const int s = find(k, top);
+        if (s >= 0) take(s, k);
+        unlock();
+        return s;
+    }
+    __device__ __forceinline__ int try_alloc_top(int k) const { return try_alloc(k, true); }
+};
+
+// zlahqr_dual's hook for a matrix in a chunk run: after a deflation, when
+// the active part (columns 0..i and the idle lanes' element) needs at most
+// mv/8 of the run, move it to the highest free run that holds it (if any),
+// writing the eigenvalues of the rows that became final first.
+struct ChunkMover {
+    const ChunkPool *P;
+    zd *out;
+    int s, k;     // current run
+    int whi;      // rows whi.. of out are written
+    zd *H;
+    double us;    // norm-range factor of the eigenvalues
+    int mv;       // move threshold in eighths of the run
+
+    __device__ __forceinline__ zd eig(zd h) const { return us != 1.0 ? zscale(us, h) : h; }
+
+    __device__ __forceinline__ void moved(zd *&Hq, int &idle, int i, int t)
+    {
+        const int ps = packed_size(i + 1);
+        const int need = chunks_for(ps + 1);
+        if (8 * need > mv * k) return;
+        const int s2 = P->try_alloc_top(need);
+        if (s2 < 0) return;
+        const PackedLayout pk;
+        if (i + 1 + t < whi) out[i + 1 + t] = eig(Hq[pk(i + 1 + t, i + 1 + t)]);   // whi <= n <= 64
+        whi = i + 1;
+        zd *H2 = P->arena + s2 * kChunk;
+        for (int e = t; e < ps; e += 64) H2[e] = Hq[e];
+        P->give(s, k);
+        lds_order();
+        s = s2;
+        k = need;
+        Hq = H2;
+        H = H2;
+        idle = need * kChunk - 1;
+    }
+};
+
+}  // namespace tkz

@@ -1,4 +1,5 @@
-"""Ragged (mixed-size) batches through the persistent work-queue kernel.
+"""Ragged (mixed-size) batches: the Hessenberg pass (one launch per LDS size)
+and one persistent QR kernel for the whole batch.
 
 Wraps tokura_get_zgeev_ragged_worksize / _prepare / _gpu / _graph and
 tokuraAutotuneRagged (include/tokura_blas.h).  The reference has no ragged
