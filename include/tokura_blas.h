@@ -82,6 +82,20 @@ int tokura_zgeev_batched_gpu(tokurablas_t *handle, int n, int batch_count,
  * Extensions (no counterpart in the reference)
  * ------------------------------------------------------------------------- */
 
+/* No counterpart in the reference.  tokura_zgeev_batched_gpu without `work`,
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * kernels, with their scratch in a fixed array of 128 MiB that the library
+ * keeps in each GPU's memory until the process ends; this call allocates
+ * nothing.  Limit: batch_count * 8 * ((n-1)(n+2) + 2n) + 256 <= 134217728
+ * bytes (3,914 matrices at n = 64, 55,553 at n = 16), else -9.  Results,
+ * flags and the other return codes as tokura_zgeev_batched_gpu.  Calls that
+ * overlap in time on one GPU (other streams or host threads) share that
+ * array and race. */
+int tokura_zgeev_batched_noalloc_gpu(tokurablas_t *handle, int n, int batch_count,
+                                     const hipDoubleComplex *A, hipDoubleComplex *eigenvalues,
+                                     char *flag, hipStream_t stream);
+
 /* Debug knob: cap on QR sweeps per eigenvalue (< 0 restores LAPACK's
  * 30*max(10,n)).  Lets tests force flag 1 from the host. */
 int tokuraSetMaxSweeps(tokurablas_t *handle, int max_sweeps);

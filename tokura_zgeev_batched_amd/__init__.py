@@ -45,6 +45,7 @@ _SIGNATURES = {
     "tokuraDestroy": ([_P], _I),
     "tokura_get_zgeeveigenvaluesgetworspacesize": ([_I, _I], _SZ),
     "tokura_zgeev_batched_gpu": ([_P, _I, _I, _P, _P, _P, _P, _P], _I),
+    "tokura_zgeev_batched_noalloc_gpu": ([_P, _I, _I, _P, _P, _P, _P], _I),
     "tokuraSetMaxSweeps": ([_P, _I], _I),
     "tokuraSetBalancing": ([_P, ctypes.c_char], _I),
     "tokuraGetLaunchConfig": ([_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 INC = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "libtokura_zgeev.so")
-SOURCES = ["zgeev_kernels.hip", "tokura_ragged.hip", "tokura_adjoint.hip", "dgeev_kernels.hip", "tokura_api.cpp"]
+SOURCES = ["zgeev_kernels.hip", "tokura_ragged.hip", "tokura_adjoint.hip", "dgeev_kernels.hip", "tokura_noalloc.hip",
+           "tokura_api.cpp"]
 ARCH = os.environ.get("TOKURA_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code (none of the
@@ -38,7 +39,7 @@ def _compile(src: str, objdir: str) -> str:
 def build(verbose: bool = False) -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=5) as ex:
+    with ThreadPoolExecutor(max_workers=6) as ex:
         objs = list(ex.map(lambda s: _compile(s, objdir), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]

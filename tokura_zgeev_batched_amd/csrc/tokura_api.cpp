@@ -72,6 +72,7 @@ extern "C" int tokuraCreate(tokurablas_t **handle)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) return 1;
     h->cu_count = prop.multiProcessorCount;
+    noalloc_setup(h->device);   // the scratch array's address on this device (no allocation)
     std::snprintf(h->arch, sizeof(h->arch), "%s", prop.gcnArchName);
     if (char *p = std::strchr(h->arch, ':')) *p = '\0';
     if (const char *cache = std::getenv("TOKURA_TUNE_CACHE")) tokuraLoadTuning(h, cache);

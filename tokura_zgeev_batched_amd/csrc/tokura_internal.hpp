@@ -64,4 +64,6 @@ int resident_blocks(K k, int threads = 64)
 }
 
 void ragged_release(tokurablas_t *h);
+// Looks up the no-allocation entry's scratch array on device `dev` (current).
+void noalloc_setup(int dev);
 int ragged_num_configs();
