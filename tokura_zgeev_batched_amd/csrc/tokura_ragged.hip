@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "tokura_internal.hpp"
+#include "zgeev_hess_reg.hpp"
 #include "zgeev_pool.hpp"
 #include "zgeev_schur.hpp"
 
@@ -402,6 +403,24 @@ __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A
     }
 }
 
+// Hessenberg of the classes c_lo..c_hi (n > 32, n <= 16 NBLK) with the
+// matrix in registers (hess_reg; no balancing): one 256-thread workgroup per
+// matrix, persistent over the classes' matrices.
+template <int NBLK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ragged_hess_reg(
+    const zd *__restrict__ A, char *work, int8_t *__restrict__ flag, zd *__restrict__ eig, int c_lo, int c_hi)
+{
+    __shared__ HregSmem S;
+    const RaggedView V(work);
+    int lo, hi, lo2, hi2;
+    cls_range(V, c_lo, lo, hi);
+    cls_range(V, c_hi, lo2, hi2);
+    for (int j = lo + (int)blockIdx.x; j < hi2; j += gridDim.x) {
+        const int b = V.order[j];
+        hess_reg<NBLK>(S, A + V.moff[b], V.nsz[b], V.H + V.hoff[b], flag + b, eig + V.eoff[b]);
+    }
+}
+
 // The classes n > 32: (class, NB)
 #define TKZ_BIG_CLS(X) X(0, 64) X(1, 60) X(2, 56) X(3, 48) X(4, 40)
 
@@ -415,13 +434,20 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     // resident workgroups per CU, queried once (outside any graph capture the
     // first call is made from tokura_zgeev_ragged_graph's capture stream, where
     // occupancy queries are not stream operations and are allowed)
+    if (bal == 0) {   // classes 0-2 (n 49..64) and 3-4 (n 33..48) with the matrix in registers
+        static const int occ4 = resident_blocks(k_ragged_hess_reg<4>, 256);
+        static const int occ3 = resident_blocks(k_ragged_hess_reg<3>, 256);
+        hipLaunchKernelGGL((k_ragged_hess_reg<4>), dim3(cus * occ4 * 4), dim3(256), 0, s, A, work, flag, eig, 0, 2);
+        hipLaunchKernelGGL((k_ragged_hess_reg<3>), dim3(cus * occ3 * 4), dim3(256), 0, s, A, work, flag, eig, 3, 4);
+    } else {
 #define X(c, nb)                                                                                          \
     {                                                                                                      \
         static const int occ = resident_blocks(k_ragged_hess_wg<nb, c>, 256);                              \
         hipLaunchKernelGGL((k_ragged_hess_wg<nb, c>), dim3(cus * occ * 4), dim3(256), 0, s, A, work, flag, bal, eig); \
     }
-    TKZ_BIG_CLS(X)
+        TKZ_BIG_CLS(X)
 #undef X
+    }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     static const int occ_h = resident_blocks(k_ragged_hess<LB, LC>);

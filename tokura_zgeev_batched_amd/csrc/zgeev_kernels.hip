@@ -18,6 +18,7 @@
 //             runs QR on the packed copy, which needs about half the LDS, so
 //             twice as many matrices are resident per CU at n = 64.
 #include "zgeev_device.hpp"
+#include "zgeev_hess_reg.hpp"
 #include "zgeev_launch.hpp"
 #include "zgeev_pool.hpp"
 
@@ -161,6 +162,21 @@ __global__ __launch_bounds__(NT) void k_hessenberg_packed_wg(int n, int batch, c
     if (tid == 0) flag[mat] = ok ? 0 : 2;
 }
 
+// Stage 1 with the matrix in registers (hess_reg, n <= 16 NBLK, no
+// balancing): 256 threads per matrix and 6 KiB of LDS, so the register file
+// (about 128 VGPRs per thread, four workgroups per CU) bounds the occupancy.
+template <int NBLK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_hessenberg_reg(int n, int batch, const zd *__restrict__ A,
+                                                        zd *__restrict__ work, int8_t *__restrict__ flag,
+                                                        zd *__restrict__ eig, int *qctr = nullptr)
+{
+    if (qctr && blockIdx.x == 0 && threadIdx.x == 0) *qctr = 0;
+    __shared__ HregSmem S;
+    const long long mat = blockIdx.x;
+    hess_reg<NBLK>(S, A + mat * (long long)n * n, n, work + mat * (long long)packed_size(n), flag + mat,
+                   eig + mat * n);
+}
+
 // Stage 2 of the split path: QR on the packed Hessenberg matrix (DUAL: two
 // bulges per sweep, LPM == 64, or LPM == 32 with n <= 32).
 template <int NB, int LPM, bool DUAL = false>
@@ -257,6 +273,13 @@ static int device_cus()
 static void launch_hess_wg(int n, int batch, const zd *A, zd *work, int8_t *flag, int bal, zd *eig, int *qctr,
                            hipStream_t s)
 {
+    if (bal == 0 && n > 32) {   // the register-resident reduction
+        if (n <= 48)
+            hipLaunchKernelGGL((k_hessenberg_reg<3>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag, eig, qctr);
+        else
+            hipLaunchKernelGGL((k_hessenberg_reg<4>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag, eig, qctr);
+        return;
+    }
     const int nb = n <= 40 ? 40 : (n <= 48 ? 48 : (n <= 56 ? 56 : (n <= 60 ? 60 : 64)));
     switch (nb) {
 #define X(b)                                                                                                    \
@@ -369,8 +392,7 @@ static hipError_t launch_split(int n, int batch, const zd *A, zd *eig, int8_t *f
     const int blocks = (batch + GPW - 1) / GPW;
     *stage = 1;
     if constexpr (NB >= 48 && !ONEWAVE)   // large matrices: one workgroup per matrix
-        hipLaunchKernelGGL((k_hessenberg_packed_wg<NB, TKZ_HESS_NT>), dim3(batch), dim3(TKZ_HESS_NT), 0, s, n, batch,
-                           A, work, flag, bal, eig);
+        launch_hess_wg(n, batch, A, work, flag, bal, eig, nullptr, s);
     else
         hipLaunchKernelGGL((k_hessenberg_packed<NB, LPM>), dim3(blocks), dim3(64), 0, s, n, batch, A, work, flag, bal,
                            eig);
