@@ -148,8 +148,10 @@ int tokuraSetVariant(tokurablas_t *handle, int n, int variant);
  * `stream`.  The schedule stays valid for any number of solves of batches
  * with the same offsets; it uses the handle's current class configuration.
  * tokura_zgeev_ragged_gpu launches, on `stream`: a check of the schedule
- * (one fill kernel), the Hessenberg pass (one launch per LDS size: five for
- * n > 32, one for n <= 32), and ONE persistent QR kernel for the whole batch
+ * (one fill kernel), the Hessenberg pass (two launches for n > 32, a
+ * workgroup per matrix with the matrix in registers, or with balancing on
+ * five, one per LDS size; one launch for n <= 32), and ONE persistent QR
+ * kernel for the whole batch
  * (a workgroup per CU whose LDS is shared by matrices of every size; its
  * wavefronts pull tiles, largest first, from one atomic queue, and a matrix
  * moves to a smaller LDS region as it deflates); tokura_zgeev_ragged_graph

@@ -26,6 +26,8 @@ struct RaggedPrepared {
     int cfg;
     int batch;
     int gen;      // the prepare that wrote the schedule (also in its header)
+    int n49_64;   // matrices with 49 <= n <= 64 (size classes 0-2)
+    int n33_48;   // matrices with 33 <= n <= 48 (size classes 3-4)
 };
 
 struct tokurablas_t {

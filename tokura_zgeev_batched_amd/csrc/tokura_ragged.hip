@@ -6,10 +6,12 @@
 //  1. k_ragged_reset     checks that `work` holds this handle's schedule for
 //                        this batch and zeroes the queue head (a fill kernel,
 //                        graph-safe).
-//  2. Householder Hessenberg of every matrix, packed result to `work`, one
-//     launch per LDS size:
-//     k_ragged_hess_wg   one 256-thread workgroup per matrix, n > 32 (five
-//                        launches, LDS for n <= 64, 60, 56, 48, 40),
+//  2. Householder Hessenberg of every matrix, packed result to `work`:
+//     k_ragged_hess_reg  one 256-thread workgroup per matrix of n > 32 with
+//                        the matrix in registers (two launches: n 49..64 and
+//                        n 33..48); with balancing on, k_ragged_hess_wg, the
+//                        LDS-resident reduction, one launch per LDS size
+//                        (n <= 64, 60, 56, 48, 40),
 //     k_ragged_hess      one wavefront per tile of small matrices (n <= 32).
 //  3. k_ragged_qr_pool   ONE persistent QR kernel for the whole batch: a
 //                        workgroup of W wavefronts per CU whose LDS is one
@@ -405,7 +407,8 @@ __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A
 
 // Hessenberg of the classes c_lo..c_hi (n > 32, n <= 16 NBLK) with the
 // matrix in registers (hess_reg; no balancing): one 256-thread workgroup per
-// matrix, persistent over the classes' matrices.
+// matrix, the grid sized on the host from the schedule; the range itself
+// comes from the header (empty for a stale schedule).
 template <int NBLK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ragged_hess_reg(
     const zd *__restrict__ A, char *work, int8_t *__restrict__ flag, zd *__restrict__ eig, int c_lo, int c_hi)
@@ -415,10 +418,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     int lo, hi, lo2, hi2;
     cls_range(V, c_lo, lo, hi);
     cls_range(V, c_hi, lo2, hi2);
-    for (int j = lo + (int)blockIdx.x; j < hi2; j += gridDim.x) {
-        const int b = V.order[j];
-        hess_reg<NBLK>(S, A + V.moff[b], V.nsz[b], V.H + V.hoff[b], flag + b, eig + V.eoff[b]);
-    }
+    const int j = lo + (int)blockIdx.x;
+    if (j >= hi2) return;
+    const int b = V.order[j];
+    hess_reg<NBLK>(S, A + V.moff[b], V.nsz[b], V.H + V.hoff[b], flag + b, eig + V.eoff[b]);
 }
 
 // The classes n > 32: (class, NB)
@@ -426,19 +429,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 
 template <int LA1, int LA2, int LB, int LC, int W, int MV>
 static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int bal, int cus,
-                                    int gen, int batch, hipStream_t s)
+                                    const RaggedPrepared &pe, hipStream_t s)
 {
     static_assert(LA1 == 64 && LA2 == 64, "classes n > 32: one matrix per wavefront");
-    hipError_t e = launch_reset(work, gen, batch, flag, s);
+    hipError_t e = launch_reset(work, pe.gen, pe.batch, flag, s);
     if (e != hipSuccess) return e;
     // resident workgroups per CU, queried once (outside any graph capture the
     // first call is made from tokura_zgeev_ragged_graph's capture stream, where
     // occupancy queries are not stream operations and are allowed)
     if (bal == 0) {   // classes 0-2 (n 49..64) and 3-4 (n 33..48) with the matrix in registers
-        static const int occ4 = resident_blocks(k_ragged_hess_reg<4>, 256);
-        static const int occ3 = resident_blocks(k_ragged_hess_reg<3>, 256);
-        hipLaunchKernelGGL((k_ragged_hess_reg<4>), dim3(cus * occ4 * 4), dim3(256), 0, s, A, work, flag, eig, 0, 2);
-        hipLaunchKernelGGL((k_ragged_hess_reg<3>), dim3(cus * occ3 * 4), dim3(256), 0, s, A, work, flag, eig, 3, 4);
+        if (pe.n49_64 > 0)
+            hipLaunchKernelGGL((k_ragged_hess_reg<4>), dim3(pe.n49_64), dim3(256), 0, s, A, work, flag, eig, 0, 2);
+        if (pe.n33_48 > 0)
+            hipLaunchKernelGGL((k_ragged_hess_reg<3>), dim3(pe.n33_48), dim3(256), 0, s, A, work, flag, eig, 3, 4);
     } else {
 #define X(c, nb)                                                                                          \
     {                                                                                                      \
@@ -459,12 +462,12 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     return hipGetLastError();
 }
 
-static hipError_t launch_ragged(int cfg, const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int bal,
-                                int cus, int gen, int batch, hipStream_t s)
+static hipError_t launch_ragged(const RaggedPrepared &pe, const zd *A, char *work, zd *eig, int8_t *flag, int itmax,
+                                int bal, int cus, hipStream_t s)
 {
-    switch (cfg) {
+    switch (pe.cfg) {
 #define X(id, a1, a2, b, c, w, mv) \
-    case id: return launch_ragged_cfg<a1, a2, b, c, w, mv>(A, work, eig, flag, itmax, bal, cus, gen, batch, s);
+    case id: return launch_ragged_cfg<a1, a2, b, c, w, mv>(A, work, eig, flag, itmax, bal, cus, pe, s);
         TKZ_RAGGED_CFGS(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -839,10 +842,11 @@ extern "C" int tokura_zgeev_ragged_prepare(tokurablas_t *handle, int batch_count
         handle->ragged_staged = true;
     }
     // remember which configuration and generation this work buffer holds
+    const RaggedPrepared rp{work, cfg, batch_count, S.hdr.gen, S.hdr.cls_end[2], S.hdr.cls_end[4] - S.hdr.cls_end[2]};
     bool found = false;
     for (auto &p : handle->prepared)
-        if (p.work == work) { p.cfg = cfg; p.batch = batch_count; p.gen = S.hdr.gen; found = true; }
-    if (!found) handle->prepared.push_back({work, cfg, batch_count, S.hdr.gen});
+        if (p.work == work) { p = rp; found = true; }
+    if (!found) handle->prepared.push_back(rp);
     return 0;
 }
 
@@ -865,9 +869,9 @@ extern "C" int tokura_zgeev_ragged_gpu(tokurablas_t *handle, int batch_count, co
     if (flag == nullptr) return -7;
     const RaggedPrepared *pe = prepared_entry(handle, work, batch_count);
     if (pe == nullptr) return -8;
-    hipError_t e = launch_ragged(pe->cfg, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
+    hipError_t e = launch_ragged(*pe, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
                                  handle->max_sweeps, handle->balance, handle->cu_count > 0 ? handle->cu_count : 1,
-                                 pe->gen, batch_count, stream);
+                                 stream);
     return e == hipSuccess ? 0 : 2;
 }
 
@@ -895,9 +899,9 @@ extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, 
         if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return 3;
         hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            hipError_t el = launch_ragged(cfg, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
+            hipError_t el = launch_ragged(*pe, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
                                           handle->max_sweeps, handle->balance,
-                                          handle->cu_count > 0 ? handle->cu_count : 1, gen, batch_count, cs);
+                                          handle->cu_count > 0 ? handle->cu_count : 1, cs);
             e = hipStreamEndCapture(cs, &G.graph);
             if (el != hipSuccess) e = el;
         }

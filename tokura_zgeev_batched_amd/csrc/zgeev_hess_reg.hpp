@@ -145,30 +145,21 @@ struct HregSmem {
     double red[4];
 };
 
-// Reflector of column j by the wavefront holding it (wave index wv uniform):
-// tau (and v, unless tau = 0) to LDS buffer j & 1, H(j+1, j) to S.beta[j].
-// R's column j is left as it is: no later step reads it (the right updates
-// multiply it by v(c) = 0, the left updates skip columns <= i), and only its
-// rows <= j are part of the result.
-template <int NBLK>
-__device__ __forceinline__ void hreg_reflect(const zd (&R)[NBLK][NBLK], HregSmem &S, int j, int wv, int q, int tr)
+// Reflector of column j by the wavefront holding it, column block B = j / 16
+// (wave index wv uniform): tau (and v, unless tau = 0) to LDS buffer j & 1,
+// H(j+1, j) to S.beta[j].  R's column j is left as it is: no later step reads
+// it (the right updates multiply it by v(c) = 0, the left updates skip
+// columns <= i), and only its rows <= j are part of the result.
+template <int NBLK, int B>
+__device__ __forceinline__ void hreg_reflect_b(const zd (&R)[NBLK][NBLK], HregSmem &S, int j, int q, int tr)
 {
-    const int tco = j & 15, wo = tco >> 2, qo = tco & 3, bo = j >> 4;
-    if (wv != wo) return;
-    zd col[NBLK];
-#pragma unroll
-    for (int a = 0; a < NBLK; ++a) {
-        col[a] = mk(0.0, 0.0);
-#pragma unroll
-        for (int b = 0; b < NBLK; ++b)
-            if (b == bo) col[a] = R[a][b];
-    }
+    const int qo = (j & 15) & 3;
     double ssq = 0.0, amx = 0.0;
 #pragma unroll
     for (int a = 0; a < NBLK; ++a)
         if (tr + 16 * a >= j + 2) {
-            ssq += col[a].re * col[a].re + col[a].im * col[a].im;
-            amx = fmax(amx, fmax(fabs(col[a].re), fabs(col[a].im)));
+            ssq += R[a][B].re * R[a][B].re + R[a][B].im * R[a][B].im;
+            amx = fmax(amx, fmax(fabs(R[a][B].re), fabs(R[a][B].im)));
         }
     const int l0 = qo * 16;
     ssq = readlane_d(hreg::row_sum(ssq), l0);
@@ -177,7 +168,7 @@ __device__ __forceinline__ void hreg_reflect(const zd (&R)[NBLK][NBLK], HregSmem
     zd ca = mk(0.0, 0.0);
 #pragma unroll
     for (int a = 0; a < NBLK; ++a)
-        if (a == aa) ca = col[a];
+        if (a == aa) ca = R[a][B];
     const zd alpha = mk(readlane_d(ca.re, la), readlane_d(ca.im, la));
     double xnorm = 0.0;
     if (amx > 0.0) {
@@ -189,31 +180,46 @@ __device__ __forceinline__ void hreg_reflect(const zd (&R)[NBLK][NBLK], HregSmem
 #pragma unroll
             for (int a = 0; a < NBLK; ++a)
                 if (tr + 16 * a >= j + 2)
-                    s2 += (col[a].re * ia) * (col[a].re * ia) + (col[a].im * ia) * (col[a].im * ia);
+                    s2 += (R[a][B].re * ia) * (R[a][B].re * ia) + (R[a][B].im * ia) * (R[a][B].im * ia);
             xnorm = amx * sqrt(readlane_d(hreg::row_sum(s2), l0));
         }
     }
     const int buf = j & 1;
-    if (xnorm == 0.0 && alpha.im == 0.0) {   // tau = 0, H(j) = I
-        S.tau[buf] = mk(0.0, 0.0);   // the same value from every lane
-        S.beta[j] = alpha.re;
-        return;
+    double beta;
+    zd tau, scal;
+    if (xnorm == 0.0 && alpha.im == 0.0) {   // tau = 0, H(j) = I: the updates of step j add zeros
+        beta = alpha.re;
+        tau = mk(0.0, 0.0);
+        scal = mk(0.0, 0.0);
+    } else {
+        const double wm = fmax(fmax(fabs(alpha.re), fabs(alpha.im)), xnorm);
+        const double iw = 1.0 / wm;
+        const double ar = alpha.re * iw, ai = alpha.im * iw, xn = xnorm * iw;
+        beta = -copysign(wm * sqrt(ar * ar + ai * ai + xn * xn), alpha.re);
+        tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
+        scal = zdiv(mk(1.0, 0.0), mk(alpha.re - beta, alpha.im));
     }
-    const double wm = fmax(fmax(fabs(alpha.re), fabs(alpha.im)), xnorm);
-    const double iw = 1.0 / wm;
-    const double ar = alpha.re * iw, ai = alpha.im * iw, xn = xnorm * iw;
-    const double beta = -copysign(wm * sqrt(ar * ar + ai * ai + xn * xn), alpha.re);
-    const zd tau = mk((beta - alpha.re) / beta, -alpha.im / beta);
-    const zd scal = zdiv(mk(1.0, 0.0), mk(alpha.re - beta, alpha.im));
     if (q == qo) {
 #pragma unroll
         for (int a = 0; a < NBLK; ++a) {
             const int r = tr + 16 * a;
-            S.xv[buf][r] = r == j + 1 ? mk(1.0, 0.0) : (r > j + 1 ? zmul(col[a], scal) : mk(0.0, 0.0));
+            S.xv[buf][r] = r == j + 1 ? mk(1.0, 0.0) : (r > j + 1 ? zmul(R[a][B], scal) : mk(0.0, 0.0));
         }
     }
-    S.tau[buf] = tau;
+    S.tau[buf] = tau;   // the same value from every lane
     S.beta[j] = beta;
+}
+
+template <int NBLK, int B = 0>
+__device__ __forceinline__ void hreg_reflect(const zd (&R)[NBLK][NBLK], HregSmem &S, int j, int wv, int q, int tr)
+{
+    if constexpr (B < NBLK) {
+        if ((j >> 4) == B) {
+            if (wv == ((j & 15) >> 2)) hreg_reflect_b<NBLK, B>(R, S, j, q, tr);
+        } else {
+            hreg_reflect<NBLK, B + 1>(R, S, j, wv, q, tr);
+        }
+    }
 }
 
 // Right and left update of step i; K = (i + 1) / 16 is the first block of
@@ -222,8 +228,7 @@ template <int NBLK, int K>
 __device__ __forceinline__ void hreg_step(zd (&R)[NBLK][NBLK], HregSmem &S, int i, int wv, int lane, int tr, int tc)
 {
     const int buf = i & 1;
-    const zd tau = mk(readlane_d(S.tau[buf].re, 0), readlane_d(S.tau[buf].im, 0));   // uniform
-    if (tau.re == 0.0 && tau.im == 0.0) return;
+    const zd tau = S.tau[buf];
     const zd *v = S.xv[buf];
     zd vc[NBLK];
 #pragma unroll
@@ -266,13 +271,20 @@ __device__ __forceinline__ void hreg_step(zd (&R)[NBLK][NBLK], HregSmem &S, int 
     }
 }
 
+// Steps i with (i + 1) / 16 == K, then the next phase: one loop per K, so the
+// loop body has a single path and R stays in place.
 template <int NBLK, int K = 0>
-__device__ __forceinline__ void hreg_dispatch(int k, zd (&R)[NBLK][NBLK], HregSmem &S, int i, int wv, int lane,
-                                              int tr, int tc)
+__device__ __forceinline__ void hreg_phases(zd (&R)[NBLK][NBLK], HregSmem &S, int n, int wv, int lane, int q,
+                                            int tr, int tc)
 {
     if constexpr (K < NBLK) {
-        if (k == K) hreg_step<NBLK, K>(R, S, i, wv, lane, tr, tc);
-        else hreg_dispatch<NBLK, K + 1>(k, R, S, i, wv, lane, tr, tc);
+        const int i1 = min(n - 2, 16 * K + 14);
+        for (int i = K == 0 ? 0 : 16 * K - 1; i <= i1; ++i) {
+            __syncthreads();   // v and tau of step i are in LDS
+            hreg_step<NBLK, K>(R, S, i, wv, lane, tr, tc);
+            if (i + 3 <= n) hreg_reflect<NBLK>(R, S, i + 1, wv, q, tr);
+        }
+        hreg_phases<NBLK, K + 1>(R, S, n, wv, lane, q, tr, tc);
     }
 }
 
@@ -315,11 +327,7 @@ __device__ __forceinline__ void hess_reg(HregSmem &S, const zd *__restrict__ src
     if (ok && tid == 0) eig0[0] = mk(ldexp(1.0, -se), 0.0);
     if (ok) {
         if (n >= 2) hreg_reflect<NBLK>(R, S, 0, wv, q, tr);
-        for (int i = 0; i + 2 <= n; ++i) {
-            __syncthreads();   // v and tau of step i are in LDS
-            hreg_dispatch<NBLK>((i + 1) >> 4, R, S, i, wv, lane, tr, tc);
-            if (i + 3 <= n) hreg_reflect<NBLK>(R, S, i + 1, wv, q, tr);
-        }
+        hreg_phases<NBLK>(R, S, n, wv, lane, q, tr, tc);
     }
     const PackedLayout pk;
     int tr_o = tr, tc_o = tc;

@@ -1,5 +1,6 @@
-"""Ragged (mixed-size) batches: the Hessenberg pass (one launch per LDS size)
-and one persistent QR kernel for the whole batch.
+"""Ragged (mixed-size) batches: the Hessenberg pass (a workgroup per matrix of
+n > 32, a wavefront per tile of small matrices) and one persistent QR kernel
+for the whole batch.
 
 Wraps tokura_get_zgeev_ragged_worksize / _prepare / _gpu / _graph and
 tokuraAutotuneRagged (include/tokura_blas.h).  The reference has no ragged
