@@ -447,7 +447,7 @@ def main():
     work = torch.empty((tk.workspace_size(n, B),), dtype=torch.uint8, device=dev)
     h = tk.Handle()
     if not args.no_tune:
-        h.autotune(n, n, 8192)          # runtime autotuner picks the kernel variant for n
+        h.autotune(n, n, min(B, 32768))   # runtime autotuner picks the kernel variant for n
     variant, lpm, mpb = h.launch_config(n)
     stream = torch.cuda.current_stream()
 

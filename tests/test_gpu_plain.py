@@ -241,3 +241,48 @@ def test_deterministic_across_calls_batches_and_graph():
         torch.cuda.synchronize()
         np.testing.assert_array_equal(eg.cpu().numpy().reshape(50, n), e1[:50])
     h.close()
+
+
+@pytest.mark.parametrize("n", [33, 40, 48, 49, 56, 64])
+def test_register_hessenberg_paths(handle, n):
+    """The rare paths of the register-resident Hessenberg reduction (n > 32)
+    against LAPACK zgeev(x) on the same matrices:
+    * a column whose tail has a sum of squares below 1e-290 while the matrix
+      norm is in range (the rescaled second pass of the reflector's norm), at
+      columns on both sides of the 16-column blocks;
+    * block-diagonal matrices (columns with zero tails: tau = 0 between
+      ordinary reflectors), with block boundaries on and off the 16-row
+      blocks of the register grid;
+    * upper Hessenberg input with complex sub-diagonal (zero tails, alpha not
+      real: beta = -|alpha|, v = e_1).
+    Bar: condition-scaled error <= 100 (units of kappa_i eps ||A||_F) and
+    absolute error <= 1e-13 ||A||_F after matching."""
+    rng = np.random.default_rng(90 + n)
+    cases = []
+    # upper Hessenberg but for column j's tail (rows j+2..), of size 1e-150:
+    # the reflectors before step j only change phases (v = e_1), so step j
+    # sees the tiny tail
+    G = rand(rng, 6, n)
+    for b, j in enumerate((0, 1, 15, 16, 17, n - 3)):
+        tail = G[b, j + 2:, j].copy()
+        G[b] = np.triu(G[b], -1)
+        G[b, j + 2:, j] = tail * 1e-150
+    cases.append(("tiny tail", G))
+    B = rand(rng, 6, n)
+    for b, cut in enumerate((1, 15, 16, 17, n // 2, n - 2)):
+        B[b, cut:, :cut] = 0
+        B[b, :cut, cut:] = 0
+    cases.append(("block diagonal", B))
+    Hh = rand(rng, 4, n)
+    for b in range(4):
+        Hh[b] = np.triu(Hh[b], -1)
+    cases.append(("Hessenberg input", Hh))
+    for name, M in cases:
+        e, flag = gpu_eigs(M, handle)
+        assert not flag.any(), name
+        cs = oracle.condscaled_err(M, e)
+        assert cs.max() <= 100, (name, cs.max())
+        ref, _ = oracle.lapack_zgeev(M)
+        for b in range(M.shape[0]):
+            perm = oracle.match(ref[b], e[b])
+            assert np.abs(e[b][perm] - ref[b]).max() <= 1e-13 * np.linalg.norm(M[b]), (name, b)

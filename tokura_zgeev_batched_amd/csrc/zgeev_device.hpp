@@ -1346,8 +1346,12 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                     row = pin && p >= k + 2;
                     col = pin && p <= k - 1;
                 }
-                ax = row ? cbp + k : cbks + p;
-                ay = ax + (row ? 1 : k + 2);
+                // both candidates, then one select each (a select of the
+                // scalar parts first would need their copies to VGPRs)
+                const int axr = cbp + k, axc = cbks + p;
+                const int ayr = axr + 1, ayc = axc + (k + 2);
+                ax = row ? axr : axc;
+                ay = row ? ayr : ayc;
                 if constexpr (DENSE) {
                     if (!(row || col)) ax = ay = idle_slot;
                 }
