@@ -81,10 +81,10 @@ static_assert(sizeof(RaggedHeader) == 184, "header size");
     X(1, 64, 64, 32, 8, 12, 5)  \
     X(2, 64, 64, 32, 16, 12, 5) \
     X(3, 64, 64, 16, 16, 12, 5) \
-    X(4, 64, 64, 32, 8, 16, 5)  \
+    X(4, 64, 64, 32, 16, 8, 5)  \
     X(5, 64, 64, 32, 8, 8, 5)   \
-    X(6, 64, 64, 16, 8, 16, 5)  \
-    X(7, 64, 64, 32, 16, 16, 5) \
+    X(6, 64, 64, 32, 8, 8, 4)   \
+    X(7, 64, 64, 32, 16, 8, 6)  \
     X(8, 64, 64, 32, 32, 12, 5) \
     X(9, 64, 64, 32, 32, 8, 5)  \
     X(10, 64, 64, 32, 32, 12, 6) \
@@ -250,13 +250,13 @@ __global__ __launch_bounds__(64 * W) void k_ragged_qr_pool(char *work, zd *__res
 {
     __shared__ zd arena[kChunks * kChunk];
     __shared__ unsigned long long bits[kBitWords];
-    __shared__ unsigned lock;
+    __shared__ unsigned lock, freed;
     const RaggedView V(work);
     const int lane = threadIdx.x & 63;
     if (threadIdx.x < kBitWords) bits[threadIdx.x] = run_mask((int)threadIdx.x, 0, kChunks);
-    if (threadIdx.x == 0) lock = 0u;
+    if (threadIdx.x == 0) lock = freed = 0u;
     __syncthreads();
-    const ChunkPool P{bits, &lock, arena, lane};
+    const ChunkPool P{bits, &lock, &freed, arena, lane};
     const int nt = V.h->ntiles;
     // tile t: its descriptor (uniform) and the chunks its QR needs
     auto desc = [&](int t) {
@@ -310,6 +310,7 @@ __global__ __launch_bounds__(64 * W) void k_ragged_qr_pool(char *work, zd *__res
             pd = desc(pend);
             pk = need(pd);
         }
+        const unsigned seen = P.frees();
         int4 td;
         int k, s0 = P.try_alloc(pk, false);
         if (s0 >= 0) {
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(64 * W) void k_ragged_qr_pool(char *work, zd *__res
                 back_open = bt >= 0;
             }
             if (bt < 0) {
-                __builtin_amdgcn_s_sleep(2);
+                P.wait_free(seen);
                 continue;
             }
             td = desc(bt);

@@ -228,10 +228,11 @@ namespace tkz {
 // A new work item takes the lowest run of free chunks that holds it; a
 // deflating matrix moves to the highest run that holds its active part, so
 // the regions that items start in stay at the bottom, where the next
-// (LPT order: equal or smaller) item fits the hole it leaves.  Searches and
-// claims run under the lock (one wavefront at a time, a few dozen
-// instructions, no wait inside); frees need no lock (they only set bits the
-// wavefront owns).  Every lane runs each atomic, the lanes that have no word
+// (LPT order: equal or smaller) item fits the hole it leaves.  Claims run
+// under the lock (one wavefront at a time, a few dozen instructions, no
+// wait inside) after a search of a snapshot without it, and nothing waits
+// for the lock beyond a few tries; frees need no lock (they only set bits
+// the wavefront owns).  Every lane runs each atomic, the lanes that have no word
 // to change with the identity operand.
 constexpr int kChunk = 32;
 constexpr int kChunks = (160 * 1024 - 512) / (16 * kChunk);   // 319
@@ -251,6 +252,7 @@ __device__ __forceinline__ unsigned long long run_mask(int q, int s, int k)
 struct ChunkPool {
     unsigned long long *bits;
     unsigned *lock;
+    unsigned *freed;   // count of frees: a waiting wavefront searches again only after one
     zd *arena;
     int lane;
 
@@ -299,29 +301,51 @@ struct ChunkPool {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         const int q = lane % kBitWords;
         atomicOr(&bits[q], lane < kBitWords ? run_mask(q, s, k) : 0ull);
+        atomicAdd(freed, lane == 0 ? 1u : 0u);
+    }
+    __device__ __forceinline__ unsigned frees() const
+    {
+        return __builtin_amdgcn_readfirstlane(__hip_atomic_load(freed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+    }
+    // Sleeps until a run is freed after `seen` frees, or for kPolls polls
+    // (a search can also fail for want of the lock): a waiting wavefront
+    // issues one LDS read per ~500 cycles instead of a search, leaving the
+    // SIMD to the wavefronts that hold matrices.
+    static constexpr int kPolls = 16;
+    __device__ __forceinline__ void wait_free(unsigned seen) const
+    {
+        for (int p = 0; p < kPolls && frees() == seen; ++p) __builtin_amdgcn_s_sleep(8);
+    }
+    // The highest (top) or lowest run of k free chunks, claimed, or -1.  A
+    // search of a snapshot of the bit words comes first, without the lock,
+    // so wavefronts that find no room never take it (a mover that wants a
+    // smaller run then seldom finds the lock held by waiting wavefronts);
+    // then the lock is tried at most kLockTries times (its holder runs one
+    // search and claim) for a second search and the claim.
+    static constexpr int kLockTries = 8;
+    __device__ __forceinline__ int try_alloc(int k, bool top) const
+    {
+        if (find(k, top) < 0) return -1;
+        for (int r = 0; r < kLockTries; ++r) {
+            if (try_lock()) {
+                const int s = find(k, top);
+                if (s >= 0) take(s, k);
+                unlock();
+                return s;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        return -1;
     }
     // The lowest run of k chunks, waiting until one is free.
     __device__ __forceinline__ int alloc_wait(int k) const
     {
         for (;;) {
-            if (try_lock()) {
-                const int s = find(k, false);
-                if (s >= 0) take(s, k);
-                unlock();
-                if (s >= 0) return s;
-            }
-            __builtin_amdgcn_s_sleep(2);
+            const unsigned seen = frees();
+            const int s = try_alloc(k, false);
+            if (s >= 0) return s;
+            wait_free(seen);
         }
-    }
-    // The highest (top) or lowest run of k free chunks, or -1, without
-    // waiting.
-    __device__ __forceinline__ int try_alloc(int k, bool top) const
-    {
-        if (!try_lock()) return -1;
-        const int s = find(k, top);
-        if (s >= 0) take(s, k);
-        unlock();
-        return s;
     }
     __device__ __forceinline__ int try_alloc_top(int k) const { return try_alloc(k, true); }
 };
