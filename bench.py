@@ -276,11 +276,12 @@ def bench_diff(dev, B=50_000, n=32, steps=3, seed=3):
 
     t_bwd_rc = timed(bwd_recompute, steps, 1)
     _, fl_rc = state["rc"]
-    # given the Schur forward's eigenvalues (the Schur form's own order) the
-    # recompute backward pairs every cotangent with itself: the same bits as
-    # the stored-Schur backward
+    # the recompute backward's Schur form comes from snapped shifts, so its
+    # gradient agrees with the stored-Schur one to rounding: the largest
+    # entry-wise difference relative to each matrix's largest gradient entry
     dA_rc, _ = backward_recompute(n, B, A, e_s, gbar, handle=h)
-    same = bool(torch.equal(dA_store, dA_rc))
+    scale = dA_store.abs().amax(dim=(1, 2)).clamp(min=1.0)
+    rel = float(((dA_rc - dA_store).abs().amax(dim=(1, 2)) / scale).max())
     return {"workload": f"differentiation, batch={B}, n={n}, loss Re sum conj(g) lambda",
             "fwd_eigvals_ms": t_fwd * 1e3, "fwd_schur_ms": t_schur * 1e3, "bwd_ms": t_bwd * 1e3,
             "bwd_recompute_ms": t_bwd_rc * 1e3,
@@ -290,7 +291,7 @@ def bench_diff(dev, B=50_000, n=32, steps=3, seed=3):
             "recompute_fwd_bwd_solves_per_s": B / (t_fwd + t_bwd_rc),
             "recompute_fwd_bwd_over_fwd_eigvals": (t_fwd + t_bwd_rc) / t_fwd,
             "recompute_flags_nonzero": int((fl_rc != 0).sum()),
-            "recompute_grad_equals_store_grad": same}
+            "recompute_grad_max_rel_diff_vs_store": rel}
 
 
 def bench_vectors(dev, cases=((32, 50_000), (64, 20_000)), steps=3, seed=11):

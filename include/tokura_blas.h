@@ -217,10 +217,13 @@ int tokuraSetBalancing(tokurablas_t *handle, char job);
  * its eigenvalues): tokura_zgeev_batched_backward_recompute_gpu rebuilds the
  * Schur form from A (which must be the forward's unmodified input) with
  * `work` of tokura_get_zgeeveigenvaluesgetworspacesize(n, batch) bytes as
- * scratch (T and the recomputed eigenvalues; U is parked in dA), pairs each
- * recomputed eigenvalue with the nearest of `eigenvalues` (the forward's
- * output, from any same-size variant, in its order), applies the cotangent
- * gbar of that forward index, and writes dA as above.  flag: 0/1/2 of the
+ * scratch (T and the recomputed eigenvalues; U is parked in dA), its QR
+ * shifts snapped to the nearest of `eigenvalues` (about half the sweeps of
+ * the forward's QR; the Schur form, hence dA, equals the stored-Schur one up
+ * to rounding), pairs each recomputed eigenvalue with the nearest of
+ * `eigenvalues` (the forward's output, from any same-size variant, in its
+ * order), applies the cotangent gbar of that forward index, and writes dA
+ * as above.  flag: 0/1/2 of the
  * recomputed forward, or 3 when the eigenvalues of matrix j are not those
  * of A_j (a pair further apart than 1e-6 * max |lambda|; dA = NaN).
  * Returns 0, -1 handle, -2 n, -3 A NULL, -4 eigenvalues or gbar NULL, -5 dA

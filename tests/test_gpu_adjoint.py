@@ -77,8 +77,10 @@ def test_recompute_after_default_forward_vs_oracle(n):
 @pytest.mark.parametrize("n", [2, 9, 16, 20, 32, 36, 50, 64])
 def test_recompute_pairs_shuffled_forward(n):
     """Forward eigenvalues and cotangents shuffled by the same random
-    permutation per matrix give the same dL/dA bits as the stored-Schur
-    backward with them in the Schur order."""
+    permutation per matrix give the same dL/dA bits as the recompute backward
+    with them in the Schur order (the snapped shifts do not depend on the
+    order of the known eigenvalues), and agree with the stored-Schur backward
+    to 1e-9 (the recompute's Schur form comes from other shifts)."""
     rng = np.random.default_rng(800 + n)
     B = 12
     M = rand(rng, B, n)
@@ -91,9 +93,12 @@ def test_recompute_pairs_shuffled_forward(n):
     g_p = torch.gather(g, 1, perm)
     print(f"[case] shuffled forward n={n}", flush=True)
     dA_rc, f2 = backward_recompute(n, B, A, e_p, g_p)
+    dA_rs, f3 = backward_recompute(n, B, A, e_s, g)
     torch.cuda.synchronize()
-    assert int(f2.abs().sum()) == 0
-    assert torch.equal(dA_rc, dA_store)
+    assert int(f2.abs().sum()) == 0 and int(f3.abs().sum()) == 0
+    assert torch.equal(dA_rc, dA_rs)
+    scale = dA_store.abs().amax(dim=(1, 2)).clamp(min=1.0)
+    assert float(((dA_rc - dA_store).abs().amax(dim=(1, 2)) / scale).max()) <= 1e-9
 
 
 @pytest.mark.parametrize("n", [4, 16, 30, 64])
@@ -212,11 +217,12 @@ def test_adjoint_vs_finite_differences():
 
 
 @pytest.mark.parametrize("n", [5, 17, 32, 33, 40, 47, 50, 64])
-def test_store_and_recompute_gradients_bitwise(n):
-    """The stored-Schur backward and the recompute backward (the same
-    two-launch Schur form rebuilt with T in the workspace and U in dA, then
-    the same adjoint arithmetic) give identical dL/dA bits when the recompute
-    is handed the Schur forward's eigenvalues."""
+def test_store_and_recompute_gradients_agree(n):
+    """The stored-Schur backward and the recompute backward (the two-launch
+    Schur form rebuilt with T in the workspace and U in dA, its QR shifts
+    snapped to the given eigenvalues, then the same adjoint arithmetic) agree
+    to 1e-9 of the gradient's largest entry, and the recompute gradient
+    matches the oracle's to the same bar."""
     rng = np.random.default_rng(900 + n)
     M = rand(rng, 40, n)
     A = _colmajor(M)
@@ -228,7 +234,14 @@ def test_store_and_recompute_gradients_bitwise(n):
     d2, f2 = backward_recompute(n, 40, A, e_s, g)
     torch.cuda.synchronize()
     assert int(f2.abs().sum()) == 0
-    assert torch.equal(d1, d2)
+    scale = d1.abs().amax(dim=(1, 2)).clamp(min=1.0)
+    assert float(((d1 - d2).abs().amax(dim=(1, 2)) / scale).max()) <= 1e-9
+    G = d2.transpose(-1, -2).cpu().numpy()
+    e = e_s.cpu().numpy()
+    gh = g.cpu().numpy()
+    for b in range(0, 40, 8):
+        Go = oracle_grad_in_order(M[b], gh[b], e[b])
+        assert np.abs(G[b] - Go).max() <= 1e-9 * max(1.0, np.abs(Go).max()), b
 
 
 def test_adjoint_flags_nan():

@@ -148,17 +148,20 @@ __global__ __launch_bounds__(NT) void k_schur_hq_wg(int n, int batch, const zd *
 
 // Second half: the QR on the packed layout (four matrices per CU at n = 64);
 // eigenvalues, T, U in place, flag 0/1 (matrices flagged 2 are skipped).
-template <int NB, int LPM>
+// SNAP: shifts snapped to the matrix's known eigenvalues, `known` + mat * n.
+template <int NB, int LPM, bool SNAP = false>
 __global__ __launch_bounds__(64) void k_schur_qr(int n, int batch, zd *__restrict__ eig, zd *Tout, long long tstride,
-                                                 zd *Uout, long long ustride, int8_t *flag, int itmax)
+                                                 zd *Uout, long long ustride, int8_t *flag, int itmax,
+                                                 const zd *__restrict__ known = nullptr)
 {
     constexpr int GPW = 64 / LPM, PER = packed_size(NB) + 1;   // + the idle lanes' element
     __shared__ zd lds[GPW * PER];
     const int lane = (int)threadIdx.x, g = lane / LPM, t = lane % LPM;
     const long long mat = (long long)blockIdx.x * GPW + g;
     if (mat >= batch || flag[mat] == 2) return;
-    const int info = schur_qr_one<LPM, (LPM == 64)>(lds + g * PER, n, t, Tout + mat * tstride,
-                                                    Uout + mat * ustride, eig + mat * n, itmax);
+    const int info = schur_qr_one<LPM, (LPM == 64), SNAP>(lds + g * PER, n, t, Tout + mat * tstride,
+                                                          Uout + mat * ustride, eig + mat * n, itmax,
+                                                          SNAP ? known + mat * n : nullptr);
     if (t == 0) flag[mat] = info != 0 ? 1 : 0;
 }
 
@@ -304,7 +307,7 @@ static int split_nb(int n)
 template <int NB, int LPM>
 static hipError_t launch_schur_split_nb(int n, int batch, const zd *A, zd *eig, zd *Tout, long long tstride,
                                         zd *Uout, long long ustride, int8_t *flag, int itmax, hipStream_t s, bool vec,
-                                        zd *VLv, zd *VRv)
+                                        zd *VLv, zd *VRv, const zd *known)
 {
     const dim3 grid((batch + 64 / LPM - 1) / (64 / LPM));
     if constexpr (NB > 32)
@@ -313,8 +316,12 @@ static hipError_t launch_schur_split_nb(int n, int batch, const zd *A, zd *eig, 
     else
         hipLaunchKernelGGL((k_schur_hq<NB, LPM>), grid, dim3(64), 0, s, n, batch, A, eig, Tout, tstride, Uout, ustride,
                            flag);
-    hipLaunchKernelGGL((k_schur_qr<NB, LPM>), grid, dim3(64), 0, s, n, batch, eig, Tout, tstride, Uout, ustride, flag,
-                       itmax);
+    if (known)
+        hipLaunchKernelGGL((k_schur_qr<NB, LPM, true>), grid, dim3(64), 0, s, n, batch, eig, Tout, tstride, Uout,
+                           ustride, flag, itmax, known);
+    else
+        hipLaunchKernelGGL((k_schur_qr<NB, LPM>), grid, dim3(64), 0, s, n, batch, eig, Tout, tstride, Uout, ustride,
+                           flag, itmax, nullptr);
     if (vec)
         hipLaunchKernelGGL((k_vectors_from_schur<NB, LPM>), grid, dim3(64), 0, s, n, batch, Tout, VLv, VRv, flag);
     return hipGetLastError();
@@ -322,13 +329,14 @@ static hipError_t launch_schur_split_nb(int n, int batch, const zd *A, zd *eig, 
 
 static hipError_t launch_schur_split(int n, int batch, const zd *A, zd *eig, zd *Tout, long long tstride,
                                      zd *Uout, long long ustride, int8_t *flag, int itmax, hipStream_t s,
-                                     bool vec = false, zd *VLv = nullptr, zd *VRv = nullptr)
+                                     bool vec = false, zd *VLv = nullptr, zd *VRv = nullptr,
+                                     const zd *known = nullptr)
 {
     switch (split_nb(n)) {
 #define X(nb, lpm)                                                                                             \
     case nb:                                                                                                   \
         return launch_schur_split_nb<nb, lpm>(n, batch, A, eig, Tout, tstride, Uout, ustride, flag, itmax, s, vec, \
-                                              VLv, VRv);
+                                              VLv, VRv, known);
         TKZ_SPLIT_NB(X)
 #undef X
     }
@@ -393,13 +401,14 @@ static hipError_t launch_adjoint(int n, int batch, const zd *schur, const zd *gb
 
 // Recompute backward: the two-launch Schur forward with T parked in `work`
 // (n^2 complex per matrix, then the recomputed eigenvalues, n per matrix:
-// the reference's workspace size) and U in dA, then the matched adjoint.
+// the reference's workspace size) and U in dA, its QR shifts snapped to the
+// forward's eigenvalues (about half the sweeps), then the matched adjoint.
 static hipError_t launch_adjoint_recompute(int n, int batch, const zd *A, const zd *ef, const zd *gbar, zd *dA,
                                            zd *work, int8_t *flag, int itmax, hipStream_t s)
 {
     const long long nn = (long long)n * n;
     zd *eo = work + nn * batch;
-    hipError_t e = launch_schur_split(n, batch, A, eo, work, nn, dA, nn, flag, itmax, s);
+    hipError_t e = launch_schur_split(n, batch, A, eo, work, nn, dA, nn, flag, itmax, s, false, nullptr, nullptr, ef);
     if (e != hipSuccess) return e;
     if (n > 16) {
         switch (adj_wg_nb(n)) {

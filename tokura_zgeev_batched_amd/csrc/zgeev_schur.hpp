@@ -267,23 +267,32 @@ __device__ __forceinline__ bool schur_hq_one(zd *H, zd *taus, const zd *__restri
 // WANTT/WANTZ on the Hessenberg matrix from Th, Z = Ug updated in place
 // (lane p owns row p, as it wrote it), then the eigenvalues to out and T
 // (zeros below the diagonal) back to Th, row by row by the lane that read
-// it.  Returns the QR INFO.
-template <int LPM, bool DENSE = false>
-__device__ __forceinline__ int schur_qr_one(zd *H, int n, int t, zd *Th, zd *Ug, zd *__restrict__ out, int itmax)
+// it.  Returns the QR INFO.  SNAP: `known` holds the matrix's eigenvalues
+// (n, any order) and the QR's shifts are snapped to them (snap_shift): the
+// same Schur form up to rounding in about half the sweeps.
+template <int LPM, bool DENSE = false, bool SNAP = false>
+__device__ __forceinline__ int schur_qr_one(zd *H, int n, int t, zd *Th, zd *Ug, zd *__restrict__ out, int itmax,
+                                            const zd *__restrict__ known = nullptr)
 {
     const PackedLayout lay{};
     const int r = t;
     if (r < n)
         for (int c = r > 0 ? r - 1 : 0; c < n; ++c) H[lay(r, c)] = Th[r + c * n];
+    const bool kh = SNAP && r < n;
+    const zd kev = kh ? known[r] : zd{};
     lds_order();
     int info = 0;
     if constexpr (LPM == 64) {
         // DENSE: H has a spare element past the matrix for the idle lanes
-        if (schur_dual(n)) info = zlahqr_dual<LPM, true, DENSE>(H, lay, n, t, itmax, Ug, n);
+        if (schur_dual(n))
+            info = zlahqr_dual<LPM, true, DENSE, NoMove, SNAP>(H, lay, n, t, itmax, Ug, n, nullptr, -1, kev, kh);
         else if (n > 1)   // inlined: as a call, its ABI would raise the kernel's VGPRs to 274 and add stack
-            [[clang::always_inline]] info = zlahqr_core<LPM, 1, true, false>(H, lay, n, t, itmax, Ug, FullLayout{n});
+            [[clang::always_inline]] info = zlahqr_core<LPM, 1, true, false, PackedLayout, FullLayout, false, SNAP>(
+                H, lay, n, t, itmax, Ug, FullLayout{n}, 0, 0, kev, kh);
     } else {
-        if (n > 1) info = zlahqr_core<LPM, 1, true, false>(H, lay, n, t, itmax, Ug, FullLayout{n});
+        if (n > 1)
+            info = zlahqr_core<LPM, 1, true, false, PackedLayout, FullLayout, false, SNAP>(
+                H, lay, n, t, itmax, Ug, FullLayout{n}, 0, 0, kev, kh);
     }
     lds_order();
     if (r < n) {
