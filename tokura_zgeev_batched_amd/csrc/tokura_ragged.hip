@@ -428,13 +428,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 // The classes n > 32: (class, NB)
 #define TKZ_BIG_CLS(X) X(0, 64) X(1, 60) X(2, 56) X(3, 48) X(4, 40)
 
+// Side streams for the graph capture: the Hessenberg launches of the three
+// groups (n 49..64, n 33..48, the small classes) touch disjoint matrices, so
+// the captured graph runs them as three branches between the schedule check
+// and the QR, and their tails overlap.  Without side streams (the
+// tokura_zgeev_ragged_gpu call) everything runs in order on one stream.
+struct RaggedFork {
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // after reset; side 0 done; side 1 done
+    bool on() const { return side[0] != nullptr; }
+};
+
 template <int LA1, int LA2, int LB, int LC, int W, int MV>
 static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int bal, int cus,
-                                    const RaggedPrepared &pe, hipStream_t s)
+                                    const RaggedPrepared &pe, hipStream_t s, const RaggedFork &fk)
 {
     static_assert(LA1 == 64 && LA2 == 64, "classes n > 32: one matrix per wavefront");
     hipError_t e = launch_reset(work, pe.gen, pe.batch, flag, s);
     if (e != hipSuccess) return e;
+    hipStream_t s1 = s, s2 = s;   // streams of the n 33..48 and the small-class Hessenberg
+    if (fk.on() && bal == 0) {
+        if (hipEventRecord(fk.ev[0], s) != hipSuccess || hipStreamWaitEvent(fk.side[0], fk.ev[0], 0) != hipSuccess ||
+            hipStreamWaitEvent(fk.side[1], fk.ev[0], 0) != hipSuccess)
+            return hipErrorUnknown;
+        s1 = fk.side[0];
+        s2 = fk.side[1];
+    }
     // resident workgroups per CU, queried once (outside any graph capture the
     // first call is made from tokura_zgeev_ragged_graph's capture stream, where
     // occupancy queries are not stream operations and are allowed)
@@ -442,7 +461,7 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
         if (pe.n49_64 > 0)
             hipLaunchKernelGGL((k_ragged_hess_reg<4>), dim3(pe.n49_64), dim3(256), 0, s, A, work, flag, eig, 0, 2);
         if (pe.n33_48 > 0)
-            hipLaunchKernelGGL((k_ragged_hess_reg<3>), dim3(pe.n33_48), dim3(256), 0, s, A, work, flag, eig, 3, 4);
+            hipLaunchKernelGGL((k_ragged_hess_reg<3>), dim3(pe.n33_48), dim3(256), 0, s1, A, work, flag, eig, 3, 4);
     } else {
 #define X(c, nb)                                                                                          \
     {                                                                                                      \
@@ -455,20 +474,25 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     static const int occ_h = resident_blocks(k_ragged_hess<LB, LC>);
-    hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s, A, work, flag, bal, eig);
+    hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s2, A, work, flag, bal, eig);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (s1 != s) {   // join the branches
+        if (hipEventRecord(fk.ev[1], s1) != hipSuccess || hipEventRecord(fk.ev[2], s2) != hipSuccess ||
+            hipStreamWaitEvent(s, fk.ev[1], 0) != hipSuccess || hipStreamWaitEvent(s, fk.ev[2], 0) != hipSuccess)
+            return hipErrorUnknown;
+    }
     // one workgroup per CU: the arena is the CU's whole LDS
     hipLaunchKernelGGL((k_ragged_qr_pool<W, LB, LC, MV>), dim3(cus), dim3(64 * W), 0, s, work, eig, flag, itmax);
     return hipGetLastError();
 }
 
 static hipError_t launch_ragged(const RaggedPrepared &pe, const zd *A, char *work, zd *eig, int8_t *flag, int itmax,
-                                int bal, int cus, hipStream_t s)
+                                int bal, int cus, hipStream_t s, const RaggedFork &fk = RaggedFork())
 {
     switch (pe.cfg) {
 #define X(id, a1, a2, b, c, w, mv) \
-    case id: return launch_ragged_cfg<a1, a2, b, c, w, mv>(A, work, eig, flag, itmax, bal, cus, pe, s);
+    case id: return launch_ragged_cfg<a1, a2, b, c, w, mv>(A, work, eig, flag, itmax, bal, cus, pe, s, fk);
         TKZ_RAGGED_CFGS(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -898,16 +922,26 @@ extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, 
         G = RaggedGraphCache();
         hipStream_t cs;
         if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return 3;
-        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        // the capture stream and two side streams (default priority) with
+        // the fork/join events of the Hessenberg branches
+        RaggedFork fk;
+        bool fk_ok = true;
+        for (auto &x : fk.side) fk_ok = fk_ok && hipStreamCreateWithFlags(&x, hipStreamNonBlocking) == hipSuccess;
+        for (auto &x : fk.ev) fk_ok = fk_ok && hipEventCreateWithFlags(&x, hipEventDisableTiming) == hipSuccess;
+        hipError_t e = fk_ok ? hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) : hipErrorOutOfMemory;
         if (e == hipSuccess) {
             hipError_t el = launch_ragged(*pe, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
                                           handle->max_sweeps, handle->balance,
-                                          handle->cu_count > 0 ? handle->cu_count : 1, cs);
+                                          handle->cu_count > 0 ? handle->cu_count : 1, cs, fk);
             e = hipStreamEndCapture(cs, &G.graph);
             if (el != hipSuccess) e = el;
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0);
         (void)hipStreamDestroy(cs);
+        for (auto &x : fk.side)
+            if (x) (void)hipStreamDestroy(x);
+        for (auto &x : fk.ev)
+            if (x) (void)hipEventDestroy(x);
         if (e != hipSuccess) {
             if (G.graph) (void)hipGraphDestroy(G.graph);
             G = RaggedGraphCache();
