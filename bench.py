@@ -156,7 +156,7 @@ def bench_ragged(dev, B=200_000, lo=4, hi=64, steps=3, seed=1):
             "ragged_matrices_per_s": B / t_rag, "ragged_ms": t_rag * 1e3,
             "plain_per_bucket_matrices_per_s": B / t_plain, "plain_per_bucket_ms": t_plain * 1e3,
             "speedup_vs_plain_per_bucket": t_plain / t_rag,
-            "ragged_config": {"id": cfg, "lanes_per_matrix_A1_A2_B_C": lanes},
+            "ragged_config": {"id": cfg, "lanes_per_matrix_A1_A2_B_C": lanes, "hess_grid_multiplier": rb.hess_grid()},
             "model_tflops": flops / t_rag / 1e12, "flags_nonzero": nbad,
             "max_rel_diff_ragged_vs_plain_n64_sample": agree}
 

@@ -123,11 +123,18 @@ int tokuraGetStageTimes(tokurablas_t *handle, int *calls, float *hessenberg_ms, 
 int tokuraAutotune(tokurablas_t *handle, int n_lo, int n_hi, int batch_count,
                    const char *cache_path, hipStream_t stream);
 
-/* Loads a table written by tokuraAutotune (entries whose device key — arch
- * and CU count — differ are ignored).  Returns the number of entries read or
- * a negative value on error.  tokuraCreate already loads the file named by
- * the environment variable TOKURA_TUNE_CACHE when it is set. */
+/* Loads a table written by tokuraAutotune / tokuraSaveTuning (entries whose
+ * device key — arch and CU count — differ are ignored): the same-size
+ * variants and the ragged tuning table.  Returns the number of entries read
+ * or a negative value on error.  tokuraCreate already loads the file named
+ * by the environment variable TOKURA_TUNE_CACHE when it is set. */
 int tokuraLoadTuning(tokurablas_t *handle, const char *cache_path);
+
+/* Writes the handle's tuning (same-size variant per n, and the ragged table
+ * that tokuraAutotuneRagged fills: one line per batch size signature) to
+ * cache_path, keyed by the device.  Returns 0, -1 handle, -3 path NULL, 5 I/O
+ * error.  No counterpart in the reference, whose tables are compiled in. */
+int tokuraSaveTuning(tokurablas_t *handle, const char *cache_path);
 
 /* Sets the variant used for size n by hand (a variant id as reported by
  * tokuraGetLaunchConfig).  Returns 0 or -2 for an invalid n / variant. */
@@ -177,13 +184,24 @@ int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, const hipDo
  * (n 49..64), A2 (33..48), B (17..32), C (1..16) (a wavefront holds
  * 64/lanes matrices of a class) and wavefronts per workgroup of the
  * persistent QR (tokuraGetRaggedWaves).  tokuraAutotuneRagged times every
- * configuration on the caller's prepared batch (buffers as for
- * tokura_zgeev_ragged_gpu; the stream is synchronised), keeps the fastest
- * and re-prepares `work` for it. */
+ * configuration on the caller's batch (buffers as for
+ * tokura_zgeev_ragged_gpu; the stream is synchronised), then the small-class
+ * Hessenberg grid (1, 2, 4 or 8 workgroups per resident slot,
+ * tokuraGetRaggedHessGrid) with the fastest, keeps the winner, records it in
+ * the handle's tuning table under the batch's size signature (each size
+ * class's share of the batch and log2 of its size) and re-prepares `work`.
+ * From then on, and after tokuraLoadTuning of a cache with such entries,
+ * tokura_zgeev_ragged_prepare takes the configuration of a batch from the
+ * table when its signature is there; tokuraSetRaggedConfig switches the
+ * table off for the handle (a configuration set by hand wins). */
 int tokuraSetRaggedConfig(tokurablas_t *handle, int config);
 int tokuraGetRaggedConfig(tokurablas_t *handle, int *config, int *lanes_a1, int *lanes_a2, int *lanes_b,
                           int *lanes_c);
 int tokuraGetRaggedWaves(tokurablas_t *handle, int *waves);
+int tokuraGetRaggedHessGrid(tokurablas_t *handle, int *multiplier);
+/* The configuration and Hessenberg grid multiplier of the schedule that the
+ * last prepare wrote to `work` for batch_count matrices (-8 if none). */
+int tokuraGetRaggedPrepared(tokurablas_t *handle, void *work, int batch_count, int *config, int *multiplier);
 int tokuraAutotuneRagged(tokurablas_t *handle, int batch_count, const int64_t *offsets_host,
                          const hipDoubleComplex *A, hipDoubleComplex *eigenvalues, void *work,
                          size_t work_bytes, char *flag, hipStream_t stream);

@@ -98,6 +98,27 @@ def test_tuning_cache_roundtrip(tmp_path):
     L.tokuraDestroy(h)
 
 
+def test_tuning_cache_save_and_ragged_lines(tmp_path):
+    """tokuraSaveTuning writes the 64 same-size lines under the handle's
+    device key; ragged lines ("ragged arch cus signature config multiplier")
+    of another device, malformed ones and out-of-range configurations are
+    ignored by tokuraLoadTuning."""
+    L = tk.lib()
+    h = make_handle()
+    p = tmp_path / "save.txt"
+    assert L.tokuraSaveTuning(h, str(p).encode()) == 0
+    body = [ln for ln in p.read_text().splitlines() if not ln.startswith("#")]
+    assert len(body) == 64 and not any(ln.startswith("ragged ") for ln in body)
+    assert L.tokuraSaveTuning(None, str(p).encode()) == -1
+    assert L.tokuraSaveTuning(h, None) == -3
+    q = tmp_path / "ragged.txt"
+    q.write_text("ragged gfx950 256 10842108421 5 4\nragged gfx950 256 zz 5 4\nragged gfx950 256 1 99 4\n")
+    assert L.tokuraLoadTuning(h, str(q).encode()) == 0
+    m = ctypes.c_int()
+    assert L.tokuraGetRaggedHessGrid(h, ctypes.byref(m)) == 0 and m.value == 4
+    L.tokuraDestroy(h)
+
+
 def test_python_layer_refuses_host_tensors():
     torch = pytest.importorskip("torch")
     with pytest.raises(tk.TokuraError):

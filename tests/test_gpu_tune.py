@@ -7,6 +7,11 @@ which replaces the reference's build-time tuner
 * A fresh handle that loads the cache gets the same variant for every n, and
   its results are bit for bit those of a handle forced to that variant.
 * Lines for another device are ignored.
+* The ragged tuning table (tokuraAutotuneRagged, tokuraSaveTuning): the
+  chosen configuration and small-class Hessenberg grid are saved under the
+  batch's size signature, and a fresh handle that loads the cache prepares
+  that batch with them; a batch of another signature keeps the handle's
+  default, and a configuration set by hand wins over the table.
 """
 import numpy as np
 import pytest
@@ -64,3 +69,50 @@ def test_tuning_lines_of_other_devices_are_ignored(tmp_path):
     assert h.load_tuning(str(path)) == 1
     assert h.launch_config(33)[0] == 19 and h.launch_config(40)[0] == before
     h.close()
+
+
+def test_ragged_autotune_cache_roundtrip(tmp_path):
+    from tokura_zgeev_batched_amd.ragged import RaggedBatch
+    rng = np.random.default_rng(5)
+    sizes = rng.integers(4, 65, 6000)
+    offs = np.zeros(len(sizes) + 1, np.int64)
+    offs[1:] = np.cumsum(sizes.astype(np.int64) ** 2)
+    g = torch.Generator(device="cuda").manual_seed(21)
+    flat = torch.complex(torch.randn(int(offs[-1]), generator=g, device="cuda", dtype=torch.float64),
+                         torch.randn(int(offs[-1]), generator=g, device="cuda", dtype=torch.float64))
+    h = tk.Handle()
+    rb = RaggedBatch(offs, handle=h)
+    print("[case] ragged autotune 6000 matrices", flush=True)
+    cfg, _ = rb.autotune(flat)
+    hm = rb.hess_grid()
+    assert hm in (1, 2, 4, 8)
+    assert rb.prepared_config() == (cfg, hm)
+    e1, f1 = rb.solve(flat)
+    e1 = e1.clone()
+    path = str(tmp_path / "tune.txt")
+    h.save_tuning(path)
+    props = torch.cuda.get_device_properties(0)
+    arch = props.gcnArchName.split(":")[0]
+    rl = [ln.split() for ln in open(path) if ln.startswith("ragged ")]
+    assert len(rl) == 1
+    assert rl[0][1] == arch and int(rl[0][2]) == props.multi_processor_count
+    assert int(rl[0][4]) == cfg and int(rl[0][5]) == hm
+    h2 = tk.Handle()
+    assert h2.load_tuning(path) == 65
+    rb2 = RaggedBatch(offs, handle=h2)          # prepare takes the table's entry
+    assert rb2.prepared_config() == (cfg, hm)
+    e2, f2 = rb2.solve(flat, graph=True)
+    torch.cuda.synchronize()
+    assert int(f1.abs().sum()) == 0 and int(f2.abs().sum()) == 0
+    assert torch.equal(e1, e2)
+    # another size signature: the handle's default configuration
+    offs8 = np.arange(2001, dtype=np.int64) * 64
+    rb3 = RaggedBatch(offs8, handle=h2)
+    assert rb3.prepared_config() == (0, 4)
+    # a configuration set by hand wins over the table
+    alt = 9 if cfg != 9 else 5
+    assert tk.lib().tokuraSetRaggedConfig(h2.ptr, alt) == 0
+    rb2.prepare()
+    assert rb2.prepared_config()[0] == alt
+    h.close()
+    h2.close()

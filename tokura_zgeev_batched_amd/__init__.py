@@ -51,6 +51,7 @@ _SIGNATURES = {
     "tokuraGetLaunchConfig": ([_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "tokuraAutotune": ([_P, _I, _I, _I, ctypes.c_char_p, _P], _I),
     "tokuraLoadTuning": ([_P, ctypes.c_char_p], _I),
+    "tokuraSaveTuning": ([_P, ctypes.c_char_p], _I),
     "tokuraSetVariant": ([_P, _I, _I], _I),
     "tokuraSetStageTiming": ([_P, _I], _I),
     "tokuraGetStageTimes": ([_P, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)], _I),
@@ -58,6 +59,8 @@ _SIGNATURES = {
     "tokuraSetRaggedConfig": ([_P, _I], _I),
     "tokuraGetRaggedConfig": ([_P] + [ctypes.POINTER(_I)] * 5, _I),
     "tokuraGetRaggedWaves": ([_P, ctypes.POINTER(_I)], _I),
+    "tokuraGetRaggedHessGrid": ([_P, ctypes.POINTER(_I)], _I),
+    "tokuraGetRaggedPrepared": ([_P, _P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "tokuraAutotuneRagged": ([_P, _I, _P, _P, _P, _P, _SZ, _P, _P], _I),
     "tokura_zgeev_ragged_prepare": ([_P, _I, _P, _P, _SZ, _P], _I),
     "tokura_zgeev_ragged_gpu": ([_P, _I, _P, _P, _P, _P, _P], _I),
@@ -177,6 +180,10 @@ class Handle:
 
     def load_tuning(self, cache_path: str) -> int:
         return int(lib().tokuraLoadTuning(self._h, cache_path.encode()))
+
+    def save_tuning(self, cache_path: str):
+        """Writes the same-size table and the ragged tuning table, keyed by the device."""
+        _check(lib().tokuraSaveTuning(self._h, cache_path.encode()), "tokuraSaveTuning")
 
 
 _DEFAULT_HANDLE = None

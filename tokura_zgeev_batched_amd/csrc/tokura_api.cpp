@@ -231,6 +231,10 @@ extern "C" int tokuraLoadTuning(tokurablas_t *handle, const char *cache_path)
         char arch[64];
         int cus, n, v;
         if (line[0] == '#') continue;
+        if (std::strncmp(line, "ragged ", 7) == 0) {   // a ragged tuning-table entry
+            count += ragged_parse_tuning(handle, line);
+            continue;
+        }
         if (std::sscanf(line, "%63s %d %d %d", arch, &cus, &n, &v) != 4) continue;
         if (std::strcmp(arch, handle->arch) != 0 || cus != handle->cu_count) continue;
         if (n < 1 || n > TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE || !variant_fits(v, n)) continue;
@@ -298,13 +302,20 @@ extern "C" int tokuraAutotune(tokurablas_t *handle, int n_lo, int n_hi, int batc
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(dA); (void)hipFree(dE); (void)hipFree(dW); (void)hipFree(dF);
-    if (rc == 0 && cache_path != nullptr) {
-        FILE *f = std::fopen(cache_path, "w");
-        if (!f) return 5;
-        std::fprintf(f, "# arch cus n variant  (tokura_zgeev_batched_amd autotune cache)\n");
-        for (int n = 1; n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE; ++n)
-            std::fprintf(f, "%s %d %d %d\n", handle->arch, handle->cu_count, n, handle->variant_for_n[n]);
-        std::fclose(f);
-    }
+    if (rc == 0 && cache_path != nullptr && tokuraSaveTuning(handle, cache_path) != 0) return 5;
     return rc;
+}
+
+extern "C" int tokuraSaveTuning(tokurablas_t *handle, const char *cache_path)
+{
+    if (handle == nullptr) return -1;
+    if (cache_path == nullptr) return -3;
+    FILE *f = std::fopen(cache_path, "w");
+    if (!f) return 5;
+    std::fprintf(f, "# arch cus n variant  (tokura_zgeev_batched_amd autotune cache)\n");
+    std::fprintf(f, "# ragged arch cus signature config hess_grid_multiplier\n");
+    for (int n = 1; n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE; ++n)
+        std::fprintf(f, "%s %d %d %d\n", handle->arch, handle->cu_count, n, handle->variant_for_n[n]);
+    ragged_write_tuning(handle, f);
+    return std::fclose(f) == 0 ? 0 : 5;
 }

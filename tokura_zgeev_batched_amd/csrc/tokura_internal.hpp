@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #include <array>
 #include <vector>
@@ -24,10 +25,20 @@ struct RaggedGraphCache {
 struct RaggedPrepared {
     void *work;
     int cfg;
+    int hmul;     // small-class Hessenberg grid: CUs x resident blocks x hmul workgroups
     int batch;
     int gen;      // the prepare that wrote the schedule (also in its header)
     int n49_64;   // matrices with 49 <= n <= 64 (size classes 0-2)
     int n33_48;   // matrices with 33 <= n <= 48 (size classes 3-4)
+};
+
+// A tuned ragged configuration for batches with one size signature
+// (ragged_signature: the batch's share of each size class in sixteenths and
+// log2 of its size).
+struct RaggedTune {
+    unsigned long long sig;
+    int cfg;
+    int hmul;
 };
 
 struct tokurablas_t {
@@ -39,6 +50,9 @@ struct tokurablas_t {
     int balance = 0;                                          // ZGEBAL job bits (1 permute, 2 scale)
     int variant_for_n[TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE + 1];  // same-size path
     int ragged_cfg = 0;                                       // ragged class configuration
+    int ragged_hmul = 4;                                      // small-class Hessenberg grid multiplier
+    bool ragged_auto = false;                                 // prepare looks up ragged_tuned first
+    std::vector<RaggedTune> ragged_tuned;                     // from tokuraAutotuneRagged / a cache
     RaggedGraphCache graph;
     void *ragged_host_stage = nullptr;                        // pinned staging of the schedule
     size_t ragged_host_bytes = 0;
@@ -66,6 +80,10 @@ int resident_blocks(K k, int threads = 64)
 }
 
 void ragged_release(tokurablas_t *h);
+// Writes the ragged lines of a tuning cache ("ragged arch cus sig cfg hmul");
+// parses one such line into the handle (returns 1 if it was taken).
+void ragged_write_tuning(const tokurablas_t *h, FILE *f);
+int ragged_parse_tuning(tokurablas_t *h, const char *line);
 // Looks up the no-allocation entry's scratch array on device `dev` (current).
 void noalloc_setup(int dev);
 int ragged_num_configs();

@@ -474,7 +474,7 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     static const int occ_h = resident_blocks(k_ragged_hess<LB, LC>);
-    hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * 4), dim3(64), 0, s2, A, work, flag, bal, eig);
+    hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * pe.hmul), dim3(64), 0, s2, A, work, flag, bal, eig);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (s1 != s) {   // join the branches
@@ -820,6 +820,41 @@ extern "C" size_t tokura_get_zgeev_ragged_worksize(int batch_count, const int64_
     return (size_t)worst;
 }
 
+// The size signature of a batch, the key of the ragged tuning table: each
+// size class's share of the batch in sixteenths (5 bits per class) and
+// floor(log2(batch)) (bits 35..39).  Entries that do not describe a
+// matrix are skipped (the schedule builder rejects such offsets).
+static unsigned long long ragged_signature(int batch, const int64_t *offsets)
+{
+    long long cnt[kNumCls] = {0};
+    for (int b = 0; b < batch; ++b) {
+        const long long d = (long long)offsets[b + 1] - (long long)offsets[b];
+        const long long n = d > 0 ? (long long)std::llround(std::sqrt((double)d)) : 0;
+        if (n >= 1 && n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE && n * n == d) ++cnt[cls_of((int)n)];
+    }
+    unsigned long long sig = 0;
+    for (int c = 0; c < kNumCls; ++c) sig |= (unsigned long long)((16 * cnt[c] + batch / 2) / batch) << (5 * c);
+    int lb = 0;
+    while ((2LL << lb) <= (long long)batch) ++lb;
+    return sig | (unsigned long long)lb << 35;
+}
+
+// The configuration for this batch: the tuned entry of its signature when
+// the handle has a tuning table (tokuraAutotuneRagged or a cache file) and
+// no configuration was set by hand, else the handle's current one.
+static void ragged_choose(const tokurablas_t *h, int batch, const int64_t *offsets, int &cfg, int &hmul)
+{
+    cfg = h->ragged_cfg;
+    hmul = h->ragged_hmul;
+    if (!h->ragged_auto) return;
+    const unsigned long long sig = ragged_signature(batch, offsets);
+    for (const auto &t : h->ragged_tuned)
+        if (t.sig == sig) {
+            cfg = t.cfg;
+            hmul = t.hmul;
+        }
+}
+
 extern "C" int tokura_zgeev_ragged_prepare(tokurablas_t *handle, int batch_count, const int64_t *offsets_host,
                                            void *work, size_t work_bytes, hipStream_t stream)
 {
@@ -828,7 +863,8 @@ extern "C" int tokura_zgeev_ragged_prepare(tokurablas_t *handle, int batch_count
     if (offsets_host == nullptr) return -3;
     if (work == nullptr) return -5;
     Schedule S;
-    const int cfg = handle->ragged_cfg;
+    int cfg, hmul;
+    ragged_choose(handle, batch_count, offsets_host, cfg, hmul);
     if (build_schedule(batch_count, offsets_host, cfg, S) != 0) return -3;
     if ((long long)work_bytes < S.hdr.total_bytes) return -5;
     const size_t meta = (size_t)S.hdr.off_H;
@@ -867,7 +903,8 @@ extern "C" int tokura_zgeev_ragged_prepare(tokurablas_t *handle, int batch_count
         handle->ragged_staged = true;
     }
     // remember which configuration and generation this work buffer holds
-    const RaggedPrepared rp{work, cfg, batch_count, S.hdr.gen, S.hdr.cls_end[2], S.hdr.cls_end[4] - S.hdr.cls_end[2]};
+    const RaggedPrepared rp{work, cfg, hmul, batch_count, S.hdr.gen, S.hdr.cls_end[2],
+                            S.hdr.cls_end[4] - S.hdr.cls_end[2]};
     bool found = false;
     for (auto &p : handle->prepared)
         if (p.work == work) { p = rp; found = true; }
@@ -960,6 +997,7 @@ extern "C" int tokuraSetRaggedConfig(tokurablas_t *handle, int cfg)
     if (handle == nullptr) return -1;
     if (cfg < 0 || cfg >= kNumRaggedCfg) return -2;
     handle->ragged_cfg = cfg;
+    handle->ragged_auto = false;   // a configuration set by hand wins over the tuning table
     return 0;
 }
 
@@ -983,36 +1021,121 @@ extern "C" int tokuraGetRaggedWaves(tokurablas_t *handle, int *waves)
     return 0;
 }
 
+extern "C" int tokuraGetRaggedPrepared(tokurablas_t *handle, void *work, int batch_count, int *config,
+                                       int *multiplier)
+{
+    if (handle == nullptr) return -1;
+    const RaggedPrepared *pe = prepared_entry(handle, work, batch_count);
+    if (pe == nullptr) return -8;
+    if (config) *config = pe->cfg;
+    if (multiplier) *multiplier = pe->hmul;
+    return 0;
+}
+
+extern "C" int tokuraGetRaggedHessGrid(tokurablas_t *handle, int *multiplier)
+{
+    if (handle == nullptr) return -1;
+    if (multiplier) *multiplier = handle->ragged_hmul;
+    return 0;
+}
+
+// Milliseconds of one ragged call (the second of two) with the handle's
+// current configuration, re-preparing `work` first; -1 on an error (rc set).
+static float time_ragged(tokurablas_t *h, int batch, const int64_t *offs, const hipDoubleComplex *A,
+                         hipDoubleComplex *eig, void *work, size_t work_bytes, char *flag, hipStream_t s,
+                         hipEvent_t e0, hipEvent_t e1, int &rc)
+{
+    rc = tokura_zgeev_ragged_prepare(h, batch, offs, work, work_bytes, s);
+    if (rc) return -1.f;
+    float ms = 0.f;
+    for (int rep = 0; rep < 2 && rc == 0; ++rep) {
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = 3; break; }
+        rc = tokura_zgeev_ragged_gpu(h, batch, A, eig, work, flag, s);
+        if (rc == 0 && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = 3;
+    }
+    return rc ? -1.f : ms;
+}
+
+// Two passes over the caller's batch: every class configuration at the
+// current Hessenberg grid multiplier, then the multipliers 1, 2, 4, 8 of the
+// small-class Hessenberg grid with the fastest configuration.  The winner
+// becomes the handle's configuration and the tuning-table entry of the
+// batch's size signature (tokuraSaveTuning writes the table).
 extern "C" int tokuraAutotuneRagged(tokurablas_t *handle, int batch_count, const int64_t *offsets_host,
                                     const hipDoubleComplex *A, hipDoubleComplex *eigenvalues, void *work,
                                     size_t work_bytes, char *flag, hipStream_t stream)
 {
     if (handle == nullptr) return -1;
-    const int keep = handle->ragged_cfg;
+    if (batch_count <= 0) return -6;
+    if (offsets_host == nullptr) return -3;
+    const int keep = handle->ragged_cfg, keep_h = handle->ragged_hmul;
+    const bool keep_auto = handle->ragged_auto;
+    handle->ragged_auto = false;   // time the configurations set below
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float best = 1e30f;
-    int bestc = keep, rc = 0;
+    int bestc = keep, besth = keep_h, rc = 0;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = 3;
     for (int c = 0; c < kNumRaggedCfg && rc == 0; ++c) {
         handle->ragged_cfg = c;
-        rc = tokura_zgeev_ragged_prepare(handle, batch_count, offsets_host, work, work_bytes, stream);
-        if (rc) break;
-        float ms = 0.f;
-        for (int rep = 0; rep < 2 && rc == 0; ++rep) {
-            if (hipEventRecord(e0, stream) != hipSuccess) { rc = 3; break; }
-            rc = tokura_zgeev_ragged_gpu(handle, batch_count, A, eigenvalues, work, flag, stream);
-            if (rc == 0 && (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                rc = 3;
-        }
+        const float ms = time_ragged(handle, batch_count, offsets_host, A, eigenvalues, work, work_bytes, flag,
+                                     stream, e0, e1, rc);
         if (rc == 0 && ms < best) { best = ms; bestc = c; }
+    }
+    handle->ragged_cfg = bestc;
+    for (int hm = 1; hm <= 8 && rc == 0; hm *= 2) {
+        if (hm == keep_h) continue;   // timed above
+        handle->ragged_hmul = hm;
+        const float ms = time_ragged(handle, batch_count, offsets_host, A, eigenvalues, work, work_bytes, flag,
+                                     stream, e0, e1, rc);
+        if (rc == 0 && ms < best) { best = ms; besth = hm; }
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    handle->ragged_cfg = rc == 0 ? bestc : keep;
+    if (rc == 0) {
+        handle->ragged_cfg = bestc;
+        handle->ragged_hmul = besth;
+        const unsigned long long sig = ragged_signature(batch_count, offsets_host);
+        bool found = false;
+        for (auto &t : handle->ragged_tuned)
+            if (t.sig == sig) { t.cfg = bestc; t.hmul = besth; found = true; }
+        if (!found) handle->ragged_tuned.push_back(RaggedTune{sig, bestc, besth});
+        handle->ragged_auto = true;
+    } else {
+        handle->ragged_cfg = keep;
+        handle->ragged_hmul = keep_h;
+        handle->ragged_auto = keep_auto;
+    }
     int rc2 = tokura_zgeev_ragged_prepare(handle, batch_count, offsets_host, work, work_bytes, stream);
     if (rc2 == 0 && hipStreamSynchronize(stream) != hipSuccess) rc2 = 3;
     return rc ? rc : rc2;
+}
+
+void ragged_write_tuning(const tokurablas_t *h, FILE *f)
+{
+    for (const auto &t : h->ragged_tuned)
+        std::fprintf(f, "ragged %s %d %llx %d %d\n", h->arch, h->cu_count, t.sig, t.cfg, t.hmul);
+}
+
+int ragged_parse_tuning(tokurablas_t *h, const char *line)
+{
+    char arch[64];
+    int cus, cfg, hmul;
+    unsigned long long sig;
+    if (std::sscanf(line, "ragged %63s %d %llx %d %d", arch, &cus, &sig, &cfg, &hmul) != 5) return 0;
+    if (std::strcmp(arch, h->arch) != 0 || cus != h->cu_count) return 0;
+    if (cfg < 0 || cfg >= kNumRaggedCfg || !(hmul == 1 || hmul == 2 || hmul == 4 || hmul == 8)) return 0;
+    for (auto &t : h->ragged_tuned)
+        if (t.sig == sig) {
+            t.cfg = cfg;
+            t.hmul = hmul;
+            h->ragged_auto = true;
+            return 1;
+        }
+    h->ragged_tuned.push_back(RaggedTune{sig, cfg, hmul});
+    h->ragged_auto = true;
+    return 1;
 }
 
 extern "C" size_t tokura_get_zgeev_ragged_schur_worksize(int batch_count, const int64_t *offsets_host)

@@ -116,6 +116,19 @@ class RaggedBatch:
         _check(lib().tokuraGetRaggedConfig(self.h.ptr, *[ctypes.byref(x) for x in v]), "tokuraGetRaggedConfig")
         return v[0].value, tuple(x.value for x in v[1:])
 
+    def prepared_config(self):
+        """(configuration, Hessenberg grid multiplier) of the schedule prepared in this batch's workspace."""
+        c, m = ctypes.c_int(), ctypes.c_int()
+        _check(lib().tokuraGetRaggedPrepared(self.h.ptr, _P(self.work.data_ptr()), self.batch, ctypes.byref(c),
+                                             ctypes.byref(m)), "tokuraGetRaggedPrepared")
+        return c.value, m.value
+
+    def hess_grid(self):
+        """Small-class Hessenberg grid multiplier of the handle (workgroups per resident slot)."""
+        m = ctypes.c_int()
+        _check(lib().tokuraGetRaggedHessGrid(self.h.ptr, ctypes.byref(m)), "tokuraGetRaggedHessGrid")
+        return m.value
+
     def split(self, eig_flat: torch.Tensor):
         """List of per-matrix eigenvalue tensors."""
         return [eig_flat[self.eig_offsets[b]:self.eig_offsets[b + 1]] for b in range(self.batch)]
