@@ -57,6 +57,8 @@ struct tokurablas_t {
     void *ragged_host_stage = nullptr;                        // pinned staging of the schedule
     size_t ragged_host_bytes = 0;
     hipEvent_t ragged_stage_event = nullptr;                  // after the last staging copy
+    hipStream_t ragged_side[3] = {nullptr, nullptr, nullptr}; // branches of the per-class launches
+    hipEvent_t ragged_fork_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // fork, then one join per branch
     bool ragged_staged = false;                               // that copy may still be in flight
     int ragged_gen = 0;                                       // prepares so far
     std::vector<RaggedPrepared> prepared;

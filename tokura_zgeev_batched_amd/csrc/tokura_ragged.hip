@@ -428,16 +428,57 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 // The classes n > 32: (class, NB)
 #define TKZ_BIG_CLS(X) X(0, 64) X(1, 60) X(2, 56) X(3, 48) X(4, 40)
 
-// Side streams for the graph capture: the Hessenberg launches of the three
-// groups (n 49..64, n 33..48, the small classes) touch disjoint matrices, so
-// the captured graph runs them as three branches between the schedule check
-// and the QR, and their tails overlap.  Without side streams (the
-// tokura_zgeev_ragged_gpu call) everything runs in order on one stream.
+// Branches for launches that touch disjoint matrices (the Hessenberg
+// groups of the eigenvalue path, the size classes of the Schur forward and
+// of the backward): side streams fork from the caller's stream after the
+// schedule check and join it again, so the launches' tails overlap.
+// Without side streams everything runs in order on the caller's stream.
 struct RaggedFork {
-    hipStream_t side[2] = {nullptr, nullptr};
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // after reset; side 0 done; side 1 done
-    bool on() const { return side[0] != nullptr; }
+    int nside = 0;
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
+    bool on() const { return nside > 0; }
+    // the stream of branch i (0: the caller's)
+    hipStream_t branch(int i, hipStream_t s) const { return (i == 0 || !on()) ? s : side[(i - 1) % nside]; }
+    // the side streams wait for what is queued on s so far
+    hipError_t begin(hipStream_t s) const
+    {
+        if (!on()) return hipSuccess;
+        if (hipEventRecord(fork, s) != hipSuccess) return hipErrorUnknown;
+        for (int i = 0; i < nside; ++i)
+            if (hipStreamWaitEvent(side[i], fork, 0) != hipSuccess) return hipErrorUnknown;
+        return hipSuccess;
+    }
+    // s waits for what is queued on the side streams
+    hipError_t end(hipStream_t s) const
+    {
+        if (!on()) return hipSuccess;
+        for (int i = 0; i < nside; ++i)
+            if (hipEventRecord(join[i], side[i]) != hipSuccess || hipStreamWaitEvent(s, join[i], 0) != hipSuccess)
+                return hipErrorUnknown;
+        return hipSuccess;
+    }
 };
+
+// The handle's side streams and events (created on first use, default
+// priority); an empty fork (serial launches) if they cannot be created.
+static RaggedFork handle_fork(tokurablas_t *h)
+{
+    RaggedFork fk;
+    bool ok = true;
+    for (auto &x : h->ragged_side)
+        ok = ok && (x != nullptr || hipStreamCreateWithFlags(&x, hipStreamNonBlocking) == hipSuccess);
+    for (auto &x : h->ragged_fork_ev)
+        ok = ok && (x != nullptr || hipEventCreateWithFlags(&x, hipEventDisableTiming) == hipSuccess);
+    if (!ok) return fk;
+    fk.nside = 3;
+    for (int i = 0; i < 3; ++i) {
+        fk.side[i] = h->ragged_side[i];
+        fk.join[i] = h->ragged_fork_ev[1 + i];
+    }
+    fk.fork = h->ragged_fork_ev[0];
+    return fk;
+}
 
 template <int LA1, int LA2, int LB, int LC, int W, int MV>
 static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int bal, int cus,
@@ -447,12 +488,11 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     hipError_t e = launch_reset(work, pe.gen, pe.batch, flag, s);
     if (e != hipSuccess) return e;
     hipStream_t s1 = s, s2 = s;   // streams of the n 33..48 and the small-class Hessenberg
-    if (fk.on() && bal == 0) {
-        if (hipEventRecord(fk.ev[0], s) != hipSuccess || hipStreamWaitEvent(fk.side[0], fk.ev[0], 0) != hipSuccess ||
-            hipStreamWaitEvent(fk.side[1], fk.ev[0], 0) != hipSuccess)
-            return hipErrorUnknown;
-        s1 = fk.side[0];
-        s2 = fk.side[1];
+    const bool fork = fk.on() && bal == 0;
+    if (fork) {
+        if (fk.begin(s) != hipSuccess) return hipErrorUnknown;
+        s1 = fk.branch(1, s);
+        s2 = fk.branch(2, s);
     }
     // resident workgroups per CU, queried once (outside any graph capture the
     // first call is made from tokura_zgeev_ragged_graph's capture stream, where
@@ -477,11 +517,7 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     hipLaunchKernelGGL((k_ragged_hess<LB, LC>), dim3(cus * occ_h * pe.hmul), dim3(64), 0, s2, A, work, flag, bal, eig);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (s1 != s) {   // join the branches
-        if (hipEventRecord(fk.ev[1], s1) != hipSuccess || hipEventRecord(fk.ev[2], s2) != hipSuccess ||
-            hipStreamWaitEvent(s, fk.ev[1], 0) != hipSuccess || hipStreamWaitEvent(s, fk.ev[2], 0) != hipSuccess)
-            return hipErrorUnknown;
-    }
+    if (fork && fk.end(s) != hipSuccess) return hipErrorUnknown;   // join the branches
     // one workgroup per CU: the arena is the CU's whole LDS
     hipLaunchKernelGGL((k_ragged_qr_pool<W, LB, LC, MV>), dim3(cus), dim3(64 * W), 0, s, work, eig, flag, itmax);
     return hipGetLastError();
@@ -670,21 +706,31 @@ static void launch_ragged_schur_cls(const zd *A, char *work, zd *eig, zd *schur,
     hipLaunchKernelGGL((k_ragged_schur_qr<NB, LPM, CLS>), dim3(cus * oq), dim3(64), 0, s, work, eig, schur, flag, itmax);
 }
 
+// The size classes' launch pairs (Hessenberg + Q, then the QR) on four
+// branches; each class keeps its two launches in order on its branch and
+// its own queue head.
 static hipError_t launch_ragged_schur(const zd *A, char *work, zd *eig, zd *schur, int8_t *flag, int itmax, int cus,
-                                      int gen, int batch, hipStream_t s)
+                                      int gen, int batch, hipStream_t s, const RaggedFork &fk = RaggedFork())
 {
     hipError_t e = launch_reset(work, gen, batch, flag, s);
     if (e != hipSuccess) return e;
-    launch_ragged_schur_cls<64, 64, 0>(A, work, eig, schur, flag, itmax, cus, s);
-    launch_ragged_schur_cls<60, 64, 1>(A, work, eig, schur, flag, itmax, cus, s);
-    launch_ragged_schur_cls<56, 64, 2>(A, work, eig, schur, flag, itmax, cus, s);
-    launch_ragged_schur_cls<48, 64, 3>(A, work, eig, schur, flag, itmax, cus, s);
-    launch_ragged_schur_cls<40, 64, 4>(A, work, eig, schur, flag, itmax, cus, s);
-    launch_ragged_schur_cls<32, 32, kClsB>(A, work, eig, schur, flag, itmax, cus, s);
-    launch_ragged_schur_cls<16, 16, kClsC>(A, work, eig, schur, flag, itmax, cus, s);
-    return hipGetLastError();
+    if (fk.begin(s) != hipSuccess) return hipErrorUnknown;
+    launch_ragged_schur_cls<64, 64, 0>(A, work, eig, schur, flag, itmax, cus, fk.branch(0, s));
+    launch_ragged_schur_cls<60, 64, 1>(A, work, eig, schur, flag, itmax, cus, fk.branch(1, s));
+    launch_ragged_schur_cls<56, 64, 2>(A, work, eig, schur, flag, itmax, cus, fk.branch(2, s));
+    launch_ragged_schur_cls<48, 64, 3>(A, work, eig, schur, flag, itmax, cus, fk.branch(3, s));
+    launch_ragged_schur_cls<40, 64, 4>(A, work, eig, schur, flag, itmax, cus, fk.branch(2, s));
+    launch_ragged_schur_cls<32, 32, kClsB>(A, work, eig, schur, flag, itmax, cus, fk.branch(1, s));
+    launch_ragged_schur_cls<16, 16, kClsC>(A, work, eig, schur, flag, itmax, cus, fk.branch(0, s));
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return fk.end(s);
 }
 
+// The backward's five launches stay in order on the caller's stream: run as
+// branches they took 38.7 ms against 35.5 ms in series on the hard-spectra
+// batch (the n 49..64 launch, one workgroup per CU, lost its slots to the
+// smaller ones).
 static hipError_t launch_ragged_adjoint(const zd *schur, const zd *gbar, zd *dA, char *work, int8_t *flag, int itmax,
                                         int cus, int gen, int batch, hipStream_t s)
 {
@@ -800,6 +846,12 @@ void ragged_release(tokurablas_t *h)
     h->ragged_host_bytes = 0;
     if (h->ragged_stage_event) (void)hipEventDestroy(h->ragged_stage_event);
     h->ragged_stage_event = nullptr;
+    for (auto &x : h->ragged_side)
+        if (x) (void)hipStreamDestroy(x);
+    for (auto &x : h->ragged_fork_ev)
+        if (x) (void)hipEventDestroy(x);
+    for (auto &x : h->ragged_side) x = nullptr;
+    for (auto &x : h->ragged_fork_ev) x = nullptr;
     h->prepared.clear();
 }
 
@@ -962,9 +1014,12 @@ extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, 
         // the capture stream and two side streams (default priority) with
         // the fork/join events of the Hessenberg branches
         RaggedFork fk;
+        fk.nside = 2;
         bool fk_ok = true;
-        for (auto &x : fk.side) fk_ok = fk_ok && hipStreamCreateWithFlags(&x, hipStreamNonBlocking) == hipSuccess;
-        for (auto &x : fk.ev) fk_ok = fk_ok && hipEventCreateWithFlags(&x, hipEventDisableTiming) == hipSuccess;
+        for (int i = 0; i < 2; ++i)
+            fk_ok = fk_ok && hipStreamCreateWithFlags(&fk.side[i], hipStreamNonBlocking) == hipSuccess &&
+                    hipEventCreateWithFlags(&fk.join[i], hipEventDisableTiming) == hipSuccess;
+        fk_ok = fk_ok && hipEventCreateWithFlags(&fk.fork, hipEventDisableTiming) == hipSuccess;
         hipError_t e = fk_ok ? hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) : hipErrorOutOfMemory;
         if (e == hipSuccess) {
             hipError_t el = launch_ragged(*pe, (const zd *)A, (char *)work, (zd *)eigenvalues, (int8_t *)flag,
@@ -977,8 +1032,9 @@ extern "C" int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, 
         (void)hipStreamDestroy(cs);
         for (auto &x : fk.side)
             if (x) (void)hipStreamDestroy(x);
-        for (auto &x : fk.ev)
+        for (auto &x : fk.join)
             if (x) (void)hipEventDestroy(x);
+        if (fk.fork) (void)hipEventDestroy(fk.fork);
         if (e != hipSuccess) {
             if (G.graph) (void)hipGraphDestroy(G.graph);
             G = RaggedGraphCache();
@@ -1167,7 +1223,7 @@ extern "C" int tokura_zgeev_ragged_schur_gpu(tokurablas_t *handle, int batch_cou
     if (pe == nullptr) return -8;
     hipError_t e = launch_ragged_schur((const zd *)A, (char *)work, (zd *)eigenvalues, (zd *)schur, (int8_t *)flag,
                                        handle->max_sweeps, handle->cu_count > 0 ? handle->cu_count : 1, pe->gen,
-                                       batch_count, stream);
+                                       batch_count, stream, handle_fork(handle));
     return e == hipSuccess ? 0 : 2;
 }
 
