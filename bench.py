@@ -104,7 +104,7 @@ def timed(fn, steps: int, warmup: int) -> float:
     return e0.elapsed_time(e1) / 1e3 / steps
 
 
-def bench_ragged(dev, B=200_000, lo=4, hi=64, steps=3, seed=1):
+def bench_ragged(dev, B=200_000, lo=4, hi=64, steps=5, seed=1):
     """Mixed sizes n ~ U{lo..hi}: persistent work-queue kernel (hipGraph) vs
     the same-size path called once per size bucket on the same matrices."""
     from tokura_zgeev_batched_amd.ragged import RaggedBatch

@@ -1,6 +1,7 @@
 """GPU parity of the pool variants of the same-size path (the two-bulge QR on
 a persistent workgroup per CU whose LDS is an arena of slots; matrices move
-to smaller slots as they deflate).
+to smaller slots as they deflate) and of the chunk variants (37-39: the LDS
+cut into chunks that the deflating matrices compact at every deflation).
 
 The arithmetic is that of variant 20 (the 256-thread Hessenberg and the
 two-bulge QR in a fixed LDS buffer), which the oracle's restatement of the
@@ -18,7 +19,7 @@ import tokura_zgeev_batched_amd as tk
 
 pytestmark = pytest.mark.gpu
 
-POOL_VARIANTS = list(range(29, 37))
+POOL_VARIANTS = list(range(29, 40))
 
 
 def randn_colmajor(B, n, seed):

@@ -39,10 +39,11 @@ int default_variant(int n)
     if (n <= 40) return 24;  // split, one matrix per wavefront, two bulges per sweep, LDS for n = 40
     if (n <= 48) return 19;  // the same with LDS for n = 48
     if (n <= 56) return 23;  // n = 56
-    // n > 56: the pool variant (one persistent workgroup per CU, matrices
-    // moving to smaller LDS slots as they deflate), layout 0; the fixed
-    // 34 KB buffers of variant 20 hold one matrix per SIMD at n = 64
-    return 29;
+    // n > 56: the chunk variant with 8 wavefronts (one persistent workgroup
+    // per CU whose LDS the deflating matrices keep compacted): 5.6% faster
+    // than the best pool layout at n = 64 x 100,000; the fixed 34 KB buffers
+    // of variant 20 hold one matrix per SIMD at n = 64
+    return 37;
 }
 
 bool variant_fits(int v, int n)

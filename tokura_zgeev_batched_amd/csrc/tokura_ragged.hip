@@ -74,22 +74,23 @@ static_assert(sizeof(RaggedHeader) == 184, "header size");
 
 // class configurations: lanes per matrix of the classes n > 48, 33..48 (one
 // matrix per wavefront), B (17..32), C (1..16); wavefronts per workgroup of
-// the persistent QR; a deflating matrix moves to a smaller LDS run when its
-// active part needs at most mv/8 of its run
-#define TKZ_RAGGED_CFGS(X)      \
-    X(0, 64, 64, 16, 8, 12, 5)  \
-    X(1, 64, 64, 32, 8, 12, 5)  \
-    X(2, 64, 64, 32, 16, 12, 5) \
-    X(3, 64, 64, 16, 16, 12, 5) \
-    X(4, 64, 64, 32, 16, 8, 5)  \
-    X(5, 64, 64, 32, 8, 8, 5)   \
-    X(6, 64, 64, 32, 8, 8, 4)   \
-    X(7, 64, 64, 32, 16, 8, 6)  \
-    X(8, 64, 64, 32, 32, 12, 5) \
-    X(9, 64, 64, 32, 32, 8, 5)  \
+// the persistent QR; the deflating matrices' mover (ChunkMover): 0 compacts
+// the arena at every deflation, 4..7 moves a matrix to the highest free run
+// when its active part needs at most mv/8 of its run
+#define TKZ_RAGGED_CFGS(X)       \
+    X(0, 64, 64, 16, 8, 12, 0)   \
+    X(1, 64, 64, 32, 8, 12, 0)   \
+    X(2, 64, 64, 32, 16, 12, 0)  \
+    X(3, 64, 64, 32, 32, 12, 0)  \
+    X(4, 64, 64, 32, 16, 8, 0)   \
+    X(5, 64, 64, 32, 8, 8, 5)    \
+    X(6, 64, 64, 32, 8, 8, 0)    \
+    X(7, 64, 64, 32, 16, 8, 6)   \
+    X(8, 64, 64, 32, 16, 12, 5)  \
+    X(9, 64, 64, 32, 32, 8, 5)   \
     X(10, 64, 64, 32, 32, 12, 6) \
-    X(11, 64, 64, 32, 8, 8, 6)  \
-    X(12, 64, 64, 32, 32, 12, 4)
+    X(11, 64, 64, 32, 8, 8, 6)   \
+    X(12, 64, 64, 32, 8, 12, 5)
 static const int g_cfg_lpm[][6] = {
 #define X(id, a1, a2, b, c, w, mv) {a1, a2, b, c, w, mv},
     TKZ_RAGGED_CFGS(X)
@@ -283,10 +284,10 @@ __global__ __launch_bounds__(64 * W) void k_ragged_qr_pool(char *work, zd *__res
             const zd *src = V.H + V.hoff[b];
             for (int e = lane; e < packed_size(n); e += 64) H[e] = src[e];
             lds_order();
-            ChunkMover mv{&P, out, s0, k, n, H, us, MV};
+            ChunkMover<MV> mv{&P, out, s0, k, n, H, us};
             int info = 0;
             if (n > 1)
-                info = zlahqr_dual<64, false, true, ChunkMover>(H, PackedLayout{}, n, lane, itmax, nullptr, 0, &mv,
+                info = zlahqr_dual<64, false, true, ChunkMover<MV>>(H, PackedLayout{}, n, lane, itmax, nullptr, 0, &mv,
                                                                 k * kChunk - 1);
             const PackedLayout pk;
             if (lane < mv.whi) out[lane] = mv.eig(mv.H[pk(lane, lane)]);   // whi <= n <= 64

@@ -249,6 +249,58 @@ __global__ __launch_bounds__(64 * W) void k_qr_pool(int n, int batch, const zd *
     }
 }
 
+// Stage 2 of the chunk variants: one persistent workgroup of W wavefronts per
+// CU whose LDS is a chunked arena (ChunkPool).  Each wavefront claims the
+// next matrix from *qctr (zeroed by the Hessenberg kernel), waits for the
+// lowest run of chunks that holds it and runs the two-bulge QR; after every
+// deflation the matrix moves to the lowest run that holds its active part
+// (ChunkMover<0>), so the arena stays compacted and more matrices are
+// resident than in fixed slots.  Eigenvalues are those of
+// k_qr_packed<NB, 64, true> bit for bit.  A wavefront waits only while it
+// holds no chunks; every wavefront leaves the loop when the counter passes
+// the batch.
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_qr_chunk(int n, int batch, const zd *__restrict__ work,
+                                                     zd *__restrict__ eig, int8_t *__restrict__ flag, int itmax,
+                                                     int *qctr)
+{
+    __shared__ zd arena[kChunks * kChunk];
+    __shared__ unsigned long long bits[kBitWords];
+    __shared__ unsigned lock, freed;
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x < kBitWords) bits[threadIdx.x] = run_mask((int)threadIdx.x, 0, kChunks);
+    if (threadIdx.x == 0) lock = freed = 0u;
+    __syncthreads();
+    const ChunkPool P{bits, &lock, &freed, arena, lane};
+    const int ps = packed_size(n);
+    const int k0 = chunks_for(ps + 1);   // + the idle lanes' element
+    for (;;) {
+        // lane 0 adds 1, the others 0 (see zgeev_pool.hpp on the control flow)
+        const int j = __builtin_amdgcn_readfirstlane(atomicAdd(qctr, lane == 0 ? 1 : 0));
+        if (j >= batch) break;
+        zd *out = eig + (long long)j * n;
+        if (__builtin_amdgcn_readfirstlane(flag[j]) == 2) {
+            if (lane < n) out[lane] = mk(__builtin_nan(""), __builtin_nan(""));
+            continue;
+        }
+        const double us = out[0].re;   // the Hessenberg kernel's norm-range factor
+        const int s0 = P.alloc_wait(k0);
+        zd *H = arena + s0 * kChunk;
+        const zd *src = work + (long long)j * ps;
+        for (int e = lane; e < ps; e += 64) H[e] = src[e];
+        lds_order();
+        ChunkMover<0> mv{&P, out, s0, k0, n, H, us};
+        int info = 0;
+        if (n > 1)
+            info = zlahqr_dual<64, false, true, ChunkMover<0>>(H, PackedLayout{}, n, lane, itmax, nullptr, 0, &mv,
+                                                              k0 * kChunk - 1);
+        const PackedLayout pk;
+        if (lane < mv.whi) out[lane] = mv.eig(mv.H[pk(lane, lane)]);   // whi <= n <= 64
+        P.give(mv.s, mv.k);
+        flag[j] = info != 0 ? 1 : 0;   // every lane stores the same byte
+    }
+}
+
 static const PoolLayout g_pool_layouts[] = {
 #define X(id, w, a0, a1, a2, a3, a4, a5) {w, {a0, a1, a2, a3, a4, a5}},
     TKZ_POOL_LAYOUTS(X)
@@ -321,6 +373,32 @@ static hipError_t launch_pool(int pl, int n, int batch, const zd *A, zd *eig, in
     return hipGetLastError();
 }
 
+// Chunk variant: the 256-thread Hessenberg, then k_qr_chunk<w> on one
+// workgroup per CU; the QR's counter sits after the packed matrices, as for
+// the pool variants.
+static hipError_t launch_chunk(int w, int n, int batch, const zd *A, zd *eig, int8_t *flag, zd *work, int itmax,
+                               int bal, hipStream_t s, int *stage, hipEvent_t mid)
+{
+    int *qctr = (int *)(work + (long long)packed_size(n) * batch);
+    *stage = 1;
+    launch_hess_wg(n, batch, A, work, flag, bal, eig, qctr, s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (mid) (void)hipEventRecord(mid, s);   // stage-timing marker: a failed record only spoils that reading
+    *stage = 2;
+    const dim3 grid(device_cus());
+    switch (w) {
+#define X(wv)                                                                                                  \
+    case wv:                                                                                                   \
+        hipLaunchKernelGGL((k_qr_chunk<wv>), grid, dim3(64 * wv), 0, s, n, batch, work, eig, flag, itmax, qctr); \
+        break;
+        X(8) X(10) X(12)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Variant table and launcher
 // ---------------------------------------------------------------------------
@@ -361,7 +439,10 @@ static hipError_t launch_pool(int pl, int n, int batch, const zd *A, zd *eig, in
     X(33, POOL, 64, 64)                 \
     X(34, POOL, 64, 64)                 \
     X(35, POOL, 64, 64)                 \
-    X(36, POOL, 64, 64)
+    X(36, POOL, 64, 64)                 \
+    X(37, CHUNK, 64, 64)                \
+    X(38, CHUNK, 64, 64)                \
+    X(39, CHUNK, 64, 64)
 // pool variants: layout id - kFirstPool of TKZ_POOL_LAYOUTS
 constexpr int kFirstPool = 29;
 
@@ -411,6 +492,9 @@ static hipError_t launch_kind(int id, int n, int batch, const zd *A, zd *eig, in
 {
     if constexpr (KIND == VK_POOL)
         return launch_pool(id - kFirstPool, n, batch, A, eig, flag, work, itmax, bal, s, stage, mid);
+    else if constexpr (KIND == VK_CHUNK)   // variant 37: 8 wavefronts per CU, 38: 12, 39: 10
+        return launch_chunk(id == 37 ? 8 : (id == 38 ? 12 : 10), n, batch, A, eig, flag, work, itmax, bal, s, stage,
+                            mid);
     else if constexpr (KIND == VK_FUSED)
         return launch_fused<NB, LPM>(n, batch, A, eig, flag, itmax, bal, s);
     else

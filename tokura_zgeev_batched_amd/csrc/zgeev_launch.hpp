@@ -11,7 +11,10 @@ namespace tkz {
 // VK_POOL: the 256-thread Hessenberg, then the two-bulge QR on a persistent
 // workgroup per CU whose LDS is an arena of slots that matrices move down
 // through as they deflate (its eigenvalues are VK_DUAL's bit for bit)
-enum VariantKind { VK_FUSED = 0, VK_SPLIT = 1, VK_DUAL = 2, VK_DUAL_1W = 3, VK_POOL = 4 };
+// VK_CHUNK: the same two stages with the QR workgroup's LDS cut into chunks
+// (the ragged path's arena) that the deflating matrices compact at every
+// deflation (VK_DUAL's eigenvalues bit for bit)
+enum VariantKind { VK_FUSED = 0, VK_SPLIT = 1, VK_DUAL = 2, VK_DUAL_1W = 3, VK_POOL = 4, VK_CHUNK = 5 };
 
 // ZGEBAL job bits of tokuraSetBalancing ('P' = permute, 'S' = scale, 'B' = both)
 enum BalanceBits { BAL_PERMUTE = 1, BAL_SCALE = 2 };

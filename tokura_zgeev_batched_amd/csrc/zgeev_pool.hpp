@@ -266,29 +266,56 @@ struct ChunkPool {
         atomicAnd(lock, lane == 0 ? ~1u : ~0u);
     }
     // Start of a run of k free chunks (1 <= k <= kChunks): the lowest
-    // (top = false) or the highest; -1 if none.  Lane l tests the starts
-    // l, l + 64, ...; every lane reads the bit words.
-    __device__ __forceinline__ int find(int k, bool top) const
+    // (top = false) or the highest; -1 if none.  Chunks [xs, xs + xk) count
+    // as free (the caller's own run, when it looks for a place to compact
+    // into); a lowest-run search looks at starts below 64 (jmax + 1) only.
+    // Lane l tests the start 64 j + l of start word j: the 64 bits from
+    // there (one funnel shift of two bit words; the word index is uniform)
+    // hold a run of at least k when their count of trailing ones is >= k,
+    // and for k > 64 the next 64 bits are tested the same way.
+    __device__ __forceinline__ int find(int k, bool top, int xs = 0, int xk = 0, int jmax = kBitWords - 1) const
     {
-        unsigned long long w[kBitWords];
+        unsigned long long w[kBitWords + 2];
 #pragma unroll
         for (int q = 0; q < kBitWords; ++q)
-            w[q] = __hip_atomic_load(&bits[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            w[q] = __hip_atomic_load(&bits[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) | run_mask(q, xs, xk);
+        w[kBitWords] = w[kBitWords + 1] = 0ull;   // past the arena: not free
         int found = -1;
 #pragma unroll
         for (int jj = 0; jj < kBitWords; ++jj) {
             const int j = top ? kBitWords - 1 - jj : jj;
-            const int s = lane + 64 * j;
-            bool ok = s + k <= kChunks;
-#pragma unroll
-            for (int q = 0; q < kBitWords; ++q) {
-                const unsigned long long m = run_mask(q, s, k);
-                ok = ok && (w[q] & m) == m;
+            if (found >= 0 || (!top && j > jmax)) break;
+            auto bits64 = [&](int q) {   // 64 bits from chunk 64 q + lane
+                return lane == 0 ? w[q] : (w[q] >> lane) | (w[q + 1] << (64 - lane));
+            };
+            // trailing ones of x (64 when all are ones; the count's argument is
+            // never 0: bit 63 set in it changes no count below 64)
+            auto ones = [](unsigned long long x) {
+                return x == ~0ull ? 64 : (int)__builtin_ctzll(~x | (1ull << 63));
+            };
+            const unsigned long long x0 = bits64(j);
+            bool ok;
+            if (k <= 64) {
+                ok = ones(x0) >= k;
+            } else {
+                const unsigned long long x1 = bits64(j + 1);
+                ok = x0 == ~0ull && ones(x1) >= k - 64;
             }
             const unsigned long long b = __ballot(ok);
-            if (found < 0 && b != 0ull) found = 64 * j + (top ? 63 - __clzll((long long)b) : __ffsll((long long)b) - 1);
+            if (b != 0ull) found = 64 * j + (top ? 63 - __clzll((long long)b) : __ffsll((long long)b) - 1);
         }
         return __builtin_amdgcn_readfirstlane(found);
+    }
+    // Whether any chunk below chunk s is free (a snapshot).
+    __device__ __forceinline__ bool free_below(int s) const
+    {
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < kBitWords; ++q) {
+            const unsigned long long w = __hip_atomic_load(&bits[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            any = any || (w & run_mask(q, 0, s)) != 0ull;
+        }
+        return __builtin_amdgcn_readfirstlane(any ? 1 : 0) != 0;
     }
     // Marks chunks [s, s+k) used (lanes 0..kBitWords-1 each change one word).
     __device__ __forceinline__ void take(int s, int k) const
@@ -348,20 +375,57 @@ struct ChunkPool {
         }
     }
     __device__ __forceinline__ int try_alloc_top(int k) const { return try_alloc(k, true); }
+    // The lowest run of `need` chunks when the caller's run [s, s + k) counts
+    // as free, claimed (the caller's own chunks in it stay claimed), or -1
+    // when that is the caller's run itself with need == k or the lock was
+    // not had in kLockTries tries.  A run less than min_shift chunks below s
+    // is replaced by s itself (shrinking in place).  The result is <= s (the
+    // caller's run holds `need` chunks); the caller copies its data down and
+    // gives back the chunks of its run that the new one does not cover.
+    __device__ __forceinline__ int compact(int need, int s, int k, int min_shift) const
+    {
+        const int jmax = s / 64;   // the result is <= s
+        auto useful = [&](int r) { return r >= 0 && (r <= s - min_shift || (r == s && need < k)); };
+        int s0 = find(need, false, s, k, jmax);
+        if (s0 > s - min_shift) s0 = s;   // a shorter move down is not worth its copy: shrink in place
+        if (!useful(s0)) return -1;
+        for (int r = 0; r < kLockTries; ++r) {
+            if (try_lock()) {
+                int s2 = find(need, false, s, k, jmax);
+                if (s2 > s - min_shift) s2 = s;
+                const bool move = useful(s2);
+                if (move) take(s2, need);
+                unlock();
+                return move ? s2 : -1;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        return -1;
+    }
 };
 
-// zlahqr_dual's hook for a matrix in a chunk run: after a deflation, when
-// the active part (columns 0..i and the idle lanes' element) needs at most
-// mv/8 of the run, move it to the highest free run that holds it (if any),
-// writing the eigenvalues of the rows that became final first.
+// zlahqr_dual's hook for a matrix in a chunk run, after every deflation
+// that leaves columns 0..i (and the idle lanes' element) active.  The
+// eigenvalues of the rows that became final (i+1 .. whi-1) are written from
+// the old run before any of it is given back.
+//  MV = 0 (compaction): the active part moves to the lowest run of free
+//    chunks that holds it, the matrix's own run counting as free
+//    (ChunkPool::compact): in place when nothing below is free (the run's
+//    tail chunks are given back, no lock), else copied down.  The arena
+//    stays compacted at its low end, where new tiles start (a simulation
+//    of one CU at n = 64: 7.3 resident matrices against 5.8 for MV = 5).
+//  MV = 4..7: when the active part needs at most MV/8 of the run, it moves
+//    to the highest free run that holds it, if any (cheaper per deflation;
+//    faster where the CU's wavefronts, not its LDS, are the limit).
+template <int MV>
 struct ChunkMover {
+    static constexpr int kMinShift = 1;   // any lower run (4 was 1-2% slower at n = 64)
     const ChunkPool *P;
     zd *out;
     int s, k;     // current run
     int whi;      // rows whi.. of out are written
     zd *H;
     double us;    // norm-range factor of the eigenvalues
-    int mv;       // move threshold in eighths of the run
 
     __device__ __forceinline__ zd eig(zd h) const { return us != 1.0 ? zscale(us, h) : h; }
 
@@ -369,15 +433,52 @@ struct ChunkMover {
     {
         const int ps = packed_size(i + 1);
         const int need = chunks_for(ps + 1);
-        if (8 * need > mv * k) return;
-        const int s2 = P->try_alloc_top(need);
+        int s2;
+        if constexpr (MV == 0) {
+            // nothing free below the run: shrink in place (giving back the
+            // run's own tail needs no lock); else the lowest run that fits
+            const bool below = P->free_below(s);
+            if (!below && need == k) return;
+            s2 = below ? P->compact(need, s, k, kMinShift) : s;
+        } else {
+            if (8 * need > MV * k) return;
+            s2 = P->try_alloc_top(need);
+        }
         if (s2 < 0) return;
         const PackedLayout pk;
         if (i + 1 + t < whi) out[i + 1 + t] = eig(Hq[pk(i + 1 + t, i + 1 + t)]);   // whi <= n <= 64
         whi = i + 1;
         zd *H2 = P->arena + s2 * kChunk;
-        for (int e = t; e < ps; e += 64) H2[e] = Hq[e];
-        P->give(s, k);
+        if constexpr (MV == 0) {
+            if (s2 != s) {
+                // s2 < s: copy down, lowest elements first, in blocks whose
+                // loads all precede their stores (a block writes only
+                // elements that this or an earlier block has read; the
+                // wavefront's LDS operations run in order)
+                // (loads past the prefix are clamped to its last element; the
+                // values are held as doubles, which stay in registers across
+                // the compiler barriers)
+                for (int e0 = 0; e0 < ps; e0 += 4 * 64) {
+                    const int e1 = e0 + t, e2 = e1 + 64, e3 = e1 + 128, e4 = e1 + 192;
+                    const zd *q1 = Hq + (e1 < ps ? e1 : ps - 1), *q2 = Hq + (e2 < ps ? e2 : ps - 1);
+                    const zd *q3 = Hq + (e3 < ps ? e3 : ps - 1), *q4 = Hq + (e4 < ps ? e4 : ps - 1);
+                    const double r1 = q1->re, i1 = q1->im, r2 = q2->re, i2 = q2->im;
+                    const double r3 = q3->re, i3 = q3->im, r4 = q4->re, i4 = q4->im;
+                    lds_order();
+                    if (e1 < ps) H2[e1] = mk(r1, i1);
+                    if (e2 < ps) H2[e2] = mk(r2, i2);
+                    if (e3 < ps) H2[e3] = mk(r3, i3);
+                    if (e4 < ps) H2[e4] = mk(r4, i4);
+                    lds_order();
+                }
+            }
+            // the chunks of the old run outside the new one
+            const int lo = s2 + need > s ? s2 + need : s;
+            if (lo < s + k) P->give(lo, s + k - lo);
+        } else {   // a separate run: copy, then give the whole old run
+            for (int e = t; e < ps; e += 64) H2[e] = Hq[e];
+            P->give(s, k);
+        }
         lds_order();
         s = s2;
         k = need;
