@@ -35,21 +35,19 @@ import tokura_zgeev_batched_amd as tk  # noqa: E402
 
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
-# HBM bytes per matrix moved by the n = 64 QR kernel, from the rocprofv3
-# FETCH_SIZE / WRITE_SIZE passes over one 20,000-matrix call of the pooled QR
-# k_qr_pool (each counter scaled by its reading for a 1 GiB device copy:
-# FETCH x2.0, WRITE x1.0): 34,563 B read + 1,086 B written.  The two-bulge
-# k_qr_packed<64,64,true> (variant 20) moved 35,480 B.
-QR64_TRAFFIC_BYTES_PER_MATRIX = 35_648.0
-# FP64 lane operations executed per matrix by the n = 64 two-bulge QR
-# arithmetic, which the pooled variants 29-36 share with variant 20
-# (rocprofv3 SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64 over one 20,000-matrix call
-# of k_qr_pool<12>, 64 lanes per wave instruction, an FMA counted as 2):
-# 64 * (2 * 311,001 + 140,622 + 69,864 + 13,011) per matrix.  It counts the
-# serial reflector chain that every lane computes redundantly, so it is
-# ~7.7x the 80/3 n^3 model.
-QR64_EXECUTED_FP64_PER_MATRIX = 5.418e7
-QR64_COUNTED_VARIANTS = (20,) + tuple(range(29, 37))
+# Per-matrix figures of the n = 64 QR kernels from rocprofv3 counter passes
+# over one 20,000-matrix call (FETCH_SIZE / WRITE_SIZE each scaled by its
+# reading for a 1 GiB device copy: FETCH x2.0, WRITE x1.0; SQ_INSTS_VALU_
+# {FMA,MUL,ADD,TRANS}_F64 x 64 lanes per wave instruction, an FMA counted as 2):
+#  - chunk variants 37-39 (k_qr_chunk<8>): 34,570 B read + 1,072 B written;
+#    64 * (2 * 310,995 + 140,690 + 69,862 + 13,011) FP64 lane operations;
+#  - pool variants 29-36 (k_qr_pool<12>) and the fixed variant 20, the same
+#    two-bulge arithmetic: 34,563 B read + 1,086 B written (variant 20 moved
+#    35,480 B); 64 * (2 * 311,001 + 140,622 + 69,864 + 13,011).
+# The FP64 count includes the serial reflector chain that every lane computes
+# redundantly, ~7.7x the 80/3 n^3 model.
+QR64_COUNTS = {v: (35_642.0, 5.4115e7) for v in (37, 38, 39)}
+QR64_COUNTS.update({v: (35_648.0, 5.418e7) for v in (20,) + tuple(range(29, 37))})
 
 
 def flops_per_solve(n: int) -> float:
@@ -502,20 +500,22 @@ def main():
                    "matrices_per_workgroup": mpb},
         "roofline": {"bound": "fp64_vector", "achieved": qr_achieved / 1e12,
                      "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": qr_achieved / PEAK_FP64,
-                     "traffic": (QR64_TRAFFIC_BYTES_PER_MATRIX * B if n == 64 else None),
+                     "traffic": (QR64_COUNTS[variant][0] * B if n == 64 and variant in QR64_COUNTS else None),
                      "traffic_unit": "bytes per launch (rocprof FETCH_SIZE+WRITE_SIZE, calibrated)",
                      "algorithmic_bytes": (8.0 * (n - 1) * (n + 2) + 16.0 * n + 16.0 * n + 1.0) * B,
-                     "kernel": ("QR stage (" + ("k_qr_pool, one workgroup per CU" if variant >= 29 else "k_qr_packed")
+                     "kernel": ("QR stage (" + ("k_qr_chunk, one workgroup per CU, compacting LDS arena"
+                                                if variant >= 37 else
+                                                ("k_qr_pool, one workgroup per CU" if variant >= 29 else "k_qr_packed"))
                                 + ") charged 80/3 n^3 flop/matrix"),
                      "compute_unit": "fp64 VALU (no MFMA is issued); MI355X fp64 vector peak 78.6 TFLOP/s",
                      "kernel_avg_ms": avg_qr_s * 1e3, "hessenberg_avg_ms": avg_hess_s * 1e3,
                      "path_achieved_tflops": path_achieved / 1e12,
                      "path_frac": path_achieved / PEAK_FP64,
                      # issued FP64 lane operations (rocprof counters per matrix, live kernel time)
-                     "executed_fp64_tflops": (QR64_EXECUTED_FP64_PER_MATRIX * B / avg_qr_s / 1e12
-                                              if n == 64 and variant in QR64_COUNTED_VARIANTS else None),
-                     "executed_fp64_frac": (QR64_EXECUTED_FP64_PER_MATRIX * B / avg_qr_s / PEAK_FP64
-                                            if n == 64 and variant in QR64_COUNTED_VARIANTS else None)},
+                     "executed_fp64_tflops": (QR64_COUNTS[variant][1] * B / avg_qr_s / 1e12
+                                              if n == 64 and variant in QR64_COUNTS else None),
+                     "executed_fp64_frac": (QR64_COUNTS[variant][1] * B / avg_qr_s / PEAK_FP64
+                                            if n == 64 and variant in QR64_COUNTS else None)},
         "flags_nonzero": nflag,
     }
 
