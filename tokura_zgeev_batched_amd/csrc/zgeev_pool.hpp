@@ -222,13 +222,17 @@ inline bool build_pool_cfg(int n, const PoolLayout &L, PoolCfg &P)
 
 namespace tkz {
 
-// ---- chunked arena for work items of any size (the ragged path) ----------
+// ---- chunked arena for work items of any size (the ragged path and the
+// same-size chunk QR) ----------
 // The arena is cut into kChunks chunks of kChunk complex elements, with one
 // bit per chunk (set = free) in kBitWords 64-bit LDS words and a lock word.
 // A new work item takes the lowest run of free chunks that holds it; a
-// deflating matrix moves to the highest run that holds its active part, so
-// the regions that items start in stay at the bottom, where the next
-// (LPT order: equal or smaller) item fits the hole it leaves.  Claims run
+// deflating matrix either compacts down (the lowest run that holds its
+// active part, its own run counting as free), which keeps the free chunks
+// together at the top, or moves to the highest free run once its active
+// part needs a fraction of its run (ChunkMover below), so that the
+// regions that items start in stay at the bottom, where the next (LPT
+// order: equal or smaller) item fits the hole it leaves.  Claims run
 // under the lock (one wavefront at a time, a few dozen instructions, no
 // wait inside) after a search of a snapshot without it, and nothing waits
 // for the lock beyond a few tries; frees need no lock (they only set bits
