@@ -132,8 +132,12 @@ int tokuraLoadTuning(tokurablas_t *handle, const char *cache_path);
 
 /* Writes the handle's tuning (same-size variant per n, and the ragged table
  * that tokuraAutotuneRagged fills: one line per batch size signature) to
- * cache_path, keyed by the device.  Returns 0, -1 handle, -3 path NULL, 5 I/O
- * error.  No counterpart in the reference, whose tables are compiled in. */
+ * cache_path, keyed by the device (architecture name, "none" for a handle
+ * made without a device, and CU count).  Lines of other devices already in
+ * the file are kept; this device's old lines are replaced.  The file is
+ * written beside cache_path and renamed over it.  Returns 0, -1 handle, -3
+ * path NULL, 5 I/O error.  No counterpart in the reference, whose tables are
+ * compiled in. */
 int tokuraSaveTuning(tokurablas_t *handle, const char *cache_path);
 
 /* Sets the variant used for size n by hand (a variant id as reported by

@@ -111,6 +111,17 @@ def test_tuning_cache_save_and_ragged_lines(tmp_path):
     assert len(body) == 64 and not any(ln.startswith("ragged ") for ln in body)
     assert L.tokuraSaveTuning(None, str(p).encode()) == -1
     assert L.tokuraSaveTuning(h, None) == -3
+    # other devices' lines survive a save; this device's lines are replaced,
+    # not duplicated; the file is replaced whole (no temporary left behind)
+    foreign = ["gfx942 304 64 20", "ragged gfx942 304 abc 3 2"]
+    p.write_text(p.read_text() + "\n".join(foreign) + "\n")
+    assert L.tokuraSaveTuning(h, str(p).encode()) == 0
+    assert L.tokuraSaveTuning(h, str(p).encode()) == 0
+    body2 = [ln for ln in p.read_text().splitlines() if not ln.startswith("#")]
+    assert all(ln in body2 for ln in foreign) and len(body2) == 64 + len(foreign)
+    assert sorted(ln for ln in body2 if ln not in foreign) == sorted(body)
+    assert [x.name for x in tmp_path.iterdir()] == ["save.txt"]
+    assert L.tokuraLoadTuning(h, str(p).encode()) == 64
     q = tmp_path / "ragged.txt"
     q.write_text("ragged gfx950 256 10842108421 5 4\nragged gfx950 256 zz 5 4\nragged gfx950 256 1 99 4\n")
     assert L.tokuraLoadTuning(h, str(q).encode()) == 0

@@ -19,6 +19,7 @@
 #include <random>
 #include <string>
 #include <vector>
+#include <unistd.h>
 
 #include "../../include/tokura_blas.h"
 #include "tokura_internal.hpp"
@@ -237,7 +238,7 @@ extern "C" int tokuraLoadTuning(tokurablas_t *handle, const char *cache_path)
             continue;
         }
         if (std::sscanf(line, "%63s %d %d %d", arch, &cus, &n, &v) != 4) continue;
-        if (std::strcmp(arch, handle->arch) != 0 || cus != handle->cu_count) continue;
+        if (std::strcmp(arch, tuning_arch(handle)) != 0 || cus != handle->cu_count) continue;
         if (n < 1 || n > TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE || !variant_fits(v, n)) continue;
         handle->variant_for_n[n] = v;
         ++count;
@@ -307,16 +308,42 @@ extern "C" int tokuraAutotune(tokurablas_t *handle, int n_lo, int n_hi, int batc
     return rc;
 }
 
+// The file keeps the lines of other devices (another architecture or CU
+// count, same-size or ragged) and gets this handle's lines in place of its
+// old ones; it is written to a temporary file beside it and renamed over it,
+// so a reader never sees a half-written cache.
 extern "C" int tokuraSaveTuning(tokurablas_t *handle, const char *cache_path)
 {
     if (handle == nullptr) return -1;
     if (cache_path == nullptr) return -3;
-    FILE *f = std::fopen(cache_path, "w");
+    const char *key = tuning_arch(handle);
+    std::vector<std::string> others;
+    if (FILE *old = std::fopen(cache_path, "r")) {
+        char line[256];
+        while (std::fgets(line, sizeof line, old)) {
+            if (line[0] == '#' || std::strchr(line, '\n') == nullptr) continue;   // comments, over-long lines
+            const char *p = std::strncmp(line, "ragged ", 7) == 0 ? line + 7 : line;
+            char arch[64];
+            int cus;
+            if (std::sscanf(p, "%63s %d", arch, &cus) != 2) continue;
+            if (std::strcmp(arch, key) == 0 && cus == handle->cu_count) continue;   // replaced below
+            others.emplace_back(line);
+        }
+        std::fclose(old);
+    }
+    const std::string tmp = std::string(cache_path) + ".tmp" + std::to_string((long long)getpid());
+    FILE *f = std::fopen(tmp.c_str(), "w");
     if (!f) return 5;
     std::fprintf(f, "# arch cus n variant  (tokura_zgeev_batched_amd autotune cache)\n");
     std::fprintf(f, "# ragged arch cus signature config hess_grid_multiplier\n");
+    for (const auto &ln : others) std::fputs(ln.c_str(), f);
     for (int n = 1; n <= TOKURA_ZGEEV_BATCHED_MAX_MATRIX_SIZE; ++n)
-        std::fprintf(f, "%s %d %d %d\n", handle->arch, handle->cu_count, n, handle->variant_for_n[n]);
+        std::fprintf(f, "%s %d %d %d\n", key, handle->cu_count, n, handle->variant_for_n[n]);
     ragged_write_tuning(handle, f);
-    return std::fclose(f) == 0 ? 0 : 5;
+    const bool ok = std::ferror(f) == 0;
+    if (std::fclose(f) != 0 || !ok || std::rename(tmp.c_str(), cache_path) != 0) {
+        std::remove(tmp.c_str());
+        return 5;
+    }
+    return 0;
 }

@@ -82,6 +82,9 @@ int resident_blocks(K k, int threads = 64)
 }
 
 void ragged_release(tokurablas_t *h);
+// The device key of a tuning cache's lines: the handle's architecture name
+// ("none" for a handle made without a device) and its CU count.
+inline const char *tuning_arch(const tokurablas_t *h) { return h->arch[0] ? h->arch : "none"; }
 // Writes the ragged lines of a tuning cache ("ragged arch cus sig cfg hmul");
 // parses one such line into the handle (returns 1 if it was taken).
 void ragged_write_tuning(const tokurablas_t *h, FILE *f);

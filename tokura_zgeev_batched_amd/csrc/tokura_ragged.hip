@@ -1172,7 +1172,7 @@ extern "C" int tokuraAutotuneRagged(tokurablas_t *handle, int batch_count, const
 void ragged_write_tuning(const tokurablas_t *h, FILE *f)
 {
     for (const auto &t : h->ragged_tuned)
-        std::fprintf(f, "ragged %s %d %llx %d %d\n", h->arch, h->cu_count, t.sig, t.cfg, t.hmul);
+        std::fprintf(f, "ragged %s %d %llx %d %d\n", tuning_arch(h), h->cu_count, t.sig, t.cfg, t.hmul);
 }
 
 int ragged_parse_tuning(tokurablas_t *h, const char *line)
@@ -1181,7 +1181,7 @@ int ragged_parse_tuning(tokurablas_t *h, const char *line)
     int cus, cfg, hmul;
     unsigned long long sig;
     if (std::sscanf(line, "ragged %63s %d %llx %d %d", arch, &cus, &sig, &cfg, &hmul) != 5) return 0;
-    if (std::strcmp(arch, h->arch) != 0 || cus != h->cu_count) return 0;
+    if (std::strcmp(arch, tuning_arch(h)) != 0 || cus != h->cu_count) return 0;
     if (cfg < 0 || cfg >= kNumRaggedCfg || !(hmul == 1 || hmul == 2 || hmul == 4 || hmul == 8)) return 0;
     for (auto &t : h->ragged_tuned)
         if (t.sig == sig) {
