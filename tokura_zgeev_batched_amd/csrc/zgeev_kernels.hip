@@ -17,6 +17,7 @@
 //             (n-1)(n+2)/2+n elements) to the caller's workspace; k_qr_packed
 //             runs QR on the packed copy, which needs about half the LDS, so
 //             twice as many matrices are resident per CU at n = 64.
+#include <atomic>
 #include "zgeev_device.hpp"
 #include "zgeev_hess_reg.hpp"
 #include "zgeev_launch.hpp"
@@ -309,15 +310,16 @@ static const PoolLayout g_pool_layouts[] = {
 
 static int device_cus()
 {
-    static int cus[64] = {0};
+    // per device, filled on first use (threads that race store the same value)
+    static std::atomic<int> cus[64];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (cus[dev] == 0) {
-        int c = 0;
+    int c = cus[dev].load(std::memory_order_relaxed);
+    if (c == 0) {
         if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c < 1) c = 256;
-        cus[dev] = c;
+        cus[dev].store(c, std::memory_order_relaxed);
     }
-    return cus[dev];
+    return c;
 }
 
 // The 256-thread Hessenberg with LDS for the smallest of 40, 48, 56, 60, 64
