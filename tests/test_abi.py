@@ -134,6 +134,11 @@ def test_python_layer_refuses_host_tensors():
     torch = pytest.importorskip("torch")
     with pytest.raises(tk.TokuraError):
         tk._dev_check(torch.zeros(2, 2, dtype=torch.complex128), "M", torch.complex128)
+    # the ragged wrapper's caller-supplied buffers (eig, flag, gbar, schur)
+    from tokura_zgeev_batched_amd.ragged import RaggedBatch
+    with pytest.raises(tk.TokuraError):
+        RaggedBatch._sized(torch.zeros(8, dtype=torch.complex128), "gbar", torch.complex128, 8,
+                           torch.device("cuda", 0))
 
 
 def test_vectors_entry_validation():

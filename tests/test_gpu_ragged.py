@@ -205,3 +205,32 @@ def test_ragged_reprepare_other_offsets_same_buffer():
     assert int(f2.abs().sum()) == 0
     check_against_lapack(mats2, e2.cpu().numpy(), rb2.eig_offsets)
     h.close()
+
+
+def test_ragged_wrapper_rejects_bad_buffers():
+    """Caller-supplied output and cotangent buffers are checked before any
+    launch: too short, wrong dtype, on the host or on another device."""
+    rng = np.random.default_rng(7)
+    mats = rand_mats(rng, [5, 12, 40])
+    flat, offs = pack(mats)
+    A = torch.from_numpy(flat).cuda()
+    rb = RaggedBatch(offs, handle=tk.Handle())
+    ne = int(rb.eig_offsets[-1])
+    bad = [dict(eig=torch.empty(ne - 1, dtype=torch.complex128, device="cuda")),
+           dict(eig=torch.empty(ne, dtype=torch.complex64, device="cuda")),
+           dict(eig=torch.empty(ne, dtype=torch.complex128)),
+           dict(flag=torch.empty(rb.batch, dtype=torch.int32, device="cuda")),
+           dict(flag=torch.empty(rb.batch - 1, dtype=torch.int8, device="cuda"))]
+    for kw in bad:
+        with pytest.raises(tk.TokuraError):
+            rb.solve(A, **kw)
+    eig, schur, flag = rb.schur(A)
+    with pytest.raises(tk.TokuraError):
+        rb.backward(schur, torch.ones(ne, dtype=torch.complex128), flag)          # host gbar
+    with pytest.raises(tk.TokuraError):
+        rb.backward(schur, torch.ones(ne - 1, dtype=torch.complex128, device="cuda"), flag)
+    with pytest.raises(tk.TokuraError):
+        rb.backward(schur[:-1], torch.ones(ne, dtype=torch.complex128, device="cuda"), flag)
+    dA = rb.backward(schur, torch.ones(ne, dtype=torch.complex128, device="cuda"), flag)
+    assert dA.shape == A.shape and bool(torch.isfinite(dA).all())
+
