@@ -135,10 +135,12 @@ def test_python_layer_refuses_host_tensors():
     with pytest.raises(tk.TokuraError):
         tk._dev_check(torch.zeros(2, 2, dtype=torch.complex128), "M", torch.complex128)
     # the ragged wrapper's caller-supplied buffers (eig, flag, gbar, schur)
-    from tokura_zgeev_batched_amd.ragged import RaggedBatch
     with pytest.raises(tk.TokuraError):
-        RaggedBatch._sized(torch.zeros(8, dtype=torch.complex128), "gbar", torch.complex128, 8,
-                           torch.device("cuda", 0))
+        tk._buf(torch.zeros(8, dtype=torch.complex128), "gbar", torch.complex128, 8, torch.device("cuda", 0))
+    with pytest.raises(tk.TokuraError):   # raw wrappers check before any launch
+        tk.zgeev_batched(torch.zeros(16, dtype=torch.complex128), 4, 1, torch.zeros(4, dtype=torch.complex128),
+                         torch.zeros(512, dtype=torch.uint8), torch.zeros(1, dtype=torch.int8))
+    assert not tk._shape_ok(0, 1) and not tk._shape_ok(65, 1) and not tk._shape_ok(4, 0) and tk._shape_ok(64, 1)
 
 
 def test_vectors_entry_validation():

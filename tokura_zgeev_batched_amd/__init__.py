@@ -205,12 +205,38 @@ def _dev_check(t: torch.Tensor, name: str, dtype=None):
         raise TokuraError(f"{name} must be contiguous")
 
 
+def _buf(t: torch.Tensor, name: str, dtype, count: int, device=None):
+    """Checks a tensor that the library gets as a raw pointer: a contiguous
+    device tensor of `dtype` (None: any dtype, `count` in bytes), on `device`
+    when given, holding at least `count` elements.  A host tensor, another
+    device or a short buffer would otherwise be read or written out of
+    bounds by the kernels."""
+    _dev_check(t, name, dtype)
+    if device is not None and t.device != device:
+        raise TokuraError(f"{name} is on {t.device}, the batch on {device}")
+    have = t.numel() if dtype is not None else t.numel() * t.element_size()
+    if have < count:
+        raise TokuraError(f"{name} holds {have} {'elements' if dtype is not None else 'bytes'}, "
+                          f"the call needs {count}")
+
+
+def _shape_ok(n: int, batch: int) -> bool:
+    # invalid n or batch: the C entry returns its own code, nothing is launched
+    return 1 <= n <= MAX_N and batch > 0
+
+
 def zgeev_batched(A_colmajor: torch.Tensor, n: int, batch: int, eig: torch.Tensor, work: torch.Tensor,
                   flag: torch.Tensor, handle: Handle | None = None, stream=None) -> int:
     """Raw reference call: buffers already laid out as the C ABI wants
     (A: batch*n*n complex128 column-major per matrix; eig: batch*n; work:
     workspace_size(n, batch) bytes; flag: batch int8).  Returns the C code."""
     h = handle or default_handle()
+    if _shape_ok(n, batch):
+        d = A_colmajor.device
+        _buf(A_colmajor, "A", torch.complex128, batch * n * n)
+        _buf(eig, "eig", torch.complex128, batch * n, d)
+        _buf(work, "work", None, workspace_size(n, batch), d)
+        _buf(flag, "flag", torch.int8, batch, d)
     return int(lib().tokura_zgeev_batched_gpu(h.ptr, n, batch, _P(A_colmajor.data_ptr()), _P(eig.data_ptr()),
                                                _P(work.data_ptr()), _P(flag.data_ptr()), _stream_ptr(stream)))
 
@@ -244,6 +270,13 @@ def dgeev_batched(A_colmajor: torch.Tensor, n: int, batch: int, wr: torch.Tensor
     matrix; wr, wi batch*n float64; work dgeev_workspace_size(n, batch) bytes;
     flag batch int8.  Returns the C code."""
     h = handle or default_handle()
+    if _shape_ok(n, batch):
+        d = A_colmajor.device
+        _buf(A_colmajor, "A", torch.float64, batch * n * n)
+        _buf(wr, "wr", torch.float64, batch * n, d)
+        _buf(wi, "wi", torch.float64, batch * n, d)
+        _buf(work, "work", None, dgeev_workspace_size(n, batch), d)
+        _buf(flag, "flag", torch.int8, batch, d)
     return int(lib().tokura_dgeev_batched_gpu(h.ptr, n, batch, _P(A_colmajor.data_ptr()), _P(wr.data_ptr()),
                                                _P(wi.data_ptr()), _P(work.data_ptr()), _P(flag.data_ptr()),
                                                _stream_ptr(stream)))
@@ -277,6 +310,16 @@ def zgeev_vectors(A_colmajor: torch.Tensor, n: int, batch: int, eig: torch.Tenso
     int8, work (optional, used for one-sided requests) at least
     vectors_workspace_size(n, batch) bytes.  Returns the C code."""
     h = handle or default_handle()
+    if _shape_ok(n, batch):
+        d = A_colmajor.device
+        _buf(A_colmajor, "A", torch.complex128, batch * n * n)
+        _buf(eig, "eig", torch.complex128, batch * n, d)
+        for V, nm in ((VL, "VL"), (VR, "VR")):
+            if V is not None:
+                _buf(V, nm, torch.complex128, batch * n * n, d)
+        if work is not None:
+            _buf(work, "work", None, vectors_workspace_size(n, batch), d)
+        _buf(flag, "flag", torch.int8, batch, d)
     return int(lib().tokura_zgeev_batched_vectors_gpu(
         h.ptr, n, batch, _P(A_colmajor.data_ptr()), _P(eig.data_ptr()),
         _P(VL.data_ptr()) if VL is not None else None, _P(VR.data_ptr()) if VR is not None else None,

@@ -20,7 +20,8 @@ from __future__ import annotations
 
 import torch
 
-from . import Handle, TokuraError, _P, _check, _dev_check, _stream_ptr, default_handle, lib, workspace_size
+from . import Handle, TokuraError, _P, _buf, _check, _dev_check, _shape_ok, _stream_ptr, default_handle, lib, \
+    workspace_size
 
 
 def schur_batched(A_colmajor: torch.Tensor, n: int, batch: int, handle: Handle | None = None, stream=None):
@@ -28,6 +29,8 @@ def schur_batched(A_colmajor: torch.Tensor, n: int, batch: int, handle: Handle |
     T then U, flag (B,))."""
     h = handle or default_handle()
     dev = A_colmajor.device
+    if _shape_ok(n, batch):
+        _buf(A_colmajor, "A", torch.complex128, batch * n * n)
     eig = torch.empty((batch, n), dtype=torch.complex128, device=dev)
     schur = torch.empty((batch, 2, n, n), dtype=torch.complex128, device=dev)
     flag = torch.empty((batch,), dtype=torch.int8, device=dev)
@@ -44,6 +47,10 @@ def backward_batched(n: int, batch: int, gbar: torch.Tensor, schur: torch.Tensor
     h = handle or default_handle()
     dev = gbar.device
     gbar = gbar.to(torch.complex128).contiguous()
+    if _shape_ok(n, batch):
+        _buf(gbar, "gbar", torch.complex128, batch * n)
+        _buf(schur, "schur", torch.complex128, batch * 2 * n * n, dev)
+        _buf(flag, "flag", torch.int8, batch, dev)
     dA = torch.empty((batch, n, n), dtype=torch.complex128, device=dev)
     rc = lib().tokura_zgeev_batched_backward_gpu(h.ptr, n, batch, None, _P(schur.data_ptr()), _P(gbar.data_ptr()),
                                                  _P(dA.data_ptr()), _P(flag.data_ptr()), _stream_ptr(stream))
@@ -61,6 +68,10 @@ def backward_recompute(n: int, batch: int, A_colmajor: torch.Tensor, eig: torch.
     dev = gbar.device
     gbar = gbar.to(torch.complex128).contiguous()
     eig = eig.contiguous()
+    if _shape_ok(n, batch):
+        _buf(gbar, "gbar", torch.complex128, batch * n)
+        _buf(A_colmajor, "A", torch.complex128, batch * n * n, dev)
+        _buf(eig, "eig", torch.complex128, batch * n, dev)
     dA = torch.empty((batch, n, n), dtype=torch.complex128, device=dev)
     flag = torch.empty((batch,), dtype=torch.int8, device=dev)
     work = torch.empty((workspace_size(n, batch),), dtype=torch.uint8, device=dev)

@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 import torch
 
-from . import Handle, TokuraError, _P, _check, _dev_check, _stream_ptr, default_handle, lib
+from . import Handle, TokuraError, _P, _buf, _check, _dev_check, _stream_ptr, default_handle, lib
 
 
 def pack(mats):
@@ -58,26 +58,16 @@ class RaggedBatch:
                                                _P(self.work.data_ptr()), self.work.numel(), _stream_ptr(stream))
         _check(rc, "tokura_zgeev_ragged_prepare")
 
-    @staticmethod
-    def _sized(t: torch.Tensor, name: str, dtype, numel: int, device):
-        # the kernels take raw pointers: a host tensor, another device or a
-        # short buffer would be read or written out of bounds
-        _dev_check(t, name, dtype)
-        if t.device != device:
-            raise TokuraError(f"{name} is on {t.device}, the batch on {device}")
-        if t.numel() < numel:
-            raise TokuraError(f"{name} holds {t.numel()} elements, the batch needs {numel}")
-
     def _bufs(self, A, eig, flag):
         _dev_check(A, "A", torch.complex128)
         if A.numel() < int(self.offsets.max()):
             raise TokuraError("A is smaller than the offsets require")
         if eig is None:
             eig = torch.empty(int(self.eig_offsets[-1]), dtype=torch.complex128, device=A.device)
-        self._sized(eig, "eig", torch.complex128, int(self.eig_offsets[-1]), A.device)
+        _buf(eig, "eig", torch.complex128, int(self.eig_offsets[-1]), A.device)
         if flag is None:
             flag = torch.empty(self.batch, dtype=torch.int8, device=A.device)
-        self._sized(flag, "flag", torch.int8, self.batch, A.device)
+        _buf(flag, "flag", torch.int8, self.batch, A.device)
         return eig, flag
 
     def solve(self, A: torch.Tensor, eig=None, flag=None, graph: bool = False, stream=None):
@@ -106,10 +96,10 @@ class RaggedBatch:
     def backward(self, schur: torch.Tensor, gbar: torch.Tensor, flag: torch.Tensor, stream=None):
         """dL/dA (flat, laid out like A) for flat cotangents gbar (sum n) of
         L = Re sum conj(gbar) lambda; flag from schur()."""
-        self._sized(schur, "schur", torch.complex128, 2 * int(self.offsets.max()), schur.device)
+        _buf(schur, "schur", torch.complex128, 2 * int(self.offsets.max()), schur.device)
         gbar = gbar.to(torch.complex128).contiguous()
-        self._sized(gbar, "gbar", torch.complex128, int(self.eig_offsets[-1]), schur.device)
-        self._sized(flag, "flag", torch.int8, self.batch, schur.device)
+        _buf(gbar, "gbar", torch.complex128, int(self.eig_offsets[-1]), schur.device)
+        _buf(flag, "flag", torch.int8, self.batch, schur.device)
         dA = torch.empty(int(self.offsets.max()), dtype=torch.complex128, device=schur.device)
         rc = lib().tokura_zgeev_ragged_backward_gpu(self.h.ptr, self.batch, _P(schur.data_ptr()), _P(gbar.data_ptr()),
                                                     _P(dA.data_ptr()), _P(self.work.data_ptr()), _P(flag.data_ptr()),
