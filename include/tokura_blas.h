@@ -157,12 +157,17 @@ int tokuraSetVariant(tokurablas_t *handle, int n, int variant);
  * schedule (size-class tiles in decreasing size, eigenvalue offsets, packed
  * Hessenberg offsets) and copies it into `work` with an asynchronous copy on
  * `stream`.  The schedule stays valid for any number of solves of batches
- * with the same offsets; it uses the handle's current class configuration.
- * tokura_zgeev_ragged_gpu launches, on `stream`: a check of the schedule
- * (one fill kernel), the Hessenberg pass (two launches for n > 32, a
- * workgroup per matrix with the matrix in registers, or with balancing on
- * five, one per LDS size; one launch for n <= 32), and ONE persistent QR
- * kernel for the whole batch
+ * with the same offsets.  It uses the class configuration that the handle's
+ * ragged tuning table holds for the batch's size signature (after
+ * tokuraAutotuneRagged or tokuraLoadTuning), else the handle's current one
+ * (tokuraSetRaggedConfig).
+ * tokura_zgeev_ragged_gpu launches, ordered after earlier work on `stream`
+ * and before later work on it: a check of the schedule (one fill kernel),
+ * the Hessenberg pass (two launches for n > 32, a workgroup per matrix with
+ * the matrix in registers, or with balancing on five, one per LDS size; one
+ * launch for n <= 32; without balancing the three run on the handle's side
+ * streams, forked from and joined back to `stream` with events), and ONE
+ * persistent QR kernel for the whole batch
  * (a workgroup per CU whose LDS is shared by matrices of every size; its
  * wavefronts pull tiles, largest first, from one atomic queue, and a matrix
  * moves to a smaller LDS region as it deflates); tokura_zgeev_ragged_graph
