@@ -189,10 +189,12 @@ int tokura_zgeev_ragged_graph(tokurablas_t *handle, int batch_count, const hipDo
                               hipDoubleComplex *eigenvalues, void *work, char *flag,
                               hipStream_t stream);
 
-/* Class configuration of the ragged path: lanes per matrix of class A1
- * (n 49..64), A2 (33..48), B (17..32), C (1..16) (a wavefront holds
- * 64/lanes matrices of a class) and wavefronts per workgroup of the
- * persistent QR (tokuraGetRaggedWaves).  tokuraAutotuneRagged times every
+/* Class configuration of the ragged path (0..15): lanes per matrix of class
+ * A1 (n 49..64), A2 (33..48), B (17..32), C (1..16) (a wavefront holds
+ * 64/lanes matrices of a class), wavefronts per workgroup of the persistent
+ * QR (tokuraGetRaggedWaves), the deflating matrices' mover and, in 13..15,
+ * a cap of eight wavefronts on n 49..64 while those lead the queue (the
+ * other four take small tiles meanwhile).  tokuraAutotuneRagged times every
  * configuration on the caller's batch (buffers as for
  * tokura_zgeev_ragged_gpu; the stream is synchronised), then the small-class
  * Hessenberg grid (1, 2, 4 or 8 workgroups per resident slot,

@@ -35,7 +35,7 @@ def check_against_lapack(mats, eig_flat, eoff, tol=1e-10):
     return worst
 
 
-@pytest.mark.parametrize("cfg", list(range(13)))
+@pytest.mark.parametrize("cfg", list(range(16)))
 def test_ragged_every_config(cfg):
     rng = np.random.default_rng(100 + cfg)
     sizes = rng.integers(1, 65, 700)

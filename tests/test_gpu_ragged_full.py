@@ -65,17 +65,20 @@ def matched_plain_handle(lanes_b, lanes_c):
     return h
 
 
-def test_ragged_full_config_every_queue_many_rounds():
+@pytest.mark.parametrize("cfg", [None, 14])   # the handle's default; a wavefront cap on n 49..64
+def test_ragged_full_config_every_queue_many_rounds(cfg):
     B = 200_000
     rng = np.random.default_rng(1)
     sizes = rng.integers(4, 65, B)
     flat, offs, buckets = ragged_batch_on_device(sizes, 1)
     h = tk.Handle()
+    if cfg is not None:
+        assert tk.lib().tokuraSetRaggedConfig(h.ptr, cfg) == 0
     rb = RaggedBatch(offs, handle=h)
     # resident matrices of the GPU at the largest class: well below each
     # class's count, so every queue serves many rounds
     per_class = np.bincount(np.digitize(sizes, [17, 33, 41, 49, 57, 61]))
-    print(f"[case] ragged full config 200000, class counts {per_class.tolist()}", flush=True)
+    print(f"[case] ragged full config 200000 cfg={cfg}, class counts {per_class.tolist()}", flush=True)
     assert per_class.min() >= 6 * 1024
     eig, flag = rb.solve(flat, graph=True)
     torch.cuda.synchronize()

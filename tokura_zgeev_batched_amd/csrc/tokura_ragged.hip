@@ -76,27 +76,34 @@ static_assert(sizeof(RaggedHeader) == 184, "header size");
 // matrix per wavefront), B (17..32), C (1..16); wavefronts per workgroup of
 // the persistent QR; the deflating matrices' mover (ChunkMover): 0 compacts
 // the arena at every deflation, 4..7 moves a matrix to the highest free run
-// when its active part needs at most mv/8 of its run
-#define TKZ_RAGGED_CFGS(X)       \
-    X(0, 64, 64, 16, 8, 12, 0)   \
-    X(1, 64, 64, 32, 8, 12, 0)   \
-    X(2, 64, 64, 32, 16, 12, 0)  \
-    X(3, 64, 64, 32, 32, 12, 0)  \
-    X(4, 64, 64, 32, 16, 8, 0)   \
-    X(5, 64, 64, 32, 8, 8, 5)    \
-    X(6, 64, 64, 32, 8, 8, 0)    \
-    X(7, 64, 64, 32, 16, 8, 6)   \
-    X(8, 64, 64, 32, 16, 12, 5)  \
-    X(9, 64, 64, 32, 32, 8, 5)   \
-    X(10, 64, 64, 32, 32, 12, 6) \
-    X(11, 64, 64, 32, 8, 8, 6)   \
-    X(12, 64, 64, 32, 8, 12, 5)
-static const int g_cfg_lpm[][6] = {
-#define X(id, a1, a2, b, c, w, mv) {a1, a2, b, c, w, mv},
+// when its active part needs at most mv/8 of its run; cap: when nonzero, only
+// the first `cap` wavefronts take tiles of n 49..64 from the front of the
+// queue while the front is there, the others take tiles from its back (at
+// n = 64 eight wavefronts per CU are faster than twelve, for the smaller
+// sizes twelve are)
+#define TKZ_RAGGED_CFGS(X)          \
+    X(0, 64, 64, 16, 8, 12, 0, 0)   \
+    X(1, 64, 64, 32, 8, 12, 0, 0)   \
+    X(2, 64, 64, 32, 16, 12, 0, 0)  \
+    X(3, 64, 64, 32, 32, 12, 0, 0)  \
+    X(4, 64, 64, 32, 16, 8, 0, 0)   \
+    X(5, 64, 64, 32, 8, 8, 5, 0)    \
+    X(6, 64, 64, 32, 8, 8, 0, 0)    \
+    X(7, 64, 64, 32, 16, 8, 6, 0)   \
+    X(8, 64, 64, 32, 16, 12, 5, 0)  \
+    X(9, 64, 64, 32, 32, 8, 5, 0)   \
+    X(10, 64, 64, 32, 32, 12, 6, 0) \
+    X(11, 64, 64, 32, 8, 8, 6, 0)   \
+    X(12, 64, 64, 32, 8, 12, 5, 0)  \
+    X(13, 64, 64, 32, 32, 12, 0, 8) \
+    X(14, 64, 64, 32, 16, 12, 0, 8) \
+    X(15, 64, 64, 32, 8, 12, 0, 8)
+static const int g_cfg_lpm[][7] = {
+#define X(id, a1, a2, b, c, w, mv, cap) {a1, a2, b, c, w, mv, cap},
     TKZ_RAGGED_CFGS(X)
 #undef X
 };
-static constexpr int kNumRaggedCfg = 13;
+static constexpr int kNumRaggedCfg = 16;
 
 static int cfg_lpm(int cfg, int cls) { return cls < kClsB ? 64 : g_cfg_lpm[cfg][cls == kClsB ? 2 : 3]; }
 
@@ -245,7 +252,7 @@ __device__ __forceinline__ void claim(const RaggedView &V, bool back, int lane, 
 // several small matrices.  A wavefront waits only while it holds no
 // chunks, so every wait ends; every wavefront leaves when the two ends of
 // the queue meet.
-template <int W, int LB, int LC, int MV>
+template <int W, int LB, int LC, int MV, int CAP>
 __global__ __launch_bounds__(64 * W) void k_ragged_qr_pool(char *work, zd *__restrict__ eig,
                                                            int8_t *__restrict__ flag, int itmax)
 {
@@ -259,6 +266,8 @@ __global__ __launch_bounds__(64 * W) void k_ragged_qr_pool(char *work, zd *__res
     __syncthreads();
     const ChunkPool P{bits, &lock, &freed, arena, lane};
     const int nt = V.h->ntiles;
+    // a wavefront past the first CAP takes back tiles while the front is in n 49..64
+    const bool back_first = CAP > 0 && __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6) >= CAP;
     // tile t: its descriptor (uniform) and the chunks its QR needs
     auto desc = [&](int t) {
         const int4 tv = V.tiles[t];
@@ -305,32 +314,50 @@ __global__ __launch_bounds__(64 * W) void k_ragged_qr_pool(char *work, zd *__res
     int4 pd = make_int4(0, 0, 0, 0);
     bool back_open = true;
     for (;;) {
-        if (pend < 0) {
-            claim(V, false, lane, nt, pend);
-            if (pend < 0) break;   // the two ends met: no tile is left anywhere
-            pd = desc(pend);
-            pk = need(pd);
-        }
-        const unsigned seen = P.frees();
-        int4 td;
-        int k, s0 = P.try_alloc(pk, false);
-        if (s0 >= 0) {
-            td = pd;
-            k = pk;
-            pend = -1;
-        } else {
-            int bt = -1;
-            if (back_open) {
+        int4 td = make_int4(0, 0, 0, 0);
+        int k = 0, s0 = -1;
+        if (back_first && pend < 0 && back_open) {
+            const unsigned long long qe = __hip_atomic_load(queue_ends(V), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int f = (int)__builtin_amdgcn_readfirstlane((unsigned)qe);   // tiles taken from the front
+            if (f < nt && desc(f).x <= 2) {   // classes 0-2: n 49..64
+                int bt = -1;
                 claim(V, true, lane, nt, bt);
-                back_open = bt >= 0;
+                if (bt >= 0) {
+                    td = desc(bt);
+                    k = need(td);
+                    s0 = P.alloc_wait(k);
+                } else {
+                    back_open = false;
+                }
             }
-            if (bt < 0) {
-                P.wait_free(seen);
-                continue;
+        }
+        if (s0 < 0) {
+            if (pend < 0) {
+                claim(V, false, lane, nt, pend);
+                if (pend < 0) break;   // the two ends met: no tile is left anywhere
+                pd = desc(pend);
+                pk = need(pd);
             }
-            td = desc(bt);
-            k = need(td);
-            s0 = P.alloc_wait(k);
+            const unsigned seen = P.frees();
+            s0 = P.try_alloc(pk, false);
+            if (s0 >= 0) {
+                td = pd;
+                k = pk;
+                pend = -1;
+            } else {
+                int bt = -1;
+                if (back_open) {
+                    claim(V, true, lane, nt, bt);
+                    back_open = bt >= 0;
+                }
+                if (bt < 0) {
+                    P.wait_free(seen);
+                    continue;
+                }
+                td = desc(bt);
+                k = need(td);
+                s0 = P.alloc_wait(k);
+            }
         }
         run(td, s0, k);   // the only call: one copy of the QR code
     }
@@ -481,7 +508,7 @@ static RaggedFork handle_fork(tokurablas_t *h)
     return fk;
 }
 
-template <int LA1, int LA2, int LB, int LC, int W, int MV>
+template <int LA1, int LA2, int LB, int LC, int W, int MV, int CAP>
 static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *flag, int itmax, int bal, int cus,
                                     const RaggedPrepared &pe, hipStream_t s, const RaggedFork &fk)
 {
@@ -520,7 +547,7 @@ static hipError_t launch_ragged_cfg(const zd *A, char *work, zd *eig, int8_t *fl
     if (e != hipSuccess) return e;
     if (fork && fk.end(s) != hipSuccess) return hipErrorUnknown;   // join the branches
     // one workgroup per CU: the arena is the CU's whole LDS
-    hipLaunchKernelGGL((k_ragged_qr_pool<W, LB, LC, MV>), dim3(cus), dim3(64 * W), 0, s, work, eig, flag, itmax);
+    hipLaunchKernelGGL((k_ragged_qr_pool<W, LB, LC, MV, CAP>), dim3(cus), dim3(64 * W), 0, s, work, eig, flag, itmax);
     return hipGetLastError();
 }
 
@@ -528,8 +555,8 @@ static hipError_t launch_ragged(const RaggedPrepared &pe, const zd *A, char *wor
                                 int bal, int cus, hipStream_t s, const RaggedFork &fk = RaggedFork())
 {
     switch (pe.cfg) {
-#define X(id, a1, a2, b, c, w, mv) \
-    case id: return launch_ragged_cfg<a1, a2, b, c, w, mv>(A, work, eig, flag, itmax, bal, cus, pe, s, fk);
+#define X(id, a1, a2, b, c, w, mv, cap) \
+    case id: return launch_ragged_cfg<a1, a2, b, c, w, mv, cap>(A, work, eig, flag, itmax, bal, cus, pe, s, fk);
         TKZ_RAGGED_CFGS(X)
 #undef X
     default: return hipErrorInvalidValue;
