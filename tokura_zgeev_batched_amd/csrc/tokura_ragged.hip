@@ -439,7 +439,7 @@ __global__ __launch_bounds__(256) void k_ragged_hess_wg(const zd *__restrict__ A
 // matrix, the grid sized on the host from the schedule; the range itself
 // comes from the header (empty for a stale schedule).
 template <int NBLK>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ragged_hess_reg(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kHregWaves<NBLK>, kHregWaves<NBLK>))) void k_ragged_hess_reg(
     const zd *__restrict__ A, char *work, int8_t *__restrict__ flag, zd *__restrict__ eig, int c_lo, int c_hi)
 {
     __shared__ HregSmem S;

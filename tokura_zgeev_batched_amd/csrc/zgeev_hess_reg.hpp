@@ -271,20 +271,23 @@ __device__ __forceinline__ void hreg_step(zd (&R)[NBLK][NBLK], HregSmem &S, int 
     }
 }
 
-// Steps i with (i + 1) / 16 == K, then the next phase: one loop per K, so the
-// loop body has a single path and R stays in place.
+// Steps i0 <= i <= G - 2 of the grid (G = 16 NBLK) with (i + 1) / 16 == K,
+// then the next phase: one loop per K, so the loop body has a single path
+// and R stays in place.
 template <int NBLK, int K = 0>
-__device__ __forceinline__ void hreg_phases(zd (&R)[NBLK][NBLK], HregSmem &S, int n, int wv, int lane, int q,
+__device__ __forceinline__ void hreg_phases(zd (&R)[NBLK][NBLK], HregSmem &S, int i0, int wv, int lane, int q,
                                             int tr, int tc)
 {
+    constexpr int G = 16 * NBLK;
     if constexpr (K < NBLK) {
-        const int i1 = min(n - 2, 16 * K + 14);
-        for (int i = K == 0 ? 0 : 16 * K - 1; i <= i1; ++i) {
+        const int i1 = 16 * K + 14 < G - 2 ? 16 * K + 14 : G - 2;
+        const int ib = K == 0 ? 0 : 16 * K - 1;
+        for (int i = ib > i0 ? ib : i0; i <= i1; ++i) {
             __syncthreads();   // v and tau of step i are in LDS
             hreg_step<NBLK, K>(R, S, i, wv, lane, tr, tc);
-            if (i + 3 <= n) hreg_reflect<NBLK>(R, S, i + 1, wv, q, tr);
+            if (i + 3 <= G) hreg_reflect<NBLK>(R, S, i + 1, wv, q, tr);
         }
-        hreg_phases<NBLK, K + 1>(R, S, n, wv, lane, q, tr, tc);
+        hreg_phases<NBLK, K + 1>(R, S, i0, wv, lane, q, tr, tc);
     }
 }
 
@@ -293,22 +296,39 @@ __device__ __forceinline__ void hreg_phases(zd (&R)[NBLK][NBLK], HregSmem &S, in
 // reduction, the packed Hessenberg matrix to dst and the flag (0, or 2 with
 // the unreduced matrix in dst).  Ends with a barrier, so S can serve the
 // next matrix at once.
+// The matrix sits at the bottom right of the G x G grid (G = 16 NBLK):
+// element (r, c) at grid (r + p, c + p), p = G - n, the rows and columns
+// above and left of it zero.  The reduction then runs grid steps p .. G - 2,
+// the last n - 1 steps of a G x G reduction, whose active blocks shrink
+// with the step: with the padding at the bottom right instead, every step
+// of an n = 49 matrix would update all four 16-row blocks.  At n = G the
+// two placements are the same.
+// OFF = false: the matrix at the top left (p = 0; for n = 16 NBLK, where
+// the placements agree, with fewer registers).
+// Wavefronts per SIMD of the kernels around it: four 256-thread workgroups
+// per CU at NBLK = 3 (108 VGPRs); three at NBLK = 4, where the 168-VGPR
+// budget holds the 4 x 4 blocks without spills (at four workgroups the
+// 128-VGPR budget spilled 15 VGPRs, 88 with the offset, and ran 4% slower
+// at n = 64).
 template <int NBLK>
+constexpr int kHregWaves = NBLK == 4 ? 3 : 4;
+template <int NBLK, bool OFF = true>
 __device__ __forceinline__ void hess_reg(HregSmem &S, const zd *__restrict__ src, int n, zd *__restrict__ dst,
                                          int8_t *__restrict__ flagp, zd *__restrict__ eig0)
 {
     const int tid = (int)threadIdx.x;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63, q = lane >> 4, tr = lane & 15, tc = 4 * wv + q;
+    const int p = OFF ? 16 * NBLK - n : 0;   // grid offset of the matrix
     zd R[NBLK][NBLK];
     double chk = 0.0, mx = 0.0;
 #pragma unroll
     for (int b = 0; b < NBLK; ++b)
 #pragma unroll
         for (int a = 0; a < NBLK; ++a) {
-            const int r = tr + 16 * a, c = tc + 16 * b;
-            const zd x = src[min(c, n - 1) * n + min(r, n - 1)];   // in bounds; zero outside n x n
-            R[a][b] = (r < n && c < n) ? x : mk(0.0, 0.0);
+            const int r = tr + 16 * a - p, c = tc + 16 * b - p;
+            const zd x = src[max(c, 0) * n + max(r, 0)];   // in bounds; zero outside n x n
+            R[a][b] = (r >= 0 && c >= 0) ? x : mk(0.0, 0.0);
             chk += (R[a][b].re - R[a][b].re) + (R[a][b].im - R[a][b].im);   // NaN iff an entry is not finite
             mx = fmax(mx, fmax(fabs(R[a][b].re), fabs(R[a][b].im)));
         }
@@ -326,8 +346,9 @@ __device__ __forceinline__ void hess_reg(HregSmem &S, const zd *__restrict__ src
     }
     if (ok && tid == 0) eig0[0] = mk(ldexp(1.0, -se), 0.0);
     if (ok) {
-        if (n >= 2) hreg_reflect<NBLK>(R, S, 0, wv, q, tr);
-        hreg_phases<NBLK>(R, S, n, wv, lane, q, tr, tc);
+        // column p < 16 (n > 16 (NBLK - 1)) is in block 0
+        if (n >= 2 && wv == ((p & 15) >> 2)) hreg_reflect_b<NBLK, 0>(R, S, p, q, tr);
+        hreg_phases<NBLK>(R, S, p, wv, lane, q, tr, tc);
     }
     const PackedLayout pk;
     int tr_o = tr, tc_o = tc;
@@ -338,8 +359,8 @@ __device__ __forceinline__ void hess_reg(HregSmem &S, const zd *__restrict__ src
     for (int b = 0; b < NBLK; ++b)
 #pragma unroll
         for (int a = 0; a < NBLK; ++a) {
-            const int r = tr_o + 16 * a, c = tc_o + 16 * b;
-            if (r < n && c < n && r <= c + 1) dst[pk(r, c)] = (r == c + 1 && ok) ? mk(S.beta[c], 0.0) : R[a][b];
+            const int r = tr_o + 16 * a - p, c = tc_o + 16 * b - p;
+            if (r >= 0 && c >= 0 && r <= c + 1) dst[pk(r, c)] = (r == c + 1 && ok) ? mk(S.beta[c + p], 0.0) : R[a][b];
         }
     if (tid == 0) *flagp = ok ? 0 : 2;
     __syncthreads();

@@ -166,16 +166,16 @@ __global__ __launch_bounds__(NT) void k_hessenberg_packed_wg(int n, int batch, c
 // Stage 1 with the matrix in registers (hess_reg, n <= 16 NBLK, no
 // balancing): 256 threads per matrix and 6 KiB of LDS, so the register file
 // (about 128 VGPRs per thread, four workgroups per CU) bounds the occupancy.
-template <int NBLK>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_hessenberg_reg(int n, int batch, const zd *__restrict__ A,
+template <int NBLK, bool OFF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kHregWaves<NBLK>, kHregWaves<NBLK>))) void k_hessenberg_reg(int n, int batch, const zd *__restrict__ A,
                                                         zd *__restrict__ work, int8_t *__restrict__ flag,
                                                         zd *__restrict__ eig, int *qctr = nullptr)
 {
     if (qctr && blockIdx.x == 0 && threadIdx.x == 0) *qctr = 0;
     __shared__ HregSmem S;
     const long long mat = blockIdx.x;
-    hess_reg<NBLK>(S, A + mat * (long long)n * n, n, work + mat * (long long)packed_size(n), flag + mat,
-                   eig + mat * n);
+    hess_reg<NBLK, OFF>(S, A + mat * (long long)n * n, n, work + mat * (long long)packed_size(n), flag + mat,
+                        eig + mat * n);
 }
 
 // Stage 2 of the split path: QR on the packed Hessenberg matrix (DUAL: two
@@ -328,10 +328,14 @@ static void launch_hess_wg(int n, int batch, const zd *A, zd *work, int8_t *flag
                            hipStream_t s)
 {
     if (bal == 0 && n > 32) {   // the register-resident reduction
-        if (n <= 48)
-            hipLaunchKernelGGL((k_hessenberg_reg<3>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag, eig, qctr);
+        if (n < 48)
+            hipLaunchKernelGGL((k_hessenberg_reg<3, true>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag, eig, qctr);
+        else if (n == 48)
+            hipLaunchKernelGGL((k_hessenberg_reg<3, false>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag, eig, qctr);
+        else if (n < 64)
+            hipLaunchKernelGGL((k_hessenberg_reg<4, true>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag, eig, qctr);
         else
-            hipLaunchKernelGGL((k_hessenberg_reg<4>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag, eig, qctr);
+            hipLaunchKernelGGL((k_hessenberg_reg<4, false>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag, eig, qctr);
         return;
     }
     const int nb = n <= 40 ? 40 : (n <= 48 ? 48 : (n <= 56 ? 56 : (n <= 60 ? 60 : 64)));
