@@ -249,9 +249,12 @@ __device__ __forceinline__ void claim(const RaggedView &V, bool back, int lane, 
 // those in whatever room there is, so the small matrices fill the gaps that
 // the large ones leave.  A tile is one matrix of n > 32 (the two-bulge
 // sweeps, moving up to a smaller run as the matrix deflates: ChunkMover) or
-// several small matrices.  A wavefront waits only while it holds no
-// chunks, so every wait ends; every wavefront leaves when the two ends of
-// the queue meet.
+// several small matrices.  With CAP > 0, wavefronts CAP.. take tiles from
+// the back first while the front tile is of n 49..64 (a peek at the front
+// counter), so that at most CAP wavefronts work on those matrices while
+// they lead the queue.  A wavefront waits only while it holds no chunks, so
+// every wait ends; every wavefront leaves when the two ends of the queue
+// meet.
 template <int W, int LB, int LC, int MV, int CAP>
 __global__ __launch_bounds__(64 * W) void k_ragged_qr_pool(char *work, zd *__restrict__ eig,
                                                            int8_t *__restrict__ flag, int itmax)
