@@ -95,6 +95,51 @@ def test_closed_forms(handle):
     assert not flag.any() and np.all(e == 0)
 
 
+@pytest.mark.parametrize("n", [33, 40, 47, 49, 55, 63])
+def test_structured_inputs_offset_hessenberg(handle, n):
+    """Sizes between multiples of 16, where the register Hessenberg places the
+    matrix at the bottom right of its grid (zero rows and columns above and
+    left of it): structured inputs whose reflectors are trivial or whose
+    spectrum is known, against closed forms and host LAPACK, through the
+    same-size path and the ragged path."""
+    rng = np.random.default_rng(5000 + n)
+    U = np.triu(rand(rng, 1, n)[0])
+    D = np.diag(np.diag(U))
+    Z = np.zeros((n, n), complex)
+    R = rand(rng, 1, n)[0]
+    R[:, n // 3] = 0.0                      # a zero column: reducible, eigenvalue 0
+    # (a zero row as well would make 0 defective, its error ~sqrt(eps) ||A||)
+    Hs = np.triu(rand(rng, 1, n)[0], -1)    # already upper Hessenberg
+    P = np.zeros((n, n), complex)           # cyclic permutation: roots of unity
+    P[1:, :-1] = np.eye(n - 1)
+    P[0, -1] = 1.0
+    M = np.stack([U, D, Z, R, Hs, P])
+    ref_hs, _ = oracle.lapack_zgeev(M[4:5])
+
+    def check(e):
+        np.testing.assert_allclose(np.sort_complex(e[0]), np.sort_complex(np.diag(U)), rtol=1e-13, atol=1e-13)
+        np.testing.assert_array_equal(e[1], np.diag(D))
+        assert np.all(e[2] == 0)
+        # eigenvalue 0 of R comes back as a rounding-size number: errors in
+        # units of kappa_i eps ||A||_F (as the hard-spectra tests)
+        assert float(oracle.condscaled_err(M[3:4], e[3:4]).max()) <= 100.0
+        assert oracle.max_relerr(ref_hs, e[4:5]) <= TOL_LAPACK
+        assert oracle.max_relerr(np.exp(2j * np.pi * np.arange(n) / n)[None], e[5:6]) < 1e-12
+
+    e, flag = gpu_eigs(M, handle)
+    assert not flag.any()
+    check(e)
+    from tokura_zgeev_batched_amd.ragged import RaggedBatch, pack
+    mats = [M[i] for i in range(len(M))] + [rand(rng, 1, k)[0] for k in (5, 20, 64)]
+    flat, offs = pack(mats)
+    rb = RaggedBatch(offs, handle=tk.Handle())
+    er, fr = rb.solve(torch.tensor(flat, device="cuda"))
+    torch.cuda.synchronize()
+    assert int((fr != 0).sum()) == 0
+    er = er.cpu().numpy()
+    check(np.stack([er[rb.eig_offsets[i]:rb.eig_offsets[i + 1]] for i in range(len(M))]))
+
+
 SCALES = (1e-305, 1e-300, 1e-290, 1e-250, 1e-200, 1e-150, 1e-120, 1e-30, 1e30, 1e120, 1e150, 1e200, 1e250,
           1e290, 1e300)
 
