@@ -128,126 +128,6 @@ __device__ void dhessenberg(double *A, const FullLayout lay, int n, int t)
     }
 }
 
-// DGEHD2 by a whole workgroup of NT threads on one LDS matrix: the real
-// counterpart of hessenberg_wg (teams of TPR threads share each row in the
-// right update and each column in the left update; the team of column i+1
-// also forms the next reflector's tail norm; three barriers per column;
-// entries below the sub-diagonal zeroed once at the end).  `red`: LDS
-// scratch of at least 2 doubles.
-template <int NT, int TPR>
-__device__ void dhessenberg_wg(double *A, const FullLayout lay, int n, double *red)
-{
-    const int tid = (int)threadIdx.x;
-    const int q = tid % TPR, row = tid / TPR;
-    const int ld = lay.ld;
-    auto tail_norm = [&](int col, int r0) {   // sum of squares and max of A(r0.., col), by team 0
-        if (row == 0) {
-            double ssq = 0.0, amx = 0.0;
-            for (int r = r0 + q; r < n; r += TPR) {
-                const double v = A[lay(r, col)];
-                ssq += v * v;
-                amx = fmax(amx, fabs(v));
-            }
-#pragma unroll
-            for (int o = 1; o < TPR; o <<= 1) {
-                ssq += __shfl_xor(ssq, o, TPR);
-                amx = fmax(amx, __shfl_xor(amx, o, TPR));
-            }
-            if (q == 0) {
-                red[0] = ssq;
-                red[1] = amx;
-            }
-        }
-    };
-    tail_norm(0, 2);
-    __syncthreads();
-    for (int i = 0; i + 2 <= n; ++i) {
-        const int m = n - 1 - i;
-        const double ssq = red[0], amx = red[1];
-        const double alpha = A[lay(i + 1, i)];
-        double xnorm = 0.0;
-        if (amx > 0.0) {
-            if (ssq > 1e-290 && ssq < 1e290) {
-                xnorm = sqrt(ssq);
-            } else {   // rare: rescaled second pass
-                const double ia = 1.0 / amx;
-                double s2 = 0.0;
-                if (row == 0)
-                    for (int r = i + 2 + q; r < n; r += TPR) {
-                        const double v = A[lay(r, i)] * ia;
-                        s2 += v * v;
-                    }
-#pragma unroll
-                for (int o = 1; o < TPR; o <<= 1) s2 += __shfl_xor(s2, o, TPR);
-                __syncthreads();   // everyone has read red[]
-                if (tid == 0) red[0] = s2;
-                __syncthreads();
-                xnorm = amx * sqrt(red[0]);
-            }
-        }
-        if (xnorm == 0.0) {   // tau = 0, H(i) = I; next tail norm
-            __syncthreads();   // everyone has read red[] and alpha
-            tail_norm(i + 1, i + 3);
-            __syncthreads();
-            continue;
-        }
-        const double beta = -copysign(dlapy2_d(alpha, xnorm), alpha);
-        const double tau = (beta - alpha) / beta;
-        const double scal = 1.0 / (alpha - beta);
-        for (int r = i + 2 + tid; r < n; r += NT) A[lay(r, i)] *= scal;
-        __syncthreads();   // v is scaled; red[] and alpha have been read
-        const double *vcol = A + lay(i + 1, i);   // v(c) = vcol[c], c >= 1, v(0) = 1
-        // right: rows r = row (+ NT/TPR ...), team lanes split the columns
-        for (int r = row; r < n; r += NT / TPR) {
-            double *rp = A + r;
-            double u = q == 0 ? rp[(i + 1) * ld] : 0.0;
-            for (int c = (q == 0 ? TPR : q); c < m; c += TPR) u += rp[(i + 1 + c) * ld] * vcol[c];
-#pragma unroll
-            for (int o = 1; o < TPR; o <<= 1) u += __shfl_xor(u, o, TPR);
-            u *= tau;
-            if (q == 0) rp[(i + 1) * ld] -= u;
-            for (int c = (q == 0 ? TPR : q); c < m; c += TPR) rp[(i + 1 + c) * ld] -= u * vcol[c];
-        }
-        __syncthreads();
-        // left: columns c = i+1+row (...), team lanes split the rows; the team
-        // of column i+1 also accumulates its tail (rows i+3..) for the next step
-        for (int c = i + 1 + row; c < n; c += NT / TPR) {
-            double *col = A + lay(i + 1, c);
-            double sv = q == 0 ? col[0] : 0.0;
-            for (int r = (q == 0 ? TPR : q); r < m; r += TPR) sv += vcol[r] * col[r];
-#pragma unroll
-            for (int o = 1; o < TPR; o <<= 1) sv += __shfl_xor(sv, o, TPR);
-            sv *= tau;
-            if (q == 0) col[0] -= sv;
-            double s = 0.0, mx = 0.0;
-            for (int r = (q == 0 ? TPR : q); r < m; r += TPR) {
-                const double x = col[r] - vcol[r] * sv;
-                col[r] = x;
-                if (r >= 2) {
-                    s += x * x;
-                    mx = fmax(mx, fabs(x));
-                }
-            }
-            if (c == i + 1) {   // the whole team takes this branch together
-#pragma unroll
-                for (int o = 1; o < TPR; o <<= 1) {
-                    s += __shfl_xor(s, o, TPR);
-                    mx = fmax(mx, __shfl_xor(mx, o, TPR));
-                }
-                if (q == 0) {
-                    red[0] = s;
-                    red[1] = mx;
-                }
-            }
-        }
-        if (tid == 0) A[lay(i + 1, i)] = beta;
-        __syncthreads();
-    }
-    for (int c = 0; c + 2 < n; ++c)
-        for (int r = c + 2 + tid; r < n; r += NT) A[lay(r, c)] = 0.0;
-    __syncthreads();
-}
-
 // DLANV2: eigenvalues (rt1r + i rt1i, rt2r + i rt2i) of the real 2x2
 // [a b; c d] in standard form, a complex pair with the positive imaginary
 // part first.  Only the eigenvalues are kept (the rotation is not needed

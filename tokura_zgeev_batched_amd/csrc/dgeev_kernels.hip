@@ -3,10 +3,12 @@
 // counterpart in the reference, which solves complex matrices only.
 //
 // Two launches, as the complex split path:
-//  1. k_dhess (k_dhess_wg for n > 32: a 256-thread workgroup per matrix)
-//              loads each matrix into LDS (full layout), checks it is finite,
-//              reduces it to upper Hessenberg form (DGEHD2) and writes the
-//              packed Hessenberg matrix (column c: rows 0..c+1) to `work`;
+//  1. k_dhess (lane groups, n <= 32) loads each matrix into LDS (full
+//              layout), checks it is finite, reduces it to upper Hessenberg
+//              form (DGEHD2) and writes the packed Hessenberg matrix (column
+//              c: rows 0..c+1) to `work`; for n > 32 k_dhess_reg does the
+//              same with the matrix in the registers of a 256-thread
+//              workgroup (dgeev_hess_reg.hpp);
 //  2. k_dqr    loads the packed matrix into a BulgeLayout LDS buffer (two
 //              spare rows per column for the double-shift bulge) and runs
 //              DLAHQR (Francis double shift, DLANV2 for converged 2x2
@@ -14,6 +16,7 @@
 // Lanes per matrix = NB (16, 32 or 64): 4, 2 or 1 matrices per wavefront.
 #include "tokura_internal.hpp"
 #include "dgeev_device.hpp"
+#include "dgeev_hess_reg.hpp"
 
 namespace tkz {
 
@@ -48,34 +51,16 @@ __global__ __launch_bounds__(64) void k_dhess(int n, int batch, const double *__
     if (t == 0) flag[mat] = ok ? 0 : 2;
 }
 
-// Stage 1 for n > 32: one matrix per workgroup of NT threads (dhessenberg_wg).
-template <int NB, int NT>
-__global__ __launch_bounds__(NT) void k_dhess_wg(int n, int /*batch: one workgroup per matrix*/, const double *__restrict__ A,
-                                                 double *__restrict__ work, int8_t *__restrict__ flag)
+// Stage 1 for n > 32 with the matrix in registers (dhess_reg): one
+// 256-thread workgroup per matrix, 16 (NBLK - 1) < n <= 16 NBLK.
+template <int NBLK, bool OFF>
+__global__ __launch_bounds__(256) void k_dhess_reg(int n, int /*batch: one workgroup per matrix*/,
+                                                   const double *__restrict__ A, double *__restrict__ work,
+                                                   int8_t *__restrict__ flag)
 {
-    constexpr int LDP = NB | 1;
-    __shared__ double lds[NB * LDP];
-    __shared__ double red[2];
+    __shared__ DHregSmem S;
     const long long mat = blockIdx.x;
-    const int tid = (int)threadIdx.x;
-    const FullLayout lay{n | 1};
-    const int nn = n * n;
-    const double *src = A + mat * nn;
-    bool bad = false;
-    for (int e = tid; e < nn; e += NT) {
-        const double v = src[e];
-        bad |= !isfinite(v);
-        lds[lay(e % n, e / n)] = v;
-    }
-    const int ok = __syncthreads_or(bad) == 0;
-    if (ok) dhessenberg_wg<NT, NT / 64>(lds, lay, n, red);
-    double *dst = work + mat * (long long)packed_size(n);
-    const PackedLayout pk;
-    for (int c = 0; c < n; ++c) {
-        const int rows = c + 2 < n ? c + 2 : n;
-        for (int r = tid; r < rows; r += NT) dst[pk(r, c)] = lds[lay(r, c)];
-    }
-    if (tid == 0) flag[mat] = ok ? 0 : 2;
+    dhess_reg<NBLK, OFF>(S, A + mat * (long long)n * n, n, work + mat * (long long)packed_size(n), flag + mat);
 }
 
 template <int NB>
@@ -113,6 +98,17 @@ __global__ __launch_bounds__(64) void k_dqr(int n, int batch, const double *__re
 
 static int dgeev_nb(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 
+// The register Hessenberg for 32 < n <= 64 (the offset placement below the
+// block multiples 48 and 64).
+static void launch_dhess_reg(int n, int batch, const double *A, double *work, int8_t *flag, hipStream_t s)
+{
+    const dim3 grid(batch), block(256);
+    if (n < 48) hipLaunchKernelGGL((k_dhess_reg<3, true>), grid, block, 0, s, n, batch, A, work, flag);
+    else if (n == 48) hipLaunchKernelGGL((k_dhess_reg<3, false>), grid, block, 0, s, n, batch, A, work, flag);
+    else if (n < 64) hipLaunchKernelGGL((k_dhess_reg<4, true>), grid, block, 0, s, n, batch, A, work, flag);
+    else hipLaunchKernelGGL((k_dhess_reg<4, false>), grid, block, 0, s, n, batch, A, work, flag);
+}
+
 static hipError_t launch_dgeev(int n, int batch, const double *A, double *wr, double *wi, double *work,
                                int8_t *flag, int itmax, hipStream_t s, int *stage)
 {
@@ -122,7 +118,7 @@ static hipError_t launch_dgeev(int n, int batch, const double *A, double *wr, do
         const dim3 grid((batch + 64 / nb - 1) / (64 / nb));                                                \
         *stage = 1;                                                                                        \
         if (nb == 64)                                                                                      \
-            hipLaunchKernelGGL((k_dhess_wg<64, 256>), dim3(batch), dim3(256), 0, s, n, batch, A, work, flag); \
+            launch_dhess_reg(n, batch, A, work, flag, s);                                                  \
         else                                                                                               \
             hipLaunchKernelGGL((k_dhess<nb>), grid, dim3(64), 0, s, n, batch, A, work, flag);              \
         hipError_t e = hipGetLastError();                                                                  \
