@@ -184,6 +184,34 @@ __device__ __forceinline__ void zlarfg2(zd x1, double x2, zd &t1, zd &v2, double
     if (ident) { t1 = mk(0.0, 0.0); v2 = mk(x2, 0.0); t2 = 0.0; beta = x1.re; }
 }
 
+// zlarfg2 for the two-bulge chain: the same arithmetic, with ZLARFG's tau = 0
+// (H = I for x2 = Im x1 = 0) kept off the fast range only, where it also
+// covers x = 0.  In the fast range such an x gets beta = -x1, tau = 2 and
+// v2 = 0: the reflection diag(-1, 1), an exact sign change of row and column
+// k.  Every later reflector, window, shift and deflation test of the QR then
+// sees the same values up to signs (negation is exact, rounding symmetric),
+// so the eigenvalues keep their bits, and the hot path loses the select of
+// the tau = 0 case.  H(k, k-1) = beta + 0i in both cases.
+__device__ __forceinline__ void zlarfg2c(zd x1, double x2, zd &t1, zd &v2, double &t2, double &beta)
+{
+    const double ss = fma(x1.re, x1.re, fma(x1.im, x1.im, x2 * x2));
+    if (ss > 1e-290 && ss < 1e290) {
+        const double y = rsq_nr(ss);
+        double nrm = ss * y;
+        nrm = fma(fma(-nrm, nrm, ss), 0.5 * y, nrm);
+        beta = -copysign(nrm, x1.re);
+        const double ib = -copysign(y, x1.re);
+        const double zr = x1.re - beta;
+        const double iz = rcp_nr(zr * zr + x1.im * x1.im);
+        t1 = mk((beta - x1.re) * ib, -x1.im * ib);
+        v2 = mk(x2 * (zr * iz), x2 * (-x1.im * iz));
+        t2 = -x2 * ib;
+    } else {
+        bool ident;
+        zlarfg2(x1, x2, t1, v2, t2, beta, ident);
+    }
+}
+
 // Application of a 2-element reflector to one element pair (x, y):
 //   s = c1 x + t2 y,  x <- x - s,  y <- y - s v
 // with (c1, v) = (conj(tau), v2) from the left (rows k, k+1) and (tau, conj(v2))
@@ -196,6 +224,21 @@ __device__ __forceinline__ void pair_op(zd &x, zd &y, const zd c1, const double 
     x.im -= si;
     y.re = fma(-sr, v.re, fma(si, v.im, y.re));
     y.im = fma(-sr, v.im, fma(-si, v.re, y.im));
+}
+
+// pair_op with the parts of c1 and v as separate values, so that a caller can
+// pass negated parts (conjugates, per-lane signs) that the compiler folds into
+// the FMAs' sign modifiers instead of forming new register pairs; the same
+// roundings as pair_op for the same values.
+__device__ __forceinline__ void pair_op_s(zd &x, zd &y, const double cr, const double ci, const double t2,
+                                          const double vr, const double vi)
+{
+    const double sr = fma(cr, x.re, fma(-ci, x.im, t2 * y.re));
+    const double si = fma(cr, x.im, fma(ci, x.re, t2 * y.im));
+    x.re -= sr;
+    x.im -= si;
+    y.re = fma(-sr, vr, fma(si, vi, y.re));
+    y.im = fma(-sr, vi, fma(-si, vr, y.im));
 }
 
 // ---- ZGEHD2 on an n x n matrix in LDS (full layout), LPM lanes ----
@@ -1222,16 +1265,6 @@ __device__ __forceinline__ void group_halves(double v, double &lo, double &hi)
     else bcast_halves16(v, lo, hi);
 }
 
-// x with its sign bit xor-ed with m (0 or 0x80000000).
-__device__ __forceinline__ double flip_sign(double x, unsigned m)
-{
-    union { double d; unsigned w[2]; } u;
-    u.d = x;
-    u.w[1] ^= m;
-    return u.d;
-}
-
-
 // ---- QR with two bulges per sweep (LPM = 64: one matrix per wavefront;
 // LPM = 32: two, n <= 32) ----
 // A "dual sweep" over an active block [l, i] is two consecutive single-shift
@@ -1384,10 +1417,16 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                 x1 = mk(h11s.re / sc, h11s.im / sc);
                 x2 = h21 / sc;
             };
-            // bulk operands of the bulge at k (cbks = cb(k)): from the left on
-            // rows k, k+1 of columns p >= k+2, from the right on columns k, k+1
-            // of rows l <= p <= k-1
-            auto phase_addr = [&](const int k, const int cbks, bool &row, bool &col, int &ax, int &ay) {
+            // bulk operands of the bulge at k: from the left on rows k, k+1 of
+            // columns p >= k+2 (elements cb(p) + k, + 1), from the right on
+            // columns k, k+1 of rows l <= p <= k-1 (cb(k) + p, cb(k+1) + p).
+            // Bulge A's k advances by one per step, so both candidates are
+            // running pointers: the row form advances by one element, the
+            // column form by k + 2 (cb(k+1) = cb(k) + k + 2), and the second
+            // operand of each form is the first one of the next step.
+            zd *pnr = H + cbp + l, *pnc = H + lay.cb(l) + p;
+            zd *const pidle = H + idle_slot;   // DENSE: the idle lanes' element
+            auto phase_addr = [&](const int k, bool &row, bool &col, zd *&ax, zd *&ay) {
                 if constexpr (WANTT) {        // columns k+2..n-1, rows 0..k-1
                     row = p >= k + 2 && p < n;
                     col = p <= k - 1;
@@ -1395,62 +1434,74 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                     row = pin && p >= k + 2;
                     col = pin && p <= k - 1;
                 }
-                // both candidates, then one select each (a select of the
-                // scalar parts first would need their copies to VGPRs)
-                const int axr = cbp + k, axc = cbks + p;
-                const int ayr = axr + 1, ayc = axc + (k + 2);
-                ax = row ? axr : axc;
-                ay = row ? ayr : ayc;
+                zd *const nr = pnr + 1, *const nc = pnc + (k + 2);
+                ax = row ? pnr : pnc;
+                ay = row ? nr : nc;
+                pnr = nr;
+                pnc = nc;
                 if constexpr (DENSE) {
-                    if (!(row || col)) ax = ay = idle_slot;
+                    if (!(row || col)) ax = ay = pidle;
                 }
             };
-            auto phase_apply = [&](const bool row, const bool col, const int ax, const int ay, zd x, zd y,
-                                   const zd tr, const zd vr, const double t2r) {
+            // sg = -1 from the left (c1 = conj(tau), v = v2), +1 from the
+            // right (c1 = tau, v = conj(v2)): sign changes are exact, so the
+            // roundings are those of pair_op with the conjugated values
+            auto phase_apply = [&](const bool row, const bool col, zd *const ax, zd *const ay, zd x, zd y,
+                                   const zd tr, const zd vr, const double t2r, const double sg) {
                 if (DENSE || row || col) {
-                    const unsigned m = row ? 0x80000000u : 0u;
-                    const zd c1 = mk(tr.re, flip_sign(tr.im, m));
-                    const zd vv = mk(vr.re, flip_sign(vr.im, m ^ 0x80000000u));
-                    pair_op(x, y, c1, t2r, vv);
-                    H[ax] = x;
-                    H[ay] = y;
+                    pair_op_s(x, y, tr.re, sg * tr.im, t2r, vr.re, -(sg * vr.im));
+                    *ax = x;
+                    *ay = y;
                 }
             };
+            // bulk operand addresses and sign of bulge A's last three steps:
+            // bulge B, three positions behind in the same sweep (H does not
+            // move within one), takes them from slot 0
+            bool rrow[3] = {false, false, false}, rcol[3] = {false, false, false};
+            zd *rax[3] = {pidle, pidle, pidle}, *ray[3] = {pidle, pidle, pidle};
+            double rsg[3] = {1.0, 1.0, 1.0};
             // one time step: bulge A at kA = l + tau, bulge B at kA - 3.
             // HA/HB: bulge A/B active; ADV bit 0: A advances, bit 1: B
             // advances (compile-time: the steady state has no per-half branch)
-            auto dstep = [&](auto HA, auto HB, auto ADV, auto PAR, const int tau) {
+            // RS: the ring slot of this step (tau mod 3, steady state
+            // only), or -1: B takes slot 0 and the ring shifts by one
+            auto dstep = [&](auto HA, auto HB, auto ADV, auto PAR, auto RS, const int tau) {
                 constexpr bool hasA = decltype(HA)::value, hasB = decltype(HB)::value;
                 constexpr int adv = decltype(ADV)::value;
                 constexpr int par = decltype(PAR)::value;   // tau & 1 (Z read-ahead register)
+                constexpr int rs = decltype(RS)::value;
+                constexpr int rb = rs < 0 ? 0 : rs, ra = rs < 0 ? 2 : rs;
                 const int kA = l + tau, kB = kA - 3;
                 // phase-1 operands (bulge B): independent of this step's chain
                 bool rowB = false, colB = false;
-                int axB = 0, ayB = 0;
+                zd *axB = pidle, *ayB = pidle;
+                double sgB = 1.0;
                 zd xB, yB;                    // used only where rowB || colB
                 if constexpr (hasB) {
-                    phase_addr(kB, lay.cb(kB), rowB, colB, axB, ayB);
+                    rowB = rrow[rb];          // phase_addr(kB, ...) of step tau - 3
+                    colB = rcol[rb];
+                    axB = rax[rb];
+                    ayB = ray[rb];
+                    sgB = rsg[rb];
                     if (DENSE || rowB || colB) {
-                        xB = H[axB];
-                        yB = H[ayB];
+                        xB = *axB;
+                        yB = *ayB;
                     }
                 }
                 // chain: reflector and window of this half's bulge; after a
                 // bulge's last step e = 0 and the row kk+2 update is a no-op
                 zd t1, v2;
                 double t2, beta;
-                bool ident;
-                zlarfg2(x1, x2, t1, v2, t2, beta, ident);
-                const zd tc = zconj(t1), vc = zconj(v2);
-                const zd xsub = ident ? x1 : mk(beta, 0.0);
-                pair_op(a, c, tc, t2, v2);    // left: column kk
-                pair_op(b, d, tc, t2, v2);    // left: column kk+1
-                pair_op(a, b, t1, t2, vc);    // right: row kk
-                pair_op(c, d, t1, t2, vc);    // right: row kk+1
+                zlarfg2c(x1, x2, t1, v2, t2, beta);
+                const zd xsub = mk(beta, 0.0);
+                pair_op_s(a, c, t1.re, -t1.im, t2, v2.re, v2.im);   // left: column kk
+                pair_op_s(b, d, t1.re, -t1.im, t2, v2.re, v2.im);   // left: column kk+1
+                pair_op_s(a, b, t1.re, t1.im, t2, v2.re, -v2.im);   // right: row kk
+                pair_op_s(c, d, t1.re, t1.im, t2, v2.re, -v2.im);   // right: row kk+1
                 const double sr = t2 * e.re;  // right: row kk+2 (H(kk+2,kk) = 0, e real)
                 const double f = -sr;
-                e.re = fma(-sr, vc.re, e.re);
-                e.im = -sr * vc.im;
+                e.re = fma(-sr, v2.re, e.re);
+                e.im = sr * v2.im;
                 // window stores: H(kk,kk-1) at cbk-1, H(kk,kk) at cbk+kk,
                 // H(kk,kk+1) at cbk+2kk+2
                 if (leader && (hasA && hasB ? true : (hasA ? half == 0 : half == 1))) {
@@ -1476,7 +1527,7 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                     group_halves<LPM>(v2.im, vA.im, vB.im);
                     group_halves<LPM>(t2, t2A, t2B);
                 }
-                if constexpr (hasB) phase_apply(rowB, colB, axB, ayB, xB, yB, tB, vB, t2B);
+                if constexpr (hasB) phase_apply(rowB, colB, axB, ayB, xB, yB, tB, vB, t2B, sgB);
                 if constexpr (WANTT && hasB) {   // Z(p, kB), Z(p, kB+1): column kB is final
                     if (zp) {
                         zd x = zcB, y = zyB;
@@ -1486,16 +1537,32 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                     }
                 }
                 lds_order();
+                if constexpr (rs < 0) {
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        rrow[s] = rrow[s + 1];
+                        rcol[s] = rcol[s + 1];
+                        rax[s] = rax[s + 1];
+                        ray[s] = ray[s + 1];
+                        rsg[s] = rsg[s + 1];
+                    }
+                }
                 if constexpr (hasA) {         // phase 2 reads after phase 1's writes
                     bool rowA, colA;
-                    int axA, ayA;
-                    phase_addr(kA, lay.cb(kA), rowA, colA, axA, ayA);
+                    zd *axA, *ayA;
+                    phase_addr(kA, rowA, colA, axA, ayA);
+                    const double sgA = rowA ? -1.0 : 1.0;
+                    rrow[ra] = rowA;
+                    rcol[ra] = colA;
+                    rax[ra] = axA;
+                    ray[ra] = ayA;
+                    rsg[ra] = sgA;
                     zd xA, yA;                // used only where rowA || colA
                     if (DENSE || rowA || colA) {
-                        xA = H[axA];
-                        yA = H[ayA];
+                        xA = *axA;
+                        yA = *ayA;
                     }
-                    phase_apply(rowA, colA, axA, ayA, xA, yA, tA, vA, t2A);
+                    phase_apply(rowA, colA, axA, ayA, xA, yA, tA, vA, t2A, sgA);
                     if constexpr (WANTT) {       // Z(p, kA), Z(p, kA+1): column kA goes to B
                         zd &zy = par ? zyO : zyE;
                         zd x = zcA, y = zy;
@@ -1582,13 +1649,14 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
             using A3 = std::integral_constant<int, 3>;
             using A2 = std::integral_constant<int, 2>;
             // bulge A alone for three steps, then B starts three positions behind
+            using RSH = std::integral_constant<int, -1>;
             if (half == 0) start();
             if constexpr (WANTT) {            // the Z read-ahead register follows tau's parity
-                dstep(T_(), F_(), A1(), P0(), 0);
-                dstep(T_(), F_(), A1(), P1(), 1);
-                dstep(T_(), F_(), A1(), P0(), 2);
+                dstep(T_(), F_(), A1(), P0(), RSH(), 0);
+                dstep(T_(), F_(), A1(), P1(), RSH(), 1);
+                dstep(T_(), F_(), A1(), P0(), RSH(), 2);
             } else {
-                for (int tau = 0; tau < 3; ++tau) dstep(T_(), F_(), A1(), P0(), tau);
+                for (int tau = 0; tau < 3; ++tau) dstep(T_(), F_(), A1(), P0(), RSH(), tau);
             }
             if (half == 1) start();
             if constexpr (WANTT) {            // B's columns l, l+1: A's first two
@@ -1596,20 +1664,32 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                 zyB = zA1;
                 int tau = 3;
                 for (; tau + 1 < nsteps - 1; tau += 2) {
-                    dstep(T_(), T_(), A3(), P1(), tau);
-                    dstep(T_(), T_(), A3(), P0(), tau + 1);
+                    dstep(T_(), T_(), A3(), P1(), RSH(), tau);
+                    dstep(T_(), T_(), A3(), P0(), RSH(), tau + 1);
                 }
-                if (tau < nsteps - 1) dstep(T_(), T_(), A3(), P1(), tau);
-                if ((nsteps - 1) & 1) dstep(T_(), T_(), A2(), P1(), nsteps - 1);   // A's last step
-                else dstep(T_(), T_(), A2(), P0(), nsteps - 1);
+                if (tau < nsteps - 1) dstep(T_(), T_(), A3(), P1(), RSH(), tau);
+                if ((nsteps - 1) & 1) dstep(T_(), T_(), A2(), P1(), RSH(), nsteps - 1);   // A's last step
+                else dstep(T_(), T_(), A2(), P0(), RSH(), nsteps - 1);
             } else {
-                for (int tau = 3; tau < nsteps - 1; ++tau) dstep(T_(), T_(), A3(), P0(), tau);
-                dstep(T_(), T_(), A2(), P0(), nsteps - 1);   // A's last step
+                // steady state by threes, ring slots fixed at compile time (no
+                // register moves); after each three the ring is in shift order
+                // again (slot 0 the oldest)
+                using RS0 = std::integral_constant<int, 0>;
+                using RS1 = std::integral_constant<int, 1>;
+                using RS2 = std::integral_constant<int, 2>;
+                int tau = 3;
+                for (; tau + 3 < nsteps; tau += 3) {
+                    dstep(T_(), T_(), A3(), P0(), RS0(), tau);
+                    dstep(T_(), T_(), A3(), P0(), RS1(), tau + 1);
+                    dstep(T_(), T_(), A3(), P0(), RS2(), tau + 2);
+                }
+                for (; tau < nsteps - 1; ++tau) dstep(T_(), T_(), A3(), P0(), RSH(), tau);
+                dstep(T_(), T_(), A2(), P0(), RSH(), nsteps - 1);   // A's last step
             }
             finish(0);
-            dstep(F_(), T_(), A2(), P0(), nsteps);
-            dstep(F_(), T_(), A2(), P0(), nsteps + 1);
-            dstep(F_(), T_(), std::integral_constant<int, 0>(), P0(), nsteps + 2);   // B's last step
+            dstep(F_(), T_(), A2(), P0(), RSH(), nsteps);
+            dstep(F_(), T_(), A2(), P0(), RSH(), nsteps + 1);
+            dstep(F_(), T_(), std::integral_constant<int, 0>(), P0(), RSH(), nsteps + 2);   // B's last step
             finish(1);
         }
         if (!conv) return i + 1;
