@@ -1383,12 +1383,13 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
             const int nsteps = i - l;          // steps of one bulge: k = l .. i-1 (>= DUAL_MIN - 1)
             const bool pin = p >= l && p <= i;
             // per-half chain state: window, reflector input, and the bulge's
-            // position kk with cbk = cb(kk).  A half without a bulge keeps
+            // position kk with pw = &H(kk, kk).  A half without a bulge keeps
             // x1 = 1 (inside the fast reflector's range); its results are
             // never stored.
             zd x1 = mk(1.0, 0.0), a = mk(0.0, 0.0), b = a, c = a, d = a, e = a;
             double x2 = 0.0;
-            int kk = l, cbk = lay.cb(l);
+            int kk = l;
+            zd *pw = H + lay.cb(l) + l;
             // Z state of this lane's row (WANTT with Z): A's carried column
             // kA; the column A's step tau needs next, read two steps ahead
             // into zyE (even tau) or zyO (odd tau) so that no register move
@@ -1404,13 +1405,12 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
             }
             auto start = [&]() {
                 kk = l;
-                cbk = lay.cb(l);
-                const int c1 = cbk + l + 2;   // cb(l + 1)
-                a = H[cbk + l];
-                c = H[cbk + l + 1];
-                b = H[c1 + l];
-                d = H[c1 + l + 1];
-                e = H[c1 + l + 2];            // l + 2 <= i: the block has >= DUAL_MIN rows
+                pw = H + lay.cb(l) + l;
+                a = pw[0];
+                c = pw[1];
+                b = pw[l + 2];                // column l + 1 starts at cb(l) + l + 2
+                d = pw[l + 3];
+                e = pw[l + 4];                // l + 2 <= i: the block has >= DUAL_MIN rows
                 const zd h11s = zsub(a, sig);
                 const double h21 = c.re;
                 const double sc = cabs1(h11s) + fabs(h21);
@@ -1426,8 +1426,23 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
             // operand of each form is the first one of the next step.
             zd *pnr = H + cbp + l, *pnc = H + lay.cb(l) + p;
             zd *const pidle = H + idle_slot;   // DENSE: the idle lanes' element
+            // LPM = 64: t is the lane id, and a step's lane sets are bit masks
+            // formed on the scalar unit
+            const int wb = i - l + 1;
+            const uint64_t pinm = (wb >= 64 ? ~0ull : (1ull << wb) - 1ull) << l;   // lanes l..i
+            const uint64_t nm = n >= 64 ? ~0ull : (1ull << n) - 1ull;              // lanes 0..n-1
             auto phase_addr = [&](const int k, bool &row, bool &col, zd *&ax, zd *&ay) {
-                if constexpr (WANTT) {        // columns k+2..n-1, rows 0..k-1
+                if constexpr (LPM == 64) {    // k <= i - 1 <= 62
+                    const uint64_t ge = k + 2 < 64 ? ~0ull << (k + 2) : 0ull;   // lanes >= k+2
+                    const uint64_t le = (1ull << k) - 1ull;                     // lanes <= k-1
+                    if constexpr (WANTT) {    // columns k+2..n-1, rows 0..k-1
+                        row = __builtin_amdgcn_inverse_ballot_w64(ge & nm);
+                        col = __builtin_amdgcn_inverse_ballot_w64(le);
+                    } else {
+                        row = __builtin_amdgcn_inverse_ballot_w64(ge & pinm);
+                        col = __builtin_amdgcn_inverse_ballot_w64(le & pinm);
+                    }
+                } else if constexpr (WANTT) {
                     row = p >= k + 2 && p < n;
                     col = p <= k - 1;
                 } else {
@@ -1502,12 +1517,12 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                 const double f = -sr;
                 e.re = fma(-sr, v2.re, e.re);
                 e.im = sr * v2.im;
-                // window stores: H(kk,kk-1) at cbk-1, H(kk,kk) at cbk+kk,
-                // H(kk,kk+1) at cbk+2kk+2
+                // window stores: H(kk,kk-1) at pw - kk - 1, H(kk,kk) at pw,
+                // H(kk,kk+1) at pw + kk + 2
                 if (leader && (hasA && hasB ? true : (hasA ? half == 0 : half == 1))) {
-                    if (kk > l) H[cbk - 1] = xsub;
-                    H[cbk + kk] = a;
-                    H[cbk + 2 * kk + 2] = b;
+                    if (kk > l) pw[-(kk + 1)] = xsub;
+                    pw[0] = a;
+                    pw[kk + 2] = b;
                 }
                 // both reflectors to every lane of the group (one matrix per
                 // wavefront: wave-uniform values read from lanes 0 and 32)
@@ -1578,18 +1593,19 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                 }
                 lds_order();
                 // advance to kk+1: b, d, e are rows kk+1 .. kk+3 of column kk+2,
-                // which starts at cb(kk+2) = cbk + 2kk + 5
+                // which starts at cb(kk+2) = cb(kk) + 2kk + 5; H(kk+1, kk+1) is
+                // at cb(kk+1) + kk + 1 = pw + kk + 3
                 if constexpr (adv != 0) {
                     if (adv == 3 || (adv == 1 ? half == 0 : half == 1)) {
-                        const int base = cbk + 3 * kk + 6;
+                        const zd *const base = pw + (2 * kk + 6);
                         x1 = c;
                         x2 = f;
                         a = d;
                         c = e;
-                        b = H[base];
-                        d = H[base + 1];
-                        e = (kk + 3 <= i) ? H[base + 2] : mk(0.0, 0.0);
-                        cbk += kk + 2;
+                        b = base[0];
+                        d = base[1];
+                        e = (kk + 3 <= i) ? base[2] : mk(0.0, 0.0);
+                        pw += kk + 3;
                         kk += 1;
                     }
                 }
