@@ -39,15 +39,14 @@ PEAK_HBM = 8.0e12
 # over one 20,000-matrix call (FETCH_SIZE / WRITE_SIZE each scaled by its
 # reading for a 1 GiB device copy: FETCH x2.0, WRITE x1.0; SQ_INSTS_VALU_
 # {FMA,MUL,ADD,TRANS}_F64 x 64 lanes per wave instruction, an FMA counted as 2):
-#  - chunk variants 37-39 (k_qr_chunk<8>): 34,570 B read + 1,072 B written;
-#    64 * (2 * 310,995 + 140,690 + 69,862 + 13,011) FP64 lane operations;
-#  - pool variants 29-36 (k_qr_pool<12>) and the fixed variant 20, the same
-#    two-bulge arithmetic: 34,563 B read + 1,086 B written (variant 20 moved
-#    35,480 B); 64 * (2 * 311,001 + 140,622 + 69,864 + 13,011).
+#  - chunk variants 37-39 (k_qr_chunk<8>): 34,573 B read + 1,071 B written;
+#    64 * (2 * 301,516 + 153,109 + 69,861 + 13,011) FP64 lane operations.
+# The pool variants 29-36 and the fixed variant 20 run the same two-bulge
+# step and move the same bytes (34,563 B read + 1,086 B written); their FP64
+# counts were taken before the step's current form, so they report none.
 # The FP64 count includes the serial reflector chain that every lane computes
 # redundantly, ~7.7x the 80/3 n^3 model.
-QR64_COUNTS = {v: (35_642.0, 5.4115e7) for v in (37, 38, 39)}
-QR64_COUNTS.update({v: (35_648.0, 5.418e7) for v in (20,) + tuple(range(29, 37))})
+QR64_COUNTS = {v: (35_644.0, 5.3697e7) for v in (37, 38, 39)}
 
 
 def flops_per_solve(n: int) -> float:
