@@ -184,14 +184,18 @@ __device__ __forceinline__ void zlarfg2(zd x1, double x2, zd &t1, zd &v2, double
     if (ident) { t1 = mk(0.0, 0.0); v2 = mk(x2, 0.0); t2 = 0.0; beta = x1.re; }
 }
 
-// zlarfg2 for the two-bulge chain: the same arithmetic, with ZLARFG's tau = 0
+// zlarfg2 for the bulge chase (one or two bulges; the m > l restart of a
+// single-bulge sweep recomputes its first reflector with it too): the same
+// arithmetic, with ZLARFG's tau = 0
 // (H = I for x2 = Im x1 = 0) kept off the fast range only, where it also
 // covers x = 0.  In the fast range such an x gets beta = -x1, tau = 2 and
 // v2 = 0: the reflection diag(-1, 1), an exact sign change of row and column
-// k.  Every later reflector, window, shift and deflation test of the QR then
-// sees the same values up to signs (negation is exact, rounding symmetric),
-// so the eigenvalues keep their bits, and the hot path loses the select of
-// the tau = 0 case.  H(k, k-1) = beta + 0i in both cases.
+// k, a unitary similarity like the identity, and the hot path loses the
+// select of the tau = 0 case.  Later steps see the same values up to signs;
+// ZLAHQR's shift takes sqrt(H(i-1,i)) sqrt(H(i,i-1)), and the complex square
+// root of a negated number rounds differently, so such a matrix can get
+// eigenvalues that differ from the tau = 0 choice in the last bits (about
+// one N(0,1) matrix in 65,536 at n = 64).  H(k, k-1) = beta + 0i in both.
 __device__ __forceinline__ void zlarfg2c(zd x1, double x2, zd &t1, zd &v2, double &t2, double &beta)
 {
     const double ss = fma(x1.re, x1.re, fma(x1.im, x1.im, x2 * x2));
@@ -967,8 +971,7 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                 constexpr int par = decltype(PAR)::value;   // (kk - m) & 1: the Z read-ahead register
                 zd t1, v2;
                 double t2, beta;
-                bool ident;
-                zlarfg2(x1, x2, t1, v2, t2, beta, ident);
+                zlarfg2c(x1, x2, t1, v2, t2, beta);
                 const int cbk = lay.cb(kk);
                 // bulk: left on columns kk+2 <= p <= I2, right on rows I1 <= p <= kk-1
 #pragma unroll
@@ -977,10 +980,10 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                     const bool rowop = (p >= kk + 2 && p <= I2);
                     const bool colop = (p >= I1 && p <= kk - 1) || zact;
                     if (rowop || colop) {
-                        const zd c1 = mk(t1.re, rowop ? -t1.im : t1.im);
-                        const zd vv = mk(v2.re, rowop ? v2.im : -v2.im);
+                        // -1 from the left (conj(tau), v2), +1 from the right (tau, conj(v2))
+                        const double sg = rowop ? -1.0 : 1.0;
                         zd x = carry[s], y = ypre[s];
-                        pair_op(x, y, c1, t2, vv);
+                        pair_op_s(x, y, t1.re, sg * t1.im, t2, v2.re, -(sg * v2.im));
                         if (rowop) {
                             H[cbp[s] + kk] = x;
                             H[cbp[s] + kk + 1] = y;
@@ -995,7 +998,7 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                         if (wantz && p < n) {
                             zd &zy = par ? zyO[s] : zyE[s];
                             zd x = zc[s], y = zy;
-                            pair_op(x, y, t1, t2, zconj(v2));
+                            pair_op_s(x, y, t1.re, t1.im, t2, v2.re, -v2.im);
                             Z[zlay(p, kk)] = x;
                             zc[s] = y;
                             if (kk + 3 <= i) zy = Z[zlay(p, kk + 3)];   // for step kk + 2
@@ -1003,18 +1006,17 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                     }
                 }
                 // window: rows/cols kk, kk+1 and row kk+2
-                if (!first && t == 0) H[lay(kk, kk - 1)] = ident ? x1 : mk(beta, 0.0);
-                const zd tc = zconj(t1), vc = zconj(v2);
-                pair_op(a, c, tc, t2, v2);    // left: column kk
-                pair_op(b, d, tc, t2, v2);    // left: column kk+1
-                pair_op(a, b, t1, t2, vc);    // right: row kk
-                pair_op(c, d, t1, t2, vc);    // right: row kk+1
+                if (!first && t == 0) H[lay(kk, kk - 1)] = mk(beta, 0.0);
+                pair_op_s(a, c, t1.re, -t1.im, t2, v2.re, v2.im);   // left: column kk
+                pair_op_s(b, d, t1.re, -t1.im, t2, v2.re, v2.im);   // left: column kk+1
+                pair_op_s(a, b, t1.re, t1.im, t2, v2.re, -v2.im);   // right: row kk
+                pair_op_s(c, d, t1.re, t1.im, t2, v2.re, -v2.im);   // right: row kk+1
                 double f = 0.0;
                 if (has_e) {                  // right: row kk+2, H(kk+2,kk) = 0 before, e real
                     const double sr = t2 * e.re;
                     f = -sr;
-                    e.re = fma(-sr, vc.re, e.re);
-                    e.im = -sr * vc.im;
+                    e.re = fma(-sr, v2.re, e.re);
+                    e.im = sr * v2.im;
                 }
                 if (t == 0) H[cbk + kk] = a;
 #pragma unroll
@@ -1076,8 +1078,7 @@ __device__ int zlahqr_core(zd *H, const Lay lay, int n, int t, int itmax_overrid
                 lds_order();
                 zd t1, v2;
                 double t2r, beta;
-                bool ident;
-                zlarfg2(x1, x2, t1, v2, t2r, beta, ident);   // the reflector of step m again
+                zlarfg2c(x1, x2, t1, v2, t2r, beta);   // the reflector of step m again (as the step)
                 zd temp = zsub(mk(1.0, 0.0), t1);
                 temp = zscale(1.0 / zabs(temp), temp);
                 const zd tempc = zconj(temp);
@@ -1546,7 +1547,7 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                 if constexpr (WANTT && hasB) {   // Z(p, kB), Z(p, kB+1): column kB is final
                     if (zp) {
                         zd x = zcB, y = zyB;
-                        pair_op(x, y, tB, t2B, zconj(vB));
+                        pair_op_s(x, y, tB.re, tB.im, t2B, vB.re, -vB.im);
                         zr[kB * zld] = x;
                         zcB = y;
                     }
@@ -1581,7 +1582,7 @@ __device__ __forceinline__ int zlahqr_dual(zd *H, const PackedLayout lay, int n,
                     if constexpr (WANTT) {       // Z(p, kA), Z(p, kA+1): column kA goes to B
                         zd &zy = par ? zyO : zyE;
                         zd x = zcA, y = zy;
-                        pair_op(x, y, tA, t2A, zconj(vA));
+                        pair_op_s(x, y, tA.re, tA.im, t2A, vA.re, -vA.im);
                         if constexpr (!hasB) {
                             if (tau == 0) zF = x;
                         }
